@@ -1,0 +1,194 @@
+// capi_misc.cpp — error reporting, device queries and raw device-memory helpers of the C ABI.
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <new>
+#include <stdexcept>
+
+#include "common.h"
+
+namespace gary {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void clear_error() { g_err[0] = '\0'; }
+
+const char *last_error() { return g_err; }
+
+}  // namespace gary
+
+namespace {
+int use_device(int device) {
+  int cur = -1;
+  GARY_HIP(hipGetDevice(&cur));
+  if (cur != device) GARY_HIP(hipSetDevice(device));
+  return GARY_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char *gary_last_error(void) { return gary::last_error(); }
+
+int gary_version(void) { return 100; }
+
+int gary_device_count(int *count) {
+  GARY_CHECK_ARG(count, "gary_device_count: null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    c = 0;
+  }
+  *count = c;
+  return GARY_OK;
+}
+
+int gary_device_synchronize(int device) {
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipDeviceSynchronize());
+  return GARY_OK;
+}
+
+int gary_malloc(int device, size_t bytes, void **ptr) {
+  GARY_CHECK_ARG(ptr, "gary_malloc: null out pointer");
+  GARY_TRY(use_device(device));
+  *ptr = nullptr;
+  if (bytes == 0) return GARY_OK;
+  GARY_HIP(hipMalloc(ptr, bytes));
+  return GARY_OK;
+}
+
+int gary_free(int device, void *ptr) {
+  if (!ptr) return GARY_OK;
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipFree(ptr));
+  return GARY_OK;
+}
+
+int gary_memcpy_h2d(int device, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return GARY_OK;
+  GARY_CHECK_ARG(dst && src, "gary_memcpy_h2d: null pointer");
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GARY_OK;
+}
+
+int gary_memcpy_d2h(int device, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return GARY_OK;
+  GARY_CHECK_ARG(dst && src, "gary_memcpy_d2h: null pointer");
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GARY_OK;
+}
+
+int gary_memset(int device, void *dst, int value, size_t bytes) {
+  if (bytes == 0) return GARY_OK;
+  GARY_CHECK_ARG(dst, "gary_memset: null pointer");
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipMemset(dst, value, bytes));
+  return GARY_OK;
+}
+
+}  // extern "C"
+
+// ---- workspace cache ------------------------------------------------------------------------
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "workspace.h"
+
+namespace gary {
+
+namespace {
+struct Buf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<std::pair<int, void *>, Buf> *g_ws = new std::map<std::pair<int, void *>, Buf>();
+}  // namespace
+
+int workspace_get(int device, void *stream, size_t bytes, void **ptr) {
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  Buf &b = (*g_ws)[{device, stream}];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      // the previous buffer may still be in use by work queued on this stream
+      GARY_HIP(hipStreamSynchronize((hipStream_t)stream));
+      GARY_HIP(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    GARY_HIP(hipMalloc(&b.ptr, want));
+    b.bytes = want;
+  }
+  *ptr = b.ptr;
+  return GARY_OK;
+}
+
+int current_device(int *device) {
+  GARY_HIP(hipGetDevice(device));
+  return GARY_OK;
+}
+
+}  // namespace gary
+
+// ---- event timer ------------------------------------------------------------------------------
+struct gary_timer {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
+extern "C" {
+
+int gary_timer_create(gary_timer **out) {
+  GARY_CHECK_ARG(out, "gary_timer_create: null out");
+  auto *t = new (std::nothrow) gary_timer();
+  if (!t) return gary::set_error(GARY_ERR_OOM, "gary_timer_create: out of memory");
+  hipError_t e1 = hipEventCreate(&t->start), e2 = hipEventCreate(&t->stop);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    if (t->start) (void)hipEventDestroy(t->start);
+    if (t->stop) (void)hipEventDestroy(t->stop);
+    delete t;
+    (void)hipGetLastError();
+    return gary::set_error(GARY_ERR_HIP, "gary_timer_create: hipEventCreate failed");
+  }
+  *out = t;
+  return GARY_OK;
+}
+
+int gary_timer_destroy(gary_timer *t) {
+  if (!t) return GARY_OK;
+  (void)hipEventDestroy(t->start);
+  (void)hipEventDestroy(t->stop);
+  delete t;
+  return GARY_OK;
+}
+
+int gary_timer_start(gary_timer *t, void *stream) {
+  GARY_CHECK_ARG(t, "gary_timer_start: null timer");
+  GARY_HIP(hipEventRecord(t->start, (hipStream_t)stream));
+  return GARY_OK;
+}
+
+int gary_timer_stop(gary_timer *t, void *stream, float *ms) {
+  GARY_CHECK_ARG(t && ms, "gary_timer_stop: null pointer");
+  GARY_HIP(hipEventRecord(t->stop, (hipStream_t)stream));
+  GARY_HIP(hipEventSynchronize(t->stop));
+  GARY_HIP(hipEventElapsedTime(ms, t->start, t->stop));
+  return GARY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// device_search.h — sorted-list search primitives for one lane (gfx950).
+//
+// Restates include/ops/Search.cuh for wave64 kernels:
+//   bsearch            <- binary_search(Indexable, key)           Search.cuh:48-61
+//   bsearch_pos        <- binary_search(Indexable, key, int &pos) Search.cuh:67-81
+//   build_cache_group  <- build_cache(list, cache, group)         Search.cuh:103-109
+//   bsearch_2phase     <- binary_search_2phase(list, cache, key)  Search.cuh:120-138
+// The cache depth N is a runtime power of two here (the reference fixes N=32 at compile time,
+// test/TriQueryTest.cu:42). Phase 2 can read its slice as one contiguous segment when the slice
+// is short (`bsearch_2phase_seg`), which turns log2(size/N) dependent loads into one 128-byte
+// read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gary {
+
+__device__ __forceinline__ bool bsearch(const int32_t *__restrict__ list, int32_t size, int32_t key) {
+  int32_t l = 0, r = size - 1;
+  while (r >= l) {
+    int32_t mid = l + ((r - l) >> 1);
+    int32_t v = list[mid];
+    if (v == key) return true;
+    if (v < key) l = mid + 1;
+    else r = mid - 1;
+  }
+  return false;
+}
+
+// On a miss `pos` = index of the last element < key (-1 if key < list[0]).
+__device__ __forceinline__ bool bsearch_pos(const int32_t *__restrict__ list, int32_t size,
+                                            int32_t key, int32_t &pos) {
+  int32_t l = 0, r = size - 1;
+  while (r >= l) {
+    int32_t p = l + ((r - l) >> 1);
+    int32_t v = list[p];
+    if (v == key) {
+      pos = p;
+      return true;
+    }
+    if (v < key) l = p + 1;
+    else r = p - 1;
+  }
+  pos = r;
+  return false;
+}
+
+// Branch-light lower bound over an LDS array of power-of-two length n (returns the number of
+// entries < key, i.e. the insertion index).
+__device__ __forceinline__ int32_t lower_bound_pow2(const int32_t *a, int32_t n, int32_t key) {
+  int32_t base = 0;
+  for (int32_t half = n >> 1; half > 0; half >>= 1)
+    if (a[base + half - 1] < key) base += half;
+  return base + (a[base] < key);
+}
+
+// cache[i] = list[i*size/N] for i in [0, N), lanes of a group of `gsize` threads cooperating.
+__device__ __forceinline__ void build_cache_group(const int32_t *__restrict__ list, int64_t size,
+                                                  int32_t *cache, int32_t n_cache, int glane,
+                                                  int gsize) {
+  for (int32_t i = glane; i < n_cache; i += gsize) cache[i] = list[(int64_t)i * size / n_cache];
+}
+
+// Phase 1 in the LDS cache, phase 2 in list[pos*size/N, (pos+1)*size/N) (Search.cuh:120-138).
+__device__ __forceinline__ bool bsearch_2phase(const int32_t *__restrict__ list, int32_t size,
+                                               const int32_t *cache, int32_t n_cache,
+                                               int32_t key) {
+  int32_t pos = 0;
+  if (bsearch_pos(cache, n_cache, key, pos)) return true;
+  if (pos < 0) return false;
+  int32_t bottom = (int32_t)((int64_t)pos * size / n_cache);
+  int32_t top = (int32_t)((int64_t)(pos + 1) * size / n_cache);
+  return bsearch(list + bottom, top - bottom, key);
+}
+
+}  // namespace gary

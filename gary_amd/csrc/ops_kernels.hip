@@ -1,0 +1,420 @@
+// ops_kernels.hip — search and intersection operators as standalone device operations.
+//
+// Reference: include/ops/Search.cuh (binary_search, build_cache, binary_search_2phase) as driven
+// by the test kernels of test/SearchTest.cu:14-183, and include/ops/Intersection.cuh
+// (intersectBinarySearch count / emit variants, :18-135, 239-300) as driven by
+// test/SetOpTest.cu:15-43. The reference's kernels run on one warp or a handful of blocks; these
+// spread a single large operation over the whole GPU (grid-stride over keys, one LDS index
+// cache per wave) and keep the reference's per-key semantics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.h"
+#include "device_search.h"
+#include "device_util.h"
+#include "workspace.h"
+
+namespace gary {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// ---- search ------------------------------------------------------------------------------
+
+// binarySearchKernel / binarySearch2PhaseKernel / the count variants: every wave builds its
+// own index cache of the whole list (Search.cuh:103-109), then threads grid-stride the keys.
+template <bool TWO_PHASE, bool COUNT>
+__global__ void __launch_bounds__(kBlock)
+    search_kernel(const int32_t *__restrict__ list, int32_t len, const int32_t *__restrict__ keys,
+                  int64_t nkeys, uint8_t *__restrict__ results, int32_t n_cache,
+                  unsigned long long *count) {
+  extern __shared__ int32_t smem[];
+  int32_t *cache = smem + (threadIdx.x / kWave) * n_cache;
+  if constexpr (TWO_PHASE) {
+    build_cache_group(list, len, cache, n_cache, lane_id(), kWave);
+    wave_sync();
+  }
+  uint32_t c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nkeys;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int32_t key = keys[i];
+    const bool f = TWO_PHASE ? bsearch_2phase(list, len, cache, n_cache, key)
+                             : bsearch(list, len, key);
+    if constexpr (COUNT) c += f;
+    else results[i] = f;
+  }
+  if constexpr (COUNT) wave_atomic_add(count, c);
+}
+
+// batchedBinarySearch{,2Phase}Kernel: one wave per item, INTERLEAVED over items.
+template <bool TWO_PHASE>
+__global__ void __launch_bounds__(kBlock)
+    search_batched_kernel(int32_t batch, const int32_t *const *__restrict__ lists,
+                          const int64_t *__restrict__ list_sizes,
+                          const int32_t *const *__restrict__ keys,
+                          const int64_t *__restrict__ key_sizes,
+                          uint8_t *const *__restrict__ results, int32_t n_cache) {
+  extern __shared__ int32_t smem[];
+  int32_t *cache = smem + (threadIdx.x / kWave) * n_cache;
+  const int lane = lane_id();
+  for (int64_t it = global_wave_id(); it < batch; it += num_global_waves()) {
+    const int32_t *list = lists[it];
+    const int32_t len = (int32_t)list_sizes[it];
+    const int32_t *k = keys[it];
+    const int64_t nk = key_sizes[it];
+    uint8_t *res = results[it];
+    if (len <= 0) {
+      for (int64_t i = lane; i < nk; i += kWave) res[i] = 0;
+      continue;
+    }
+    if constexpr (TWO_PHASE) {
+      wave_sync();
+      build_cache_group(list, len, cache, n_cache, lane, kWave);
+      wave_sync();
+    }
+    for (int64_t i = lane; i < nk; i += kWave)
+      res[i] = TWO_PHASE ? bsearch_2phase(list, len, cache, n_cache, k[i]) : bsearch(list, len, k[i]);
+  }
+}
+
+// ---- intersection --------------------------------------------------------------------------
+
+enum { kIsBsearch = GARY_ISECT_BSEARCH, kIsCached = GARY_ISECT_CACHED_BSEARCH,
+       kIsMerge = GARY_ISECT_MERGE, kIsBitmap = GARY_ISECT_BITMAP };
+
+// Membership flags of the keys of A in B (A is the shorter list). For EMIT, flags[i] = 1 when
+// A[i] is in B and each block writes its found-count to block_counts[blockIdx]; otherwise the
+// count is accumulated into *count.
+template <int METHOD, bool EMIT>
+__global__ void __launch_bounds__(kBlock)
+    isect_probe_kernel(const int32_t *__restrict__ A, int32_t na, const int32_t *__restrict__ B,
+                       int32_t nb, int32_t n_cache, uint8_t *__restrict__ flags,
+                       int32_t *__restrict__ block_counts, unsigned long long *count) {
+  extern __shared__ int32_t smem[];
+  __shared__ uint32_t s_wave[kWavesPerBlock];
+  int32_t *cache = smem + (threadIdx.x / kWave) * n_cache;
+  if constexpr (METHOD == kIsCached) {
+    build_cache_group(B, nb, cache, n_cache, lane_id(), kWave);
+    wave_sync();
+  }
+  uint32_t c = 0;
+  // EMIT: contiguous per-block chunks so that block order = key order for the compaction
+  const int64_t per = EMIT ? ((int64_t)na + gridDim.x - 1) / gridDim.x : 0;
+  const int64_t beg = EMIT ? (int64_t)blockIdx.x * per : (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t end = EMIT ? min((int64_t)na, beg + per) : (int64_t)na;
+  const int64_t step = EMIT ? kBlock : (int64_t)gridDim.x * kBlock;
+  for (int64_t i = EMIT ? beg + threadIdx.x : beg; i < end; i += step) {
+    const int32_t key = A[i];
+    bool f;
+    if constexpr (METHOD == kIsCached) f = bsearch_2phase(B, nb, cache, n_cache, key);
+    else f = bsearch(B, nb, key);
+    c += f;
+    if constexpr (EMIT) flags[i] = f;
+  }
+  if constexpr (EMIT) {
+    const uint32_t s = wave_sum_u32(c);
+    if (lane_id() == 0) s_wave[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < kWavesPerBlock; ++w) t += s_wave[w];
+      block_counts[blockIdx.x] = (int32_t)t;
+    }
+  } else {
+    wave_atomic_add(count, c);
+  }
+}
+
+// Merge-path count over the whole grid: thread t merges diagonal segment t of merge(A, B).
+__device__ __forceinline__ int32_t mp_split(const int32_t *__restrict__ A, int32_t na,
+                                            const int32_t *__restrict__ B, int32_t nb, int64_t k) {
+  int64_t lo = k > nb ? k - nb : 0, hi = k < na ? k : na;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (A[mid] <= B[k - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  return (int32_t)lo;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    isect_merge_kernel(const int32_t *__restrict__ A, int32_t na, const int32_t *__restrict__ B,
+                       int32_t nb, unsigned long long *count) {
+  const int64_t L = (int64_t)na + nb;
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t per = (L + T - 1) / T;
+  const int64_t k0 = min(t * per, L), k1 = min(k0 + per, L);
+  uint32_t c = 0;
+  if (k0 < k1) {
+    int32_t i = mp_split(A, na, B, nb, k0);
+    int32_t j = (int32_t)(k0 - i);
+    for (int64_t k = k0; k < k1; ++k) {
+      if (i < na && (j >= nb || A[i] <= B[j])) {
+        ++i;
+      } else {
+        c += (i > 0 && A[i - 1] == B[j]);
+        ++j;
+      }
+    }
+  }
+  wave_atomic_add(count, c);
+}
+
+// Bitmap count: each block owns windows of WORDS*32 consecutive ids; it marks the elements of B
+// inside its window in an LDS bitmap, then tests the elements of A inside the window.
+template <int WORDS>
+__global__ void __launch_bounds__(kBlock)
+    isect_bitmap_kernel(const int32_t *__restrict__ A, int32_t na, const int32_t *__restrict__ B,
+                        int32_t nb, unsigned long long *count) {
+  __shared__ uint32_t bits[WORDS];
+  __shared__ int32_t s_range[4];
+  const int64_t lo_id = min(A[0], B[0]);
+  const int64_t hi_id = max(A[na - 1], B[nb - 1]);
+  const int64_t span = hi_id - lo_id + 1;
+  const int64_t nwin = (span + (int64_t)WORDS * 32 - 1) / ((int64_t)WORDS * 32);
+  uint32_t c = 0;
+  for (int64_t w = blockIdx.x; w < nwin; w += gridDim.x) {
+    const int64_t w0 = lo_id + w * (int64_t)WORDS * 32;
+    const int64_t w1 = w0 + (int64_t)WORDS * 32;  // exclusive
+    for (int i = threadIdx.x; i < WORDS; i += kBlock) bits[i] = 0;
+    if (threadIdx.x == 0) {
+      // index ranges of B and A inside [w0, w1): lower bounds by binary search
+      int32_t l = 0, r = nb;
+      while (l < r) { int32_t md = (l + r) >> 1; if (B[md] < w0) l = md + 1; else r = md; }
+      s_range[0] = l;
+      r = nb;
+      while (l < r) { int32_t md = (l + r) >> 1; if (B[md] < w1) l = md + 1; else r = md; }
+      s_range[1] = l;
+      l = 0; r = na;
+      while (l < r) { int32_t md = (l + r) >> 1; if (A[md] < w0) l = md + 1; else r = md; }
+      s_range[2] = l;
+      r = na;
+      while (l < r) { int32_t md = (l + r) >> 1; if (A[md] < w1) l = md + 1; else r = md; }
+      s_range[3] = l;
+    }
+    __syncthreads();
+    for (int32_t i = s_range[0] + threadIdx.x; i < s_range[1]; i += kBlock) {
+      const uint32_t off = (uint32_t)(B[i] - w0);
+      atomicOr(&bits[off >> 5], 1u << (off & 31));
+    }
+    __syncthreads();
+    for (int32_t i = s_range[2] + threadIdx.x; i < s_range[3]; i += kBlock) {
+      const uint32_t off = (uint32_t)(A[i] - w0);
+      c += (bits[off >> 5] >> (off & 31)) & 1u;
+    }
+    __syncthreads();
+  }
+  wave_atomic_add(count, c);
+}
+
+// Exclusive scan of block counts in one workgroup (nblocks <= 4096) and the total.
+__global__ void __launch_bounds__(kBlock)
+    scan_counts_kernel(int32_t *__restrict__ counts, int32_t n, unsigned long long *total) {
+  __shared__ uint32_t s_w[kWavesPerBlock];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int32_t base = 0; base < n; base += kBlock) {
+    const int32_t i = base + threadIdx.x;
+    const uint32_t v = i < n ? (uint32_t)counts[i] : 0u;
+    uint32_t incl = wave_inclusive_scan_u32(v);
+    if (lane_id() == kWave - 1) s_w[threadIdx.x / kWave] = incl;
+    __syncthreads();
+    uint32_t off = s_carry;
+    for (int w = 0; w < (int)(threadIdx.x / kWave); ++w) off += s_w[w];
+    if (i < n) counts[i] = (int32_t)(off + incl - v);
+    __syncthreads();
+    if (threadIdx.x == kBlock - 1) s_carry = off + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = s_carry;
+}
+
+// Ordered compaction: block b writes its found keys starting at offsets[b], in key order.
+__global__ void __launch_bounds__(kBlock)
+    compact_kernel(const int32_t *__restrict__ A, int32_t na, const uint8_t *__restrict__ flags,
+                   const int32_t *__restrict__ offsets, int32_t *__restrict__ out) {
+  __shared__ uint32_t s_w[kWavesPerBlock];
+  const int64_t per = ((int64_t)na + gridDim.x - 1) / gridDim.x;
+  const int64_t beg = (int64_t)blockIdx.x * per, end = min((int64_t)na, beg + per);
+  uint32_t base = (uint32_t)offsets[blockIdx.x];
+  for (int64_t c0 = beg; c0 < end; c0 += kBlock) {
+    const int64_t i = c0 + threadIdx.x;
+    const bool f = i < end && flags[i];
+    const uint64_t mask = __ballot(f);
+    if (lane_id() == 0) s_w[threadIdx.x / kWave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t off = base;
+    for (int w = 0; w < (int)(threadIdx.x / kWave); ++w) off += s_w[w];
+    if (f) out[off + lanes_below(mask)] = A[i];
+    uint32_t tot = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) tot += s_w[w];
+    __syncthreads();
+    base += tot;
+  }
+}
+
+int check_search_args(int64_t len, int64_t nkeys, int mode, int cache_depth) {
+  GARY_CHECK_ARG(len >= 0 && len <= INT32_MAX && nkeys >= 0, "search: len=%lld nkeys=%lld",
+                 (long long)len, (long long)nkeys);
+  GARY_CHECK_ARG(mode == GARY_SEARCH_PLAIN || mode == GARY_SEARCH_2PHASE, "search: bad mode %d",
+                 mode);
+  if (mode == GARY_SEARCH_2PHASE)
+    GARY_CHECK_ARG(cache_depth >= 1 && cache_depth <= 1024 && (cache_depth & (cache_depth - 1)) == 0,
+                   "search: cache_depth=%d must be a power of two in [1, 1024]", cache_depth);
+  return GARY_OK;
+}
+
+int grid_for(int64_t items, int cap = 4096) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((items + kBlock - 1) / kBlock, cap));
+}
+
+}  // namespace
+}  // namespace gary
+
+extern "C" {
+
+int gary_search(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64_t nkeys,
+                uint8_t *d_results, int mode, int cache_depth, void *stream) {
+  using namespace gary;
+  GARY_TRY(check_search_args(len, nkeys, mode, cache_depth));
+  if (nkeys == 0) return GARY_OK;
+  GARY_CHECK_ARG(d_keys && d_results, "gary_search: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (len == 0) {
+    GARY_HIP(hipMemsetAsync(d_results, 0, (size_t)nkeys, st));
+    return GARY_OK;
+  }
+  GARY_CHECK_ARG(d_list, "gary_search: null list");
+  const int grid = grid_for(nkeys);
+  if (mode == GARY_SEARCH_2PHASE)
+    search_kernel<true, false><<<grid, kBlock, kWavesPerBlock * cache_depth * sizeof(int32_t), st>>>(
+        d_list, (int32_t)len, d_keys, nkeys, d_results, cache_depth, nullptr);
+  else
+    search_kernel<false, false><<<grid, kBlock, 0, st>>>(d_list, (int32_t)len, d_keys, nkeys,
+                                                         d_results, 0, nullptr);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
+int gary_search_count(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64_t nkeys,
+                      uint64_t *d_count, int mode, int cache_depth, void *stream) {
+  using namespace gary;
+  GARY_TRY(check_search_args(len, nkeys, mode, cache_depth));
+  GARY_CHECK_ARG(d_count, "gary_search_count: null counter");
+  if (nkeys == 0 || len == 0) return GARY_OK;
+  GARY_CHECK_ARG(d_list && d_keys, "gary_search_count: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const int grid = grid_for(nkeys);
+  auto *cnt = (unsigned long long *)d_count;
+  if (mode == GARY_SEARCH_2PHASE)
+    search_kernel<true, true><<<grid, kBlock, kWavesPerBlock * cache_depth * sizeof(int32_t), st>>>(
+        d_list, (int32_t)len, d_keys, nkeys, nullptr, cache_depth, cnt);
+  else
+    search_kernel<false, true><<<grid, kBlock, 0, st>>>(d_list, (int32_t)len, d_keys, nkeys,
+                                                        nullptr, 0, cnt);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
+int gary_search_batched(int32_t batch, const int32_t *const *d_lists, const int64_t *d_list_sizes,
+                        const int32_t *const *d_keys, const int64_t *d_key_sizes,
+                        uint8_t *const *d_results, int mode, int cache_depth, void *stream) {
+  using namespace gary;
+  GARY_TRY(check_search_args(0, 0, mode, cache_depth));
+  GARY_CHECK_ARG(batch >= 0, "gary_search_batched: batch=%d", batch);
+  if (batch == 0) return GARY_OK;
+  GARY_CHECK_ARG(d_lists && d_list_sizes && d_keys && d_key_sizes && d_results,
+                 "gary_search_batched: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const int grid = (int)std::min<int64_t>((batch + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
+  if (mode == GARY_SEARCH_2PHASE)
+    search_batched_kernel<true><<<grid, kBlock, kWavesPerBlock * cache_depth * sizeof(int32_t), st>>>(
+        batch, d_lists, d_list_sizes, d_keys, d_key_sizes, d_results, cache_depth);
+  else
+    search_batched_kernel<false><<<grid, kBlock, 0, st>>>(batch, d_lists, d_list_sizes, d_keys,
+                                                          d_key_sizes, d_results, 0);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
+int gary_intersect_count(const int32_t *d_a, int64_t na, const int32_t *d_b, int64_t nb,
+                         int method, int cache_depth, uint64_t *d_count, void *stream) {
+  using namespace gary;
+  GARY_CHECK_ARG(d_count, "gary_intersect_count: null counter");
+  GARY_CHECK_ARG(na >= 0 && nb >= 0 && na <= INT32_MAX && nb <= INT32_MAX,
+                 "gary_intersect_count: sizes %lld %lld", (long long)na, (long long)nb);
+  GARY_CHECK_ARG(method >= GARY_ISECT_BSEARCH && method <= GARY_ISECT_BITMAP,
+                 "gary_intersect_count: bad method %d", method);
+  if (method == GARY_ISECT_CACHED_BSEARCH)
+    GARY_TRY(check_search_args(0, 0, GARY_SEARCH_2PHASE, cache_depth));
+  if (na == 0 || nb == 0) return GARY_OK;
+  GARY_CHECK_ARG(d_a && d_b, "gary_intersect_count: null list");
+  hipStream_t st = (hipStream_t)stream;
+  auto *cnt = (unsigned long long *)d_count;
+  if (method == GARY_ISECT_MERGE) {
+    isect_merge_kernel<<<grid_for((na + nb) / 16 + 1), kBlock, 0, st>>>(d_a, (int32_t)na, d_b,
+                                                                       (int32_t)nb, cnt);
+  } else if (method == GARY_ISECT_BITMAP) {
+    isect_bitmap_kernel<4096><<<2048, kBlock, 0, st>>>(d_a, (int32_t)na, d_b, (int32_t)nb, cnt);
+  } else {
+    if (na > nb) { std::swap(d_a, d_b); std::swap(na, nb); }
+    const int grid = grid_for(na);
+    if (method == GARY_ISECT_CACHED_BSEARCH)
+      isect_probe_kernel<kIsCached, false>
+          <<<grid, kBlock, kWavesPerBlock * cache_depth * sizeof(int32_t), st>>>(
+              d_a, (int32_t)na, d_b, (int32_t)nb, cache_depth, nullptr, nullptr, cnt);
+    else
+      isect_probe_kernel<kIsBsearch, false><<<grid, kBlock, 0, st>>>(
+          d_a, (int32_t)na, d_b, (int32_t)nb, 0, nullptr, nullptr, cnt);
+  }
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
+int gary_intersect_emit(const int32_t *d_a, int64_t na, const int32_t *d_b, int64_t nb,
+                        int method, int cache_depth, int32_t *d_out, uint64_t *d_count,
+                        void *stream) {
+  using namespace gary;
+  GARY_CHECK_ARG(d_count, "gary_intersect_emit: null counter");
+  GARY_CHECK_ARG(na >= 0 && nb >= 0 && na <= INT32_MAX && nb <= INT32_MAX,
+                 "gary_intersect_emit: sizes %lld %lld", (long long)na, (long long)nb);
+  GARY_CHECK_ARG(method == GARY_ISECT_BSEARCH || method == GARY_ISECT_CACHED_BSEARCH,
+                 "gary_intersect_emit: method %d unsupported (bsearch or cached_bsearch)", method);
+  if (method == GARY_ISECT_CACHED_BSEARCH)
+    GARY_TRY(check_search_args(0, 0, GARY_SEARCH_2PHASE, cache_depth));
+  hipStream_t st = (hipStream_t)stream;
+  if (na == 0 || nb == 0) {
+    GARY_HIP(hipMemsetAsync(d_count, 0, sizeof(uint64_t), st));
+    return GARY_OK;
+  }
+  GARY_CHECK_ARG(d_a && d_b && d_out, "gary_intersect_emit: null pointer");
+  if (na > nb) { std::swap(d_a, d_b); std::swap(na, nb); }
+  const int grid = grid_for(na, 2048);
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  void *ws = nullptr;
+  const size_t flag_bytes = ((size_t)na + 255) & ~(size_t)255;
+  GARY_TRY(workspace_get(dev, st, flag_bytes + sizeof(int32_t) * (size_t)grid, &ws));
+  uint8_t *flags = (uint8_t *)ws;
+  int32_t *bcounts = (int32_t *)((char *)ws + flag_bytes);
+  const size_t lds = method == GARY_ISECT_CACHED_BSEARCH
+                         ? kWavesPerBlock * cache_depth * sizeof(int32_t) : 0;
+  if (method == GARY_ISECT_CACHED_BSEARCH)
+    isect_probe_kernel<kIsCached, true><<<grid, kBlock, lds, st>>>(
+        d_a, (int32_t)na, d_b, (int32_t)nb, cache_depth, flags, bcounts, nullptr);
+  else
+    isect_probe_kernel<kIsBsearch, true><<<grid, kBlock, 0, st>>>(
+        d_a, (int32_t)na, d_b, (int32_t)nb, 0, flags, bcounts, nullptr);
+  GARY_HIP(hipGetLastError());
+  scan_counts_kernel<<<1, kBlock, 0, st>>>(bcounts, grid, (unsigned long long *)d_count);
+  GARY_HIP(hipGetLastError());
+  compact_kernel<<<grid, kBlock, 0, st>>>(d_a, (int32_t)na, flags, bcounts, d_out);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
+}  // extern "C"

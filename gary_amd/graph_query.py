@@ -1,0 +1,74 @@
+"""graph_query — triangle counting entry points (include/kernels/GraphQueryHom.cuh).
+
+The reference launches `triangleCountHom<useCache, N, Policy><<<grid, block, smem>>>(DeviceGraph,
+size_t*)` directly (test/TriQueryTest.cu:99-128); here the same choice is made through a
+strategy and a launch configuration, and the kernels run on the device behind libgary's C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+from ._lib import DeviceGraph, LaunchCfg, check, lib
+from .device import DeviceBuffer, DeviceGraphCtx
+
+AUTO_LEARNED = 0
+HEURISTIC = 1
+MERGE = 2
+CACHED_BSEARCH = 3
+BSEARCH = 4
+BITMAP = 5
+STRATEGIES = {"learned": AUTO_LEARNED, "heuristic": HEURISTIC, "merge": MERGE,
+              "cached_bsearch": CACHED_BSEARCH, "bsearch": BSEARCH, "bitmap": BITMAP}
+
+INTERLEAVED = 0
+CHUNKED = 1
+
+_P = C.c_void_p
+_bind = _lib.bind
+_bind("gary_tc_count", C.c_int, [C.POINTER(DeviceGraph), C.c_int, C.POINTER(LaunchCfg), _P, _P])
+_bind("gary_tc_count_sync", C.c_int, [C.POINTER(DeviceGraph), C.c_int, C.POINTER(LaunchCfg),
+                                      C.POINTER(C.c_uint64)])
+_bind("gary_tc_count_legacy", C.c_int, [C.POINTER(DeviceGraph), _P, _P])
+_bind("gary_index_cache_smem", C.c_int64, [C.c_int32, C.c_int64])
+
+
+def launch_cfg(block=0, grid=0, group=0, cache_depth=0, partition=INTERLEAVED, bin_small=0,
+               bin_hub=0) -> LaunchCfg:
+    return LaunchCfg(block, grid, group, cache_depth, partition, bin_small, bin_hub, 0)
+
+
+def _graph(g) -> DeviceGraph:
+    return g.deviceGraph() if isinstance(g, DeviceGraphCtx) else g
+
+
+def getIndexCacheSMem(block: int, cacheSize: int) -> int:
+    """Elements of LDS index cache per workgroup (src/GraphQueryHom.cu:11-13, 64-lane waves)."""
+    return int(lib.gary_index_cache_smem(block, cacheSize))
+
+
+def triangleCountHom(g, strategy: int = HEURISTIC, cfg: LaunchCfg | None = None) -> int:
+    """Triangle count of an oriented graph; returns the uint64 count."""
+    dg = _graph(g)
+    out = C.c_uint64(0)
+    check(lib.gary_tc_count_sync(C.byref(dg), strategy, C.byref(cfg) if cfg else None,
+                                 C.byref(out)))
+    return out.value
+
+
+def triangleCountHomAsync(g, d_count: DeviceBuffer, strategy: int = HEURISTIC,
+                          cfg: LaunchCfg | None = None, stream: int | None = None) -> None:
+    """Accumulate into a caller-zeroed device counter (the reference's `size_t *count`)."""
+    dg = _graph(g)
+    check(lib.gary_tc_count(C.byref(dg), strategy, C.byref(cfg) if cfg else None, d_count.ptr,
+                            stream))
+
+
+def triangleCountLegacy(g, d_count: DeviceBuffer, stream: int | None = None) -> None:
+    """triangleCountLegacy(DeviceGraph, unsigned*): 32-bit counter, raises on overflow."""
+    dg = _graph(g)
+    check(lib.gary_tc_count_legacy(C.byref(dg), d_count.ptr, stream))
+
+
+def synchronize(device: int = 0) -> None:
+    check(lib.gary_device_synchronize(device))

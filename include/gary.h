@@ -1,0 +1,218 @@
+/*
+ * gary.h — C ABI of the MI355X (gfx950) graph pattern-matching library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an int status
+ * (GARY_OK = 0, negative = error class) and never throws or exits. A thread-local message for
+ * the last failure is available from gary_last_error().
+ *
+ * Counts are always uint64_t. Device pointers are HIP device pointers of the device the
+ * context / call is bound to; `stream` is a hipStream_t passed as void* (NULL = null stream).
+ *
+ * Reference interfaces replaced (hirayaku/Gary, paths relative to the reference root):
+ *   DeviceCSR / DeviceCOO / DeviceGraph ........ include/DeviceGraph.cuh:15-72
+ *   DeviceGraphCtx ............................. include/DeviceGraph.cuh:75-133, src/DeviceGraph.cu:14-96
+ *   binary_search / build_cache / 2-phase ...... include/ops/Search.cuh:33-138 (test kernels test/SearchTest.cu:14-183)
+ *   intersectBinarySearch / intersect .......... include/ops/Intersection.cuh:18-199, 239-300
+ *   triangleCountHom / triangleCountLegacy ..... include/kernels/GraphQueryHom.cuh:15-43, src/GraphQueryHom.cu:15-38
+ *   getIndexCacheSMem .......................... src/GraphQueryHom.cu:11-13
+ *   rectCountHom (declared only there) ......... include/kernels/GraphQueryHom.cuh:48
+ *   cpuTriCount's CSR input (COO -> CSR) ....... src/Graph.cpp:19-91, 141-155
+ */
+#ifndef GARY_H_
+#define GARY_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- type ABI (include/Defines.h:9-10) --------------------------------------------------- */
+typedef int32_t gary_vid_t; /* vidT */
+typedef int64_t gary_eid_t; /* eidT */
+
+/* ---- status codes ------------------------------------------------------------------------- */
+enum {
+  GARY_OK = 0,
+  GARY_ERR_INVALID = -1,     /* bad argument (null pointer, bad size, bad enum, invalid CSR) */
+  GARY_ERR_HIP = -2,         /* a HIP runtime call failed */
+  GARY_ERR_OOM = -3,         /* device or host allocation failed */
+  GARY_ERR_OVERFLOW = -4,    /* result does not fit the requested output type */
+  GARY_ERR_UNSUPPORTED = -5, /* valid request this build does not handle */
+  GARY_ERR_STATE = -6        /* object not initialised / wrong state */
+};
+
+/* ---- device graph views (layout-identical to DeviceCSR / DeviceCOO / DeviceGraph,
+ *      include/DeviceGraph.cuh:15-72) ----------------------------------------------------- */
+typedef struct gary_device_csr {
+  gary_vid_t m;       /* number of vertices */
+  gary_eid_t nnz;     /* number of stored (directed) edges */
+  gary_eid_t *indPtr; /* [m+1] row pointers */
+  gary_vid_t *colIdx; /* [nnz] sorted adjacency lists */
+  gary_vid_t *deg;    /* [m] list lengths */
+} gary_device_csr;
+
+typedef struct gary_device_coo {
+  gary_vid_t m;
+  gary_eid_t nnz;
+  gary_vid_t *rowIdx; /* [nnz] source of each edge */
+  gary_vid_t *colIdx; /* [nnz] shared with the CSR */
+  gary_vid_t *deg;    /* [m]   shared with the CSR */
+} gary_device_coo;
+
+typedef struct gary_device_graph {
+  gary_device_csr csr;
+  gary_device_coo coo;
+} gary_device_graph;
+
+/* ---- errors / device ----------------------------------------------------------------------- */
+const char *gary_last_error(void);
+int gary_version(void);
+int gary_device_count(int *count);
+int gary_device_synchronize(int device);
+
+/* ---- host graph construction (src/Graph.cpp:19-91, 141-155 semantics, plus dedup and
+ *      degree orientation) -------------------------------------------------------------------- */
+
+/* Graph generators. All are deterministic functions of their arguments (splitmix64 streams,
+ * integer-only sampling). Output arrays hold `ntuples` directed tuples (u, v); loops and
+ * duplicates are left in for gary_csr_from_edges to remove. */
+int gary_gen_er(int32_t n, int64_t m, uint64_t seed, int32_t *src, int32_t *dst);
+int gary_gen_rmat(int32_t scale, int64_t ntuples, double a, double b, double c, uint64_t seed,
+                  int32_t *src, int32_t *dst);
+/* Chung-Lu: expected degrees w_i ~ (i + i0)^(-1/(gamma-1)), capped at max_w, `ntuples`
+ * endpoint pairs drawn with probability proportional to w. */
+int gary_gen_chung_lu(int32_t n, int64_t ntuples, double gamma, double max_w, uint64_t seed,
+                      int32_t *src, int32_t *dst);
+
+enum {
+  GARY_CSR_ORIENT_DEGREE = 0, /* u -> v iff (deg u, u) < (deg v, v); one arc per undirected edge */
+  GARY_CSR_SYMMETRIC = 1,     /* both arcs of every undirected edge */
+  GARY_CSR_ORIENT_ID = 2      /* u -> v iff u > v (the reference's GraphCOO::orientation) */
+};
+
+/* Symmetrize, drop self-loops, dedup, then build a CSR in `mode`. Lists are sorted ascending.
+ * indPtr has n+1 entries; colIdx must hold `cap` entries (ntuples suffices for the oriented
+ * modes, 2*ntuples for GARY_CSR_SYMMETRIC). *nnz_out receives the stored edge count. */
+int gary_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const int32_t *dst,
+                        int mode, int64_t *indPtr, int32_t *colIdx, int64_t cap,
+                        int64_t *nnz_out);
+
+/* ---- device graph context (DeviceGraphCtx) ------------------------------------------------- */
+typedef struct gary_ctx gary_ctx;
+
+int gary_ctx_create(int device, gary_ctx **out);
+int gary_ctx_destroy(gary_ctx *ctx);
+/* loadFromHost(shared_ptr<GraphCSR>): H2D of indPtr and colIdx; deg is derived on the device. */
+int gary_ctx_load_csr(gary_ctx *ctx, int32_t m, int64_t nnz, const int64_t *indPtr,
+                      const int32_t *colIdx);
+/* populateDevice(): derive the COO rowIdx on the device. */
+int gary_ctx_populate(gary_ctx *ctx);
+/* getDeviceCSR() + getDeviceCOO() in one view; valid while the ctx lives and is not reloaded. */
+int gary_ctx_device_graph(gary_ctx *ctx, gary_device_graph *out);
+/* getHostCSR() after detach: D2H copy of m+1 row pointers, nnz column indices, m degrees. */
+int gary_ctx_copy_to_host(gary_ctx *ctx, int64_t *indPtr, int32_t *colIdx, int32_t *deg);
+/* clear(): release device buffers. */
+int gary_ctx_clear(gary_ctx *ctx);
+
+/* ---- event timer (utils::CUDATimer, common/Timer.h:38-64, common/Timer.cpp:8-36) ---------- */
+typedef struct gary_timer gary_timer;
+int gary_timer_create(gary_timer **out);
+int gary_timer_destroy(gary_timer *t);
+/* Records the start event on `stream`. */
+int gary_timer_start(gary_timer *t, void *stream);
+/* Records the stop event on `stream`, waits for it and returns the elapsed milliseconds. */
+int gary_timer_stop(gary_timer *t, void *stream, float *ms);
+
+/* ---- device memory helpers (so hosts without a HIP toolchain can drive the ABI) --------- */
+int gary_malloc(int device, size_t bytes, void **ptr);
+int gary_free(int device, void *ptr);
+int gary_memcpy_h2d(int device, void *dst, const void *src, size_t bytes);
+int gary_memcpy_d2h(int device, void *dst, const void *src, size_t bytes);
+int gary_memset(int device, void *dst, int value, size_t bytes);
+
+/* ---- search and intersection operators (include/ops/Search.cuh, Intersection.cuh) ------- */
+enum {
+  GARY_SEARCH_PLAIN = 0,  /* binary_search, Search.cuh:48-61 */
+  GARY_SEARCH_2PHASE = 1  /* build_cache + binary_search_2phase, Search.cuh:93-138 */
+};
+/* Per-key membership of keys[i] in the sorted list (binarySearchKernel /
+ * binarySearch2PhaseKernel, test/SearchTest.cu:14-67). results[i] is 0/1. cache_depth is N of
+ * Array<T,N> (power of two, 8..1024) for GARY_SEARCH_2PHASE and ignored otherwise. */
+int gary_search(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64_t nkeys,
+                uint8_t *d_results, int mode, int cache_depth, void *stream);
+/* Number of keys found (binarySearchCountKernel / binarySearch2PhaseCountKernel), accumulated
+ * into *d_count. */
+int gary_search_count(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64_t nkeys,
+                      uint64_t *d_count, int mode, int cache_depth, void *stream);
+/* Batched search, one list per item (batchedBinarySearch{,2Phase}Kernel,
+ * test/SearchTest.cu:69-136). All pointer arrays are device arrays of device pointers. */
+int gary_search_batched(int32_t batch, const int32_t *const *d_lists, const int64_t *d_list_sizes,
+                        const int32_t *const *d_keys, const int64_t *d_key_sizes,
+                        uint8_t *const *d_results, int mode, int cache_depth, void *stream);
+
+enum {
+  GARY_ISECT_BSEARCH = 0,        /* intersectBinarySearch(sA,sB), Intersection.cuh:56-71 */
+  GARY_ISECT_CACHED_BSEARCH = 1, /* intersectBinarySearch(sA,sB,cache), :118-135 */
+  GARY_ISECT_MERGE = 2,          /* wave merge-path (the reference's intersectMerge is a stub) */
+  GARY_ISECT_BITMAP = 3          /* LDS bitmap over the shorter list's id window */
+};
+/* |A ∩ B| of two sorted duplicate-free device lists, accumulated into *d_count. */
+int gary_intersect_count(const int32_t *d_a, int64_t na, const int32_t *d_b, int64_t nb,
+                         int method, int cache_depth, uint64_t *d_count, void *stream);
+/* Intersection with output (Intersection.cuh:18-50, 76-112, 239-271). Writes |A ∩ B| values
+ * to d_out (capacity min(na, nb)) in ascending order, and the count to *d_count (overwritten). */
+int gary_intersect_emit(const int32_t *d_a, int64_t na, const int32_t *d_b, int64_t nb,
+                        int method, int cache_depth, int32_t *d_out, uint64_t *d_count,
+                        void *stream);
+
+/* ---- triangle counting (include/kernels/GraphQueryHom.cuh:15-43, src/GraphQueryHom.cu) --- */
+enum {
+  GARY_TC_AUTO_LEARNED = 0,   /* MLP strategy/cost selector (falls back to HEURISTIC untrained) */
+  GARY_TC_HEURISTIC = 1,      /* degree-binned, plain degree-threshold strategy choice */
+  GARY_TC_MERGE = 2,          /* every edge intersected by wave merge-path */
+  GARY_TC_CACHED_BSEARCH = 3, /* triangleCountHom<true,N,Policy>: two-phase LDS-cached search */
+  GARY_TC_BSEARCH = 4,        /* triangleCountHom<false>: plain binary search */
+  GARY_TC_BITMAP = 5,         /* pivot-vertex LDS bitmap/hash of N+(u), probes N+(v) */
+  GARY_TC_NUM_STRATEGIES = 6
+};
+enum {
+  GARY_PARTITION_INTERLEAVED = 0, /* WorkPartition::INTERLEAVED, DeviceIterator.cuh:117-131 */
+  GARY_PARTITION_CHUNKED = 1      /* WorkPartition::CHUNKED */
+};
+
+/* Launch knobs. Zero fields take the library default (or the autotuned value). */
+typedef struct gary_launch_cfg {
+  int32_t block;       /* threads per workgroup of the edge-centric kernels (64, 128, 192, 256) */
+  int32_t grid;        /* workgroups (0 = library default) */
+  int32_t group;       /* lanes per edge in the edge-centric kernels: 8, 16, 32 or 64 */
+  int32_t cache_depth; /* N of the per-group LDS index cache (2-phase search), power of two */
+  int32_t partition;   /* GARY_PARTITION_* for the edge-centric strategies */
+  int32_t bin_small;   /* pivots with out-degree <= bin_small use the small-table wave kernel */
+  int32_t bin_hub;     /* pivots with out-degree > bin_hub get a whole workgroup */
+  int32_t reserved;
+} gary_launch_cfg;
+
+/* Elements (not bytes) of LDS index cache for a workgroup of `block` threads with depth
+ * `cache_size` (getIndexCacheSMem, src/GraphQueryHom.cu:11-13; waves of 64 lanes). */
+int64_t gary_index_cache_smem(int32_t block, int64_t cache_size);
+
+/* Triangle count of an acyclically oriented graph: Σ over arcs (u,v) of |N+(u) ∩ N+(v)|.
+ * Accumulates into the device counter *d_count (the caller zeroes it), on `stream`. cfg may be
+ * NULL. The graph's COO (populate) is required by the edge-centric strategies. */
+int gary_tc_count(const gary_device_graph *g, int strategy, const gary_launch_cfg *cfg,
+                  uint64_t *d_count, void *stream);
+/* Synchronous convenience: returns the count in *h_count. */
+int gary_tc_count_sync(const gary_device_graph *g, int strategy, const gary_launch_cfg *cfg,
+                       uint64_t *h_count);
+/* triangleCountLegacy(DeviceGraph, unsigned*): accumulates into a 32-bit device counter.
+ * Unlike the reference it counts in 64 bits and fails with GARY_ERR_OVERFLOW instead of
+ * wrapping when the sum would not fit. Synchronizes `stream`. */
+int gary_tc_count_legacy(const gary_device_graph *g, uint32_t *d_count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GARY_H_ */

@@ -1,0 +1,89 @@
+/*
+ * oracle.h — CPU restatement of hirayaku/Gary's pattern-matching hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP implementation in
+ * gary_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * Nothing in the product links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the reference
+ * root). Counts are uint64_t (the reference truncates its CPU count to 32 bits at
+ * test/TriQueryTest.cu:17,36; that truncation is deliberately not reproduced).
+ */
+#ifndef GARY_ORACLE_H_
+#define GARY_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/ops/Search.cuh:48-61 — thread-sequential binary search (membership). */
+int or_binary_search(const int32_t *list, int64_t size, int32_t key);
+
+/* include/ops/Search.cuh:67-81 — binary search that reports the probe position; on a miss
+ * *pos is the index of the last element < key (-1 when key < list[0]). */
+int or_binary_search_pos(const int32_t *list, int64_t size, int32_t key, int32_t *pos);
+
+/* include/ops/Search.cuh:93-100 — cache[i] = list[i*size/N], i in [0, N). size >= 1. */
+void or_build_cache(const int32_t *list, int64_t size, int32_t *cache, int32_t n_cache);
+
+/* include/ops/Search.cuh:120-138 — phase 1 in the cache, phase 2 in
+ * list[pos*size/N, (pos+1)*size/N). size >= 1. */
+int or_binary_search_2phase(const int32_t *list, int64_t size, const int32_t *cache,
+                            int32_t n_cache, int32_t key);
+
+/* |A ∩ B| by a sequential merge (std::set_intersection, test/TriQueryTest.cu:30-31). */
+uint64_t or_intersect_count(const int32_t *a, int64_t na, const int32_t *b, int64_t nb);
+
+/* include/ops/Intersection.cuh:56-71 (n_cache == 0) and :118-135 (n_cache > 0): swap so that
+ * |A| <= |B|, lanes stride over A and search B. Writes the per-lane partial counts (the
+ * reference returns per-thread counts) to partials[lanes] when non-null; returns their sum.
+ * Returns 0 when either list is empty (Intersection.cuh:190,197). */
+uint64_t or_intersect_bsearch_lanes(const int32_t *a, int64_t na, const int32_t *b, int64_t nb,
+                                    int32_t n_cache, int32_t lanes, uint64_t *partials);
+
+/* Intersection with output in ascending order (Intersection.cuh:18-50, 76-112 with
+ * OutSorted; std::set_intersection output as in test/SetOpTest.cu:94-96). Returns the count. */
+int64_t or_intersect_emit(const int32_t *a, int64_t na, const int32_t *b, int64_t nb,
+                          int32_t *out);
+
+/* COO -> CSR with the semantics of src/Graph.cpp:19-91 (symmetrize, toCSR_) and
+ * src/Graph.cpp:34-61 (orientation), plus self-loop removal and dedup, which the reference
+ * leaves out. mode: 0 = orient by (degree, id) low -> high, 1 = symmetric, 2 = orient by id
+ * keeping src > dst (Graph.cpp:34-61). Returns 0, or -1 when cap is too small / bad input. */
+int or_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const int32_t *dst,
+                      int32_t mode, int64_t *indPtr, int32_t *colIdx, int64_t cap,
+                      int64_t *nnz_out);
+
+/* test/TriQueryTest.cu:17-37 — cpuTriCount: for each v (OpenMP, schedule(dynamic)), for each
+ * u in N(v): |set_intersection(N(u), N(v))|. threads <= 0 uses the OpenMP default. */
+uint64_t or_tri_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t threads);
+
+/* Number of OpenMP threads a parallel region of `threads` (<= 0 = default) really uses. */
+int32_t or_omp_threads(int32_t threads);
+
+/* 4-vertex homomorphism counts on a symmetric simple CSR (sorted lists, no loops). No
+ * reference counterpart: graph_query::rectCountHom is declared but never defined
+ * (include/kernels/GraphQueryHom.cuh:48). Definitions, with A the adjacency matrix:
+ *   tri   = #triangles = tr(A^3)/6
+ *   k4    = #4-cliques,          hom(K4)      = 24*k4
+ *   hom_diamond = sum over arcs (a,b) of |N(a) ∩ N(b)|^2
+ *   hom_c4      = tr(A^4) = sum over (u,w) of codeg(u,w)^2 with codeg(u,u) = deg(u)
+ * hom_c4 here is computed by the dense wedge scan (cost sum_v deg(v)^2): small graphs only.
+ * Returns 0, or -4 when a value overflows 64 bits (checked in 128-bit arithmetic). */
+typedef struct or_hom4_result {
+  uint64_t tri;
+  uint64_t k4;
+  uint64_t hom_k4;
+  uint64_t hom_diamond;
+  uint64_t hom_c4;
+  uint64_t c4; /* #4-cycles = (hom_c4 - 4*sum C(d,2) - 2*m) / 8 */
+} or_hom4_result;
+int or_hom4(int32_t n, const int64_t *indPtr, const int32_t *colIdx, int32_t threads,
+            or_hom4_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""Pins the CPU oracle against independent answers: numpy, networkx/brute-force fixtures and
+closed forms. No GPU."""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+
+
+def _sorted_unique(rng, n, lo, hi):
+    return np.unique(rng.integers(lo, hi, size=n)).astype(np.int32)
+
+
+# ---- search (Search.cuh:33-138) --------------------------------------------------------------
+
+@pytest.mark.parametrize("size", list(range(1, 70)) + [97, 128, 129, 200])
+def test_binary_search_exhaustive(size):
+    rng = np.random.default_rng(size)
+    lst = np.sort(rng.choice(np.arange(-5, 3 * size + 5), size=size, replace=False)).astype(np.int32)
+    for key in range(int(lst[0]) - 1, int(lst[-1]) + 2):
+        want = bool(np.isin(key, lst))
+        assert oracle.binary_search(lst, key) == want
+        found, pos = oracle.binary_search_pos(lst, key)
+        assert found == want
+        if found:
+            assert lst[pos] == key
+        else:  # index of the last element < key
+            assert pos == int(np.searchsorted(lst, key, side="left")) - 1
+
+
+@pytest.mark.parametrize("n_cache", [1, 2, 8, 16, 32, 64, 128])
+@pytest.mark.parametrize("size", [1, 2, 3, 7, 31, 32, 33, 63, 64, 65, 100, 200, 1000])
+def test_binary_search_2phase_exhaustive(n_cache, size):
+    rng = np.random.default_rng(1000 * n_cache + size)
+    lst = np.sort(rng.choice(np.arange(0, 4 * size + 8), size=size, replace=False)).astype(np.int32)
+    cache = oracle.build_cache(lst, n_cache)
+    # cache[i] = list[i*size/N] (Search.cuh:93-100): closed form, 64-bit products
+    idx = (np.arange(n_cache, dtype=np.int64) * size) // n_cache
+    assert np.array_equal(cache, lst[idx])
+    keys = np.arange(int(lst[0]) - 1, int(lst[-1]) + 2)
+    want = np.isin(keys, lst)
+    got = np.array([oracle.binary_search_2phase(lst, cache, k) for k in keys])
+    assert np.array_equal(got, want)
+
+
+def test_search_reference_shape_seed0():
+    # the shape of test/SearchTest.cu:196-210 (4 Mi sorted ints in [2^29, 2^30], 32 Ki keys),
+    # scaled down; membership must equal std::binary_search == numpy isin.
+    rng = np.random.default_rng(0)
+    lst = np.sort(rng.integers(2**29, 2**30, size=1 << 16)).astype(np.int32)
+    lst = np.unique(lst)
+    keys = rng.integers(2**29, 2**30, size=4096).astype(np.int32)
+    keys[:1000] = rng.choice(lst, 1000)
+    cache = oracle.build_cache(lst, 32)
+    want = np.isin(keys, lst)
+    assert np.array_equal([oracle.binary_search(lst, k) for k in keys], want)
+    assert np.array_equal([oracle.binary_search_2phase(lst, cache, k) for k in keys], want)
+
+
+# ---- intersection (Intersection.cuh) ---------------------------------------------------------
+
+def test_intersections_random():
+    rng = np.random.default_rng(5)
+    for trial in range(300):
+        na, nb = rng.integers(0, 300, size=2)
+        hi = int(rng.integers(1, 1000))
+        a = _sorted_unique(rng, na, 0, hi)
+        b = _sorted_unique(rng, nb, 0, hi)
+        want = np.intersect1d(a, b)
+        assert oracle.intersect_count(a, b) == want.size
+        assert np.array_equal(oracle.intersect_emit(a, b), want)
+        for n_cache in (0, 8, 32, 64):
+            tot, part = oracle.intersect_bsearch_lanes(a, b, n_cache, lanes=32)
+            assert tot == want.size and int(part.sum()) == want.size
+
+
+def test_intersections_exhaustive_tiny():
+    universe = range(6)
+    subsets = [np.array(c, np.int32) for k in range(0, 5) for c in itertools.combinations(universe, k)]
+    for a in subsets[::3]:
+        for b in subsets:
+            want = np.intersect1d(a, b)
+            assert oracle.intersect_count(a, b) == want.size
+            assert oracle.intersect_bsearch_lanes(a, b, 8, 4)[0] == want.size
+
+
+def test_intersect_lane_partials_follow_shorter_list():
+    # per-lane partials: lanes stride over the shorter list (Intersection.cuh:130)
+    a = np.arange(0, 100, 2, dtype=np.int32)     # 50 items
+    b = np.arange(0, 1000, 1, dtype=np.int32)    # superset
+    tot, part = oracle.intersect_bsearch_lanes(b, a, 0, lanes=32)  # swapped internally
+    assert tot == 50
+    assert [int(x) for x in part] == [2 if l < 18 else 1 for l in range(32)]
+
+
+# ---- CSR build (Graph.cpp:19-91) ---------------------------------------------------------------
+
+def test_csr_build_matches_networkx(golden_graphs):
+    for rec in golden_graphs:
+        n = rec["n"]
+        src, dst = np.array(rec["src"], np.int32), np.array(rec["dst"], np.int32)
+        indPtr, colIdx = oracle.csr_from_edges(n, src, dst, mode=1)
+        got = sorted([u, int(v)] for u in range(n) for v in colIdx[indPtr[u]:indPtr[u + 1]] if u < v)
+        assert got == rec["edges_sorted"], rec["name"]
+        deg_full = np.diff(indPtr)
+        for mode in (0, 2):
+            ip, ci = oracle.csr_from_edges(n, src, dst, mode=mode)
+            assert ip[-1] == rec["m"]
+            for u in range(n):
+                row = ci[ip[u]:ip[u + 1]]
+                assert np.all(np.diff(row) > 0)
+                for v in row:
+                    if mode == 0:
+                        assert (deg_full[u], u) < (deg_full[v], v)
+                    else:
+                        assert u > v
+
+
+def test_tri_count_golden(golden_graphs):
+    for rec in golden_graphs:
+        n = rec["n"]
+        src, dst = np.array(rec["src"], np.int32), np.array(rec["dst"], np.int32)
+        for mode in (0, 2):  # orientation invariance (degree vs id)
+            indPtr, colIdx = oracle.csr_from_edges(n, src, dst, mode=mode)
+            assert oracle.tri_count(indPtr, colIdx) == rec["triangles"], rec["name"]
+
+
+def test_tri_count_closed_forms():
+    for k in (3, 10, 50):
+        s, d = zip(*itertools.combinations(range(k), 2))
+        ip, ci = oracle.csr_from_edges(k, np.array(s), np.array(d), 0)
+        assert oracle.tri_count(ip, ci) == k * (k - 1) * (k - 2) // 6
+    # star and bipartite: no triangles
+    ip, ci = oracle.csr_from_edges(101, np.zeros(100, np.int32), np.arange(1, 101), 0)
+    assert oracle.tri_count(ip, ci) == 0
+
+
+def test_hom4_golden(golden_graphs):
+    for rec in golden_graphs:
+        n = rec["n"]
+        ip, ci = oracle.csr_from_edges(n, np.array(rec["src"], np.int32),
+                                       np.array(rec["dst"], np.int32), 1)
+        h = oracle.hom4(ip, ci)
+        assert h["tri"] == rec["triangles"], rec["name"]
+        assert h["k4"] == rec["k4"], rec["name"]
+        assert h["hom_k4"] == rec["hom_k4"], rec["name"]
+        assert h["hom_diamond"] == rec["hom_diamond"], rec["name"]
+        assert h["hom_c4"] == rec["hom_c4"], rec["name"]
+        assert h["c4"] == rec["c4"], rec["name"]
+
+
+def test_golden_brute_force_consistency(golden_graphs):
+    # fixtures with brute-force hom counts must agree with their closed forms
+    n_brute = 0
+    for rec in golden_graphs:
+        if "brute_hom_c4" in rec:
+            n_brute += 1
+            assert rec["brute_hom_k4"] == rec["hom_k4"]
+            assert rec["brute_hom_diamond"] == rec["hom_diamond"]
+            assert rec["brute_hom_c4"] == rec["hom_c4"]
+    assert n_brute >= 8
+
+
+def test_oracle_plants_fail():
+    # a planted mistake (dropping one element of a list) must change the count
+    indPtr, colIdx = oracle.csr_from_edges(5, np.array([0, 0, 1, 0, 1, 2]), np.array([1, 2, 2, 3, 3, 3]), 0)
+    t = oracle.tri_count(indPtr, colIdx)
+    assert t == 4
+    ci = colIdx.copy()
+    ci[2] = 4  # replace arc 0->3 by 0->4 (vertex 4 has no arcs): two triangles disappear
+    assert oracle.tri_count(indPtr, ci) == t - 2
