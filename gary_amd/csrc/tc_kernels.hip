@@ -141,263 +141,376 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
 // ======================================================================================
 // Pivot-centric kernels
 // ======================================================================================
+//
+// A pivot u owns an LDS membership set of N+(u); every row N+(v), v in N+(u), is streamed
+// through it by a whole wave (lanes stride the row, four loads in flight per lane). On R-MAT
+// scale 20, 99% of the probes fall in rows with at least 64 entries, so striding rows keeps the
+// lanes busy without any per-probe row lookup.
 
 __device__ __forceinline__ uint32_t hash_slot(int32_t w, int32_t log_t) {
   return ((uint32_t)w * 0x9E3779B1u) >> (32 - log_t);
 }
 
-// LDS membership structure of one pivot list L[0..d): bitmap over [lo, hi] if the window fits
-// `bitmap_words` words and is not much larger than a hash table would be, else a hash table of
-// 2^log_t slots (log_t chosen with load factor <= 1/2).
+// Membership set of one pivot list N+(u) in LDS.
+//
+// Bitmap mode (log_t == 0), a two-level rank bitmap over x = w - base, x < span:
+//   coarse[cwords] : bit c set iff some element has x >> s == c
+//   prefix[cwords] : number of set coarse bits before word i
+//   exact[...]     : one block of 2^s bits per set coarse bit, in coarse-bit order
+// Membership costs at most three LDS reads and no loop; elements of one sorted row share coarse
+// words, so neighbouring lanes mostly read the same addresses.
+// Hash mode (log_t > 0): open addressing over 2^log_t slots (empty = -1, linear probing), used
+// when the bitmap does not fit.
 struct PivotSet {
-  int32_t lo, hi;
-  int32_t log_t;   // hash slots = 1 << log_t; 0 means bitmap mode
-  int32_t words;   // bitmap words (bitmap mode)
+  int32_t base;
+  uint32_t span;
+  int32_t s;       // bitmap: log2 of the ids per coarse bit (>= 5)
+  int32_t cwords;  // bitmap: coarse words
+  int32_t log_t;   // hash: log2 slots; 0 in bitmap mode
+  int32_t words;   // words of the table in use (cleared at build)
 };
 
-__device__ __forceinline__ PivotSet pivot_plan(int32_t lo, int32_t hi, int32_t d,
-                                               int32_t table_words) {
-  PivotSet s;
-  s.lo = lo;
-  s.hi = hi;
-  int32_t log_t = 6;
-  while ((1 << log_t) < 2 * d) ++log_t;
-  const int64_t span = (int64_t)hi - lo + 1;
-  const int64_t words = (span + 31) >> 5;
-  if (words <= table_words && words <= 4 * (1 << log_t)) {
-    s.log_t = 0;
-    s.words = (int32_t)words;
-  } else {
-    s.log_t = log_t;
-    s.words = 0;
+__device__ __forceinline__ bool pivot_plan(int32_t lo, int32_t hi, int32_t d, int32_t table_words,
+                                           PivotSet &ps) {
+  ps.base = lo;
+  ps.span = (uint32_t)(hi - lo) + 1u;
+  int32_t best_s = 5;
+  int64_t best = INT64_MAX;
+  for (int32_t s = 5; s <= 20; ++s) {
+    const int64_t cw = ((int64_t)ps.span + (1ll << (s + 5)) - 1) >> (s + 5);
+    const int64_t w = 2 * cw + ((int64_t)d << (s - 5));
+    if (w < best) {
+      best = w;
+      best_s = s;
+    }
   }
-  return s;
+  if (best <= table_words) {
+    ps.s = best_s;
+    ps.cwords = (int32_t)(((int64_t)ps.span + (1ll << (best_s + 5)) - 1) >> (best_s + 5));
+    ps.log_t = 0;
+    ps.words = (int32_t)best;
+    return true;
+  }
+  int32_t log_t = 5;
+  while ((1 << log_t) < 4 * d) ++log_t;
+  while ((1 << log_t) > table_words && (1 << (log_t - 1)) >= 2 * d) --log_t;
+  if ((1 << log_t) <= table_words) {
+    ps.s = 0;
+    ps.cwords = 0;
+    ps.log_t = log_t;
+    ps.words = 1 << log_t;
+    return true;
+  }
+  return false;
 }
 
-// Probe w against the pivot set (one LDS read for the bitmap; linear probing for the hash).
-__device__ __forceinline__ uint32_t pivot_probe(const int32_t *table, const PivotSet &s,
-                                                int32_t w) {
-  if (w < s.lo || w > s.hi) return 0;
-  if (s.log_t == 0) {
-    const uint32_t off = (uint32_t)(w - s.lo);
-    return ((uint32_t)table[off >> 5] >> (off & 31)) & 1u;
+// Builds the set from the sorted list L[0..d). tid/nthr: this thread among the builders (a wave
+// or a workgroup); BLOCK selects workgroup-wide synchronisation (the prefix scan then runs on
+// wave 0 of the workgroup).
+template <bool BLOCK>
+__device__ __forceinline__ void pivot_sync() {
+  if constexpr (BLOCK) __syncthreads();
+  else wave_sync();
+}
+
+template <bool BLOCK>
+__device__ __forceinline__ void pivot_build(int32_t *t, const PivotSet &ps,
+                                            const int32_t *__restrict__ L, int32_t d, int tid,
+                                            int nthr) {
+  const int32_t fillv = ps.log_t ? -1 : 0;
+  for (int32_t i = tid; i < ps.words; i += nthr) t[i] = fillv;
+  pivot_sync<BLOCK>();
+  if (ps.log_t) {
+    const uint32_t mask = (1u << ps.log_t) - 1;
+    for (int32_t i = tid; i < d; i += nthr) {
+      const int32_t w = L[i];
+      uint32_t h = hash_slot(w, ps.log_t);
+      while (atomicCAS(&t[h], -1, w) != -1) h = (h + 1) & mask;
+    }
+    pivot_sync<BLOCK>();
+    return;
   }
-  const uint32_t mask = (1u << s.log_t) - 1;
-  uint32_t h = hash_slot(w, s.log_t);
+  uint32_t *coarse = (uint32_t *)t;
+  int32_t *prefix = t + ps.cwords;
+  uint32_t *exact = (uint32_t *)(t + 2 * ps.cwords);
+  const int32_t s = ps.s;
+  for (int32_t i = tid; i < d; i += nthr) {
+    const uint32_t x = (uint32_t)(L[i] - ps.base);
+    if (x < ps.span) atomicOr(&coarse[x >> (s + 5)], 1u << ((x >> s) & 31));
+  }
+  pivot_sync<BLOCK>();
+  if (tid < kWave) {
+    uint32_t carry = 0;
+    for (int32_t c0 = 0; c0 < ps.cwords; c0 += kWave) {
+      const int32_t i = c0 + tid;
+      const uint32_t v = i < ps.cwords ? (uint32_t)__popc(coarse[i]) : 0u;
+      const uint32_t incl = wave_inclusive_scan_u32(v);
+      if (i < ps.cwords) prefix[i] = (int32_t)(carry + incl - v);
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+    }
+  }
+  pivot_sync<BLOCK>();
+  for (int32_t i = tid; i < d; i += nthr) {
+    const uint32_t x = (uint32_t)(L[i] - ps.base);
+    if (x >= ps.span) continue;
+    const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
+    const uint32_t r = (uint32_t)prefix[cw] + (uint32_t)__popc(coarse[cw] & ((1u << b) - 1));
+    atomicOr(&exact[(r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)], 1u << (x & 31));
+  }
+  pivot_sync<BLOCK>();
+}
+
+__device__ __forceinline__ uint32_t pivot_probe(const int32_t *t, const PivotSet &ps, int32_t w) {
+  const uint32_t x = (uint32_t)(w - ps.base);
+  if (x >= ps.span) return 0;
+  if (ps.log_t == 0) {
+    const int32_t s = ps.s;
+    const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
+    const uint32_t word = (uint32_t)t[cw];
+    if (!((word >> b) & 1u)) return 0;
+    const uint32_t r = (uint32_t)t[ps.cwords + cw] + (uint32_t)__popc(word & ((1u << b) - 1));
+    const uint32_t e = (uint32_t)t[2 * ps.cwords + (r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)];
+    return (e >> (x & 31)) & 1u;
+  }
+  const uint32_t mask = (1u << ps.log_t) - 1;
+  uint32_t h = hash_slot(w, ps.log_t);
   while (true) {
-    const int32_t t = table[h];
-    if (t == w) return 1;
-    if (t == -1) return 0;
+    const int32_t v = t[h];
+    if (v == w) return 1;
+    if (v == -1) return 0;
     h = (h + 1) & mask;
   }
 }
 
-// Wave-per-pivot kernel. Each wave owns `table_words` words of LDS for its pivot set plus the
-// 64-entry row tables of the flattened (v, w) loop.
-template <int WAVES, int TABLE_WORDS>
-__global__ void __launch_bounds__(WAVES * 64)
-    tc_pivot_wave_kernel(gary_device_graph g, const int32_t *__restrict__ pivots,
-                         const int32_t *__restrict__ d_npivots, unsigned long long *count) {
-  const int32_t npivots = *d_npivots;
-  __shared__ int32_t s_table[WAVES][TABLE_WORDS];
-  __shared__ int32_t s_end[WAVES][64];
-  __shared__ int64_t s_ptr[WAVES][64];
-  const int wib = threadIdx.x / kWave;
-  const int lane = lane_id();
-  int32_t *table = s_table[wib];
-  int32_t *rend = s_end[wib];
-  int64_t *rptr = s_ptr[wib];
+// Per-wave LDS scratch of the flattened row streaming: 64 rows of metadata.
+struct RowBatch {
+  int32_t end[kWave];   // inclusive prefix of the rows' 16-byte chunk counts
+  int32_t dv[kWave];    // row lengths
+  int64_t pv[kWave];    // row starts (element index into colIdx)
+};
+
+// The rows r0, r0 + stride, ... (< d) of pivot list L, 64 rows per batch. A batch is flattened
+// into the 16-byte chunks that overlap its rows; lane l takes chunks q0 + U*l .. q0 + U*l + U-1,
+// so one binary search over the batch's chunk prefix (LDS) places the lane, its U loads are
+// issued together, and the wave's loads cover one contiguous stretch of colIdx. Elements of a
+// chunk outside the chunk's row are masked; the array's last partial chunk is read element-wise.
+template <int U>
+__device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
+                                                const int32_t *__restrict__ L, int32_t d,
+                                                int32_t r0, int32_t stride, const int32_t *table,
+                                                const PivotSet &ps, RowBatch &rb, int lane) {
   const uint32_t m = (uint32_t)g.csr.m;
-  const int64_t nw = (int64_t)gridDim.x * WAVES;
+  const int64_t nnz = g.csr.nnz;
+  const int32_t *__restrict__ col = g.csr.colIdx;
   uint32_t c = 0;
-  for (int64_t p = (int64_t)blockIdx.x * WAVES + wib; p < npivots; p += nw) {
-    const int32_t u = pivots ? pivots[p] : (int32_t)p;
-    if ((uint32_t)u >= m) continue;
-    const int32_t d = g.csr.deg[u];
-    if (d < 2) continue;
-    const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
-    const PivotSet s = pivot_plan(L[0], L[d - 1], d, TABLE_WORDS);
-    if (s.log_t == 0 || (1 << s.log_t) <= TABLE_WORDS) {
-      // ---- build the LDS set
-      const int32_t fill_words = s.log_t == 0 ? s.words : (1 << s.log_t);
-      const int32_t fill_val = s.log_t == 0 ? 0 : -1;
-      for (int32_t i = lane; i < fill_words; i += kWave) table[i] = fill_val;
-      wave_sync();
-      for (int32_t i = lane; i < d; i += kWave) {
-        const int32_t w = L[i];
-        if (s.log_t == 0) {
-          const uint32_t off = (uint32_t)(w - s.lo);  // < 32*words for a sorted list
-          if (off < (uint32_t)s.words * 32u) atomicOr((uint32_t *)&table[off >> 5], 1u << (off & 31));
-        } else {
-          const uint32_t mask = (1u << s.log_t) - 1;
-          uint32_t h = hash_slot(w, s.log_t);
-          while (atomicCAS(&table[h], -1, w) != -1) h = (h + 1) & mask;
-        }
+  for (int32_t b = r0; b < d; b += stride * kWave) {
+    const int32_t r = b + lane * stride;
+    int32_t dv = 0;
+    int64_t pv = 0;
+    if (r < d) {
+      const int32_t v = L[r];
+      if ((uint32_t)v < m) {
+        dv = g.csr.deg[v];
+        pv = g.csr.indPtr[v];
       }
-      wave_sync();
-      // ---- stream N+(v) for v in N+(u), 64 rows at a time, flattened over lanes
-      for (int32_t c0 = 0; c0 < d; c0 += kWave) {
-        const int32_t i = c0 + lane;
-        int32_t dv = 0;
-        int64_t pv = 0;
-        if (i < d) {
-          const int32_t v = L[i];
-          if ((uint32_t)v < m) {
-            dv = g.csr.deg[v];
-            pv = g.csr.indPtr[v];
+    }
+    const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + 3) >> 2) - (pv >> 2)) : 0;
+    const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
+    const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    rb.end[lane] = incl;
+    rb.dv[lane] = dv;
+    rb.pv[lane] = pv;
+    wave_sync();
+    for (int32_t q0 = 0; q0 < total; q0 += U * kWave) {
+      const int32_t qb = q0 + lane * U;
+      if (qb >= total) continue;
+      int32_t k = 0;  // first row with end > qb
+#pragma unroll
+      for (int32_t half = kWave / 2; half > 0; half >>= 1)
+        if (rb.end[k + half - 1] <= qb) k += half;
+      k += (rb.end[k] <= qb);
+      int32_t kend = rb.end[k];
+      int32_t first = k ? rb.end[k - 1] : 0;
+      int64_t lo = rb.pv[k];
+      int32_t len = rb.dv[k];
+      int4 data[U];
+      int32_t jlo[U], jhi[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t q = qb + u;
+        jlo[u] = 4;
+        jhi[u] = 0;
+        if (q < total) {
+          while (q >= kend) {  // rows without chunks share their predecessor's end
+            ++k;
+            first = kend;
+            kend = rb.end[k];
+            lo = rb.pv[k];
+            len = rb.dv[k];
+          }
+          const int64_t e0 = 4 * ((lo >> 2) + (q - first));
+          jlo[u] = (int32_t)max<int64_t>(0, lo - e0);
+          jhi[u] = (int32_t)min<int64_t>(4, lo + len - e0);
+          if (e0 + 3 < nnz) {
+            data[u] = *reinterpret_cast<const int4 *>(col + e0);
+          } else {
+            data[u].x = e0 < nnz ? col[e0] : 0;
+            data[u].y = e0 + 1 < nnz ? col[e0 + 1] : 0;
+            data[u].z = e0 + 2 < nnz ? col[e0 + 2] : 0;
+            data[u].w = 0;
           }
         }
-        const uint32_t incl = wave_inclusive_scan_u32((uint32_t)dv);
-        const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
-        rend[lane] = (int32_t)incl;
-        rptr[lane] = pv;
-        wave_sync();
-        for (uint32_t f = lane; f < total; f += kWave) {
-          // row j = first index with rend[j] > f
-          int32_t j = 0;
+      }
 #pragma unroll
-          for (int32_t half = 32; half > 0; half >>= 1)
-            if ((uint32_t)rend[j + half - 1] <= f) j += half;
-          j += ((uint32_t)rend[j] <= f);
-          const uint32_t start = j ? (uint32_t)rend[j - 1] : 0u;
-          const int32_t w = g.csr.colIdx[rptr[j] + (f - start)];
-          c += pivot_probe(table, s, w);
-        }
-        wave_sync();
+      for (int u = 0; u < U; ++u) {
+        c += (jlo[u] <= 0 && 0 < jhi[u]) ? pivot_probe(table, ps, data[u].x) : 0u;
+        c += (jlo[u] <= 1 && 1 < jhi[u]) ? pivot_probe(table, ps, data[u].y) : 0u;
+        c += (jlo[u] <= 2 && 2 < jhi[u]) ? pivot_probe(table, ps, data[u].z) : 0u;
+        c += (jlo[u] <= 3 && 3 < jhi[u]) ? pivot_probe(table, ps, data[u].w) : 0u;
       }
-    } else {
-      // The pivot set does not fit this wave's LDS: intersect edge by edge (plain search of the
-      // shorter list in the longer one). The dispatcher sends such pivots to the hub kernel, so
-      // this path only guards against a mis-binned call.
-      for (int32_t k = 0; k < d; ++k) {
-        const int32_t v = L[k];
-        if ((uint32_t)v >= m) continue;
-        c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + g.csr.indPtr[v], g.csr.deg[v],
-                                            nullptr, 0, lane);
+    }
+    wave_sync();  // rb is rewritten by the next batch
+  }
+  return c;
+}
+
+// Edge-by-edge fallback for a pivot whose set does not fit in LDS: rows r0, r0+stride, ...
+__device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
+                                                   const int32_t *__restrict__ L, int32_t d,
+                                                   int32_t r0, int32_t stride, int lane) {
+  const uint32_t m = (uint32_t)g.csr.m;
+  uint32_t c = 0;
+  for (int32_t k = r0; k < d; k += stride) {
+    const int32_t v = L[k];
+    if ((uint32_t)v >= m) continue;
+    c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + g.csr.indPtr[v], g.csr.deg[v],
+                                        nullptr, 0, lane);
+  }
+  return c;
+}
+
+// Persistent pivot kernel.
+// Phase 1: each workgroup takes the next pivot of `big` from an atomic cursor, builds one
+//   shared set of TABLE_WORDS words and its WAVES waves split the rows.
+// Phase 2: each wave takes `kSmallBatch` pivots of `small` at a time and works alone with a
+//   1/WAVES share of the table memory.
+// counts[0] = #small, counts[1] = #big; cursors[0..1] start at zero.
+constexpr int kSmallBatch = 16;
+constexpr int kUnroll = 2;
+
+template <int WAVES, int TABLE_WORDS>
+__global__ void __launch_bounds__(WAVES * 64)
+    tc_pivot_kernel(gary_device_graph g, const int32_t *__restrict__ small,
+                    const int32_t *__restrict__ big, const int32_t *__restrict__ counts,
+                    int32_t *__restrict__ cursors, unsigned long long *count) {
+  __shared__ int32_t table[TABLE_WORDS];
+  __shared__ RowBatch s_rows[WAVES];
+  __shared__ int32_t s_item;
+  const int lane = lane_id();
+  const int wib = threadIdx.x / kWave;
+  RowBatch &rb = s_rows[wib];
+  const int32_t n_small = counts[0], n_big = counts[1];
+  const uint32_t m = (uint32_t)g.csr.m;
+  uint32_t c = 0;
+  // ---- phase 1: big pivots, one workgroup each
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&cursors[1], 1);
+    __syncthreads();
+    const int32_t p = s_item;
+    if (p >= n_big) break;
+    const int32_t u = big[p];
+    const int32_t d = (uint32_t)u < m ? g.csr.deg[u] : 0;
+    if (d >= 2) {
+      const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
+      PivotSet s;
+      if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
+        pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
+        c += stream_rows<kUnroll>(g, L, d, wib, WAVES, table, s, rb, lane);
+      } else {
+        c += pivot_fallback(g, L, d, wib, WAVES, lane);
       }
+    }
+    __syncthreads();  // the set and s_item are rewritten for the next pivot
+  }
+  // ---- phase 2: small pivots, one wave each
+  int32_t *wt = table + wib * (TABLE_WORDS / WAVES);
+  while (true) {
+    int32_t base = 0;
+    if (lane == 0) base = atomicAdd(&cursors[0], kSmallBatch);
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, kWave));
+    if (base >= n_small) break;
+    const int32_t end = min(base + kSmallBatch, n_small);
+    for (int32_t p = base; p < end; ++p) {
+      const int32_t u = small[p];
+      if ((uint32_t)u >= m) continue;
+      const int32_t d = g.csr.deg[u];
+      if (d < 2) continue;
+      const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
+      PivotSet s;
+      if (!pivot_plan(L[0], L[d - 1], d, TABLE_WORDS / WAVES, s)) {
+        c += pivot_fallback(g, L, d, 0, 1, lane);
+        continue;
+      }
+      pivot_build<false>(wt, s, L, d, lane, kWave);
+      c += stream_rows<kUnroll>(g, L, d, 0, 1, wt, s, rb, lane);
+      wave_sync();  // the set is rebuilt for the next pivot
     }
   }
   wave_atomic_add(count, c);
-}
-
-// Workgroup-per-hub-pivot kernel: the pivot set lives in dynamic LDS of `table_words` words;
-// rows of the flattened loop are processed BLOCK at a time.
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
-    tc_pivot_block_kernel(gary_device_graph g, const int32_t *__restrict__ pivots,
-                          const int32_t *__restrict__ d_npivots, int32_t table_words,
-                          unsigned long long *count) {
-  const int32_t npivots = *d_npivots;
-  extern __shared__ int32_t dyn[];
-  __shared__ int32_t s_end[BLOCK];
-  __shared__ int64_t s_ptr[BLOCK];
-  __shared__ uint32_t s_wsum[BLOCK / 64];
-  __shared__ uint64_t s_red[BLOCK / 64];
-  int32_t *table = dyn;
-  const int lane = lane_id();
-  const int wib = threadIdx.x / kWave;
-  const uint32_t m = (uint32_t)g.csr.m;
-  uint32_t c = 0;
-  for (int64_t p = blockIdx.x; p < npivots; p += gridDim.x) {
-    const int32_t u = pivots ? pivots[p] : (int32_t)p;
-    if ((uint32_t)u >= m) continue;
-    const int32_t d = g.csr.deg[u];
-    if (d < 2) continue;
-    const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
-    const PivotSet s = pivot_plan(L[0], L[d - 1], d, table_words);
-    if (s.log_t != 0 && (1 << s.log_t) > table_words) {
-      // set too large for LDS: edge-by-edge binary search, one wave per arc
-      for (int32_t k = wib; k < d; k += BLOCK / 64) {
-        const int32_t v = L[k];
-        if ((uint32_t)v >= m) continue;
-        c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + g.csr.indPtr[v], g.csr.deg[v],
-                                            nullptr, 0, lane);
-      }
-      continue;
-    }
-    const int32_t fill_words = s.log_t == 0 ? s.words : (1 << s.log_t);
-    const int32_t fill_val = s.log_t == 0 ? 0 : -1;
-    __syncthreads();  // previous pivot's readers are done with the table
-    for (int32_t i = threadIdx.x; i < fill_words; i += BLOCK) table[i] = fill_val;
-    __syncthreads();
-    for (int32_t i = threadIdx.x; i < d; i += BLOCK) {
-      const int32_t w = L[i];
-      if (s.log_t == 0) {
-        const uint32_t off = (uint32_t)(w - s.lo);  // < 32*words for a sorted list
-        if (off < (uint32_t)s.words * 32u) atomicOr((uint32_t *)&table[off >> 5], 1u << (off & 31));
-      } else {
-        const uint32_t mask = (1u << s.log_t) - 1;
-        uint32_t h = hash_slot(w, s.log_t);
-        while (atomicCAS(&table[h], -1, w) != -1) h = (h + 1) & mask;
-      }
-    }
-    __syncthreads();
-    for (int32_t c0 = 0; c0 < d; c0 += BLOCK) {
-      const int32_t i = c0 + threadIdx.x;
-      int32_t dv = 0;
-      int64_t pv = 0;
-      if (i < d) {
-        const int32_t v = L[i];
-        if ((uint32_t)v < m) {
-          dv = g.csr.deg[v];
-          pv = g.csr.indPtr[v];
-        }
-      }
-      // block-wide inclusive scan of dv
-      uint32_t incl = wave_inclusive_scan_u32((uint32_t)dv);
-      if (lane == kWave - 1) s_wsum[wib] = incl;
-      __syncthreads();
-      uint32_t off = 0;
-      for (int k = 0; k < wib; ++k) off += s_wsum[k];
-      uint32_t total = 0;
-      for (int k = 0; k < BLOCK / 64; ++k) total += s_wsum[k];
-      incl += off;
-      s_end[threadIdx.x] = (int32_t)incl;
-      s_ptr[threadIdx.x] = pv;
-      __syncthreads();
-      for (uint32_t f = threadIdx.x; f < total; f += BLOCK) {
-        int32_t j = 0;
-        for (int32_t half = BLOCK / 2; half > 0; half >>= 1)
-          if ((uint32_t)s_end[j + half - 1] <= f) j += half;
-        j += ((uint32_t)s_end[j] <= f);
-        const uint32_t start = j ? (uint32_t)s_end[j - 1] : 0u;
-        const int32_t w = g.csr.colIdx[s_ptr[j] + (f - start)];
-        c += pivot_probe(table, s, w);
-      }
-      __syncthreads();  // s_end / s_ptr / s_wsum are rewritten by the next chunk
-    }
-  }
-  block_atomic_add<BLOCK>(count, c, s_red);
 }
 
 // ======================================================================================
 // Degree binning of pivots (device work lists)
 // ======================================================================================
 
-// bins: 0 = wave kernel with the small table, 1 = wave kernel with the large table,
-// 2 = workgroup (hub) kernel. Vertices with out-degree < 2 are dropped (no triangle has them as
-// its lowest vertex). Wave-aggregated atomics: one atomic per wave and bin.
-__global__ void bin_pivots_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small,
-                                  int32_t t_large, int32_t *__restrict__ lists, int64_t stride,
-                                  int32_t *__restrict__ counters) {
+// bins: 0 = small (2 <= d <= t_small), 1 = big (t_small < d <= t_big), 2 = hub (d > t_big).
+// Vertices with out-degree < 2 are dropped (no triangle has them as its lowest vertex). Each
+// workgroup bins a contiguous vertex range: LDS counters first, then one global atomic per bin
+// reserves the workgroup's slots, then a second pass writes the ids.
+__device__ __forceinline__ int pivot_bin(const int32_t *__restrict__ deg, int64_t v, int64_t end,
+                                         int32_t t_small, int32_t t_big) {
+  if (v >= end) return -1;
+  const int32_t d = deg[v];
+  if (d < 2) return -1;
+  return d <= t_small ? 0 : (d <= t_big ? 1 : 2);
+}
+
+__global__ void __launch_bounds__(256)
+    bin_pivots_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small, int32_t t_big,
+                      int32_t *__restrict__ lists, int64_t stride, int32_t *__restrict__ counters) {
+  __shared__ int32_t s_cnt[3], s_base[3];
   const int lane = lane_id();
-  for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; base < m;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = base + lane;
-    int bin = -1;
-    if (v < m) {
-      const int32_t d = deg[v];
-      if (d >= 2) bin = d <= t_small ? 0 : (d <= t_large ? 1 : 2);
+  const int64_t chunk = (((int64_t)m + gridDim.x - 1) / gridDim.x + 63) & ~63ll;
+  const int64_t beg = (int64_t)blockIdx.x * chunk, end = min((int64_t)m, beg + chunk);
+  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
+    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const uint64_t mask = __ballot(bin == b);
+      if (mask && lane == __ffsll((unsigned long long)mask) - 1)
+        atomicAdd(&s_cnt[b], __popcll(mask));
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    s_base[threadIdx.x] = s_cnt[threadIdx.x]
+        ? atomicAdd(&counters[threadIdx.x == 2 ? 3 : threadIdx.x], s_cnt[threadIdx.x]) : 0;
+    s_cnt[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
+    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       const uint64_t mask = __ballot(bin == b);
       if (!mask) continue;
-      int32_t pos0 = 0;
       const int leader = __ffsll((unsigned long long)mask) - 1;
-      if (lane == leader) pos0 = atomicAdd(&counters[b], __popcll(mask));
+      int32_t pos0 = 0;
+      if (lane == leader) pos0 = atomicAdd(&s_cnt[b], __popcll(mask));
       pos0 = __shfl(pos0, leader, kWave);
-      if (bin == b) lists[b * stride + pos0 + lanes_below(mask)] = (int32_t)v;
+      if (bin == b) lists[b * stride + s_base[b] + pos0 + lanes_below(mask)] = (int32_t)(base + lane);
     }
   }
 }
@@ -413,14 +526,14 @@ namespace {
 
 struct TcCfg {
   int block = 256, grid = 0, group = 64, cache_depth = 64, partition = GARY_PARTITION_INTERLEAVED;
-  int bin_small = 128, bin_hub = 1024;
+  int bin_small = 64, bin_hub = 2048;
 };
 
-constexpr int kSmallWords = 256;    // wave kernel, small pivot sets (<= 128 keys hashed)
-constexpr int kLargeWords = 2048;   // wave kernel, large pivot sets (<= 1024 keys hashed)
-constexpr int kHubWords = 8192;     // workgroup kernel (32 KiB of dynamic LDS)
-constexpr int kHubBlock = 512;
 constexpr int kPivotWaves = 4;
+constexpr int kPivotWords = 4096;  // 16 KiB: big-pivot set, or 4 x 1024-word small-pivot sets
+constexpr int kHubWaves = 8;
+constexpr int kHubWords = 8192;    // 32 KiB hub set
+constexpr int kHubGrid = 512;
 
 int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
   if (in) {
@@ -442,8 +555,8 @@ int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
   GARY_CHECK_ARG(c.partition == GARY_PARTITION_INTERLEAVED || c.partition == GARY_PARTITION_CHUNKED,
                  "launch cfg: bad partition %d", c.partition);
   GARY_CHECK_ARG(c.grid >= 0, "launch cfg: grid=%d", c.grid);
-  GARY_CHECK_ARG(c.bin_small >= 2 && c.bin_small <= kSmallWords / 2 && c.bin_hub >= c.bin_small &&
-                     c.bin_hub <= kLargeWords / 2,
+  GARY_CHECK_ARG(c.bin_small >= 2 && c.bin_small <= kPivotWords / kPivotWaves / 2 &&
+                     c.bin_hub >= c.bin_small && c.bin_hub <= kPivotWords / 2,
                  "launch cfg: bins small=%d hub=%d out of range", c.bin_small, c.bin_hub);
   const size_t lds = (size_t)(c.block / c.group) * c.cache_depth * sizeof(int32_t);
   GARY_CHECK_ARG(lds <= 65536, "launch cfg: index cache needs %zu bytes of LDS", lds);
@@ -487,33 +600,35 @@ int launch_edge(const gary_device_graph &g, const TcCfg &c, unsigned long long *
   }
 }
 
-// Pivot bins: workspace = 3 lists of m int32 + 3 counters.
+// Pivot path: bin pivots by out-degree on the device, then the hub launch (d > bin_hub, 8-wave
+// workgroups with 32 KiB sets) and the main persistent launch (big pivots per workgroup, small
+// pivots per wave). Workspace = 3 lists of m int32 + counters + cursors; no host round trip.
 int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long *cnt,
                  hipStream_t st) {
   const int32_t m = g.csr.m;
   if (m == 0 || g.csr.nnz == 0) return GARY_OK;
+  GARY_CHECK_ARG(((uintptr_t)g.csr.colIdx & 15) == 0,
+                 "pivot strategies read colIdx in 16-byte chunks: colIdx must be 16-byte aligned");
   int dev = 0;
   GARY_TRY(current_device(&dev));
   void *ws = nullptr;
   const size_t stride = ((size_t)m + 63) & ~(size_t)63;
   GARY_TRY(workspace_get(dev, st, sizeof(int32_t) * (3 * stride + 64), &ws));
   int32_t *lists = (int32_t *)ws;
+  // counters: [0] #small, [1] #big, [2] always 0, [3] #hub; cursors: [4..5] main, [6..7] hub
   int32_t *counters = lists + 3 * stride;
-  GARY_HIP(hipMemsetAsync(counters, 0, sizeof(int32_t) * 4, st));
-  const int bgrid = (int)std::min<int64_t>(((int64_t)m + 255) / 256, 2048);
+  GARY_HIP(hipMemsetAsync(counters, 0, sizeof(int32_t) * 8, st));
+  const int bgrid = (int)std::min<int64_t>(((int64_t)m + 4095) / 4096, 1024);
   bin_pivots_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, c.bin_small, c.bin_hub, lists,
                                            (int64_t)stride, counters);
   GARY_HIP(hipGetLastError());
+  // hubs: small list empty (counters[2] == 0), big list = bin 2
+  tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt);
+  GARY_HIP(hipGetLastError());
   const int grid = c.grid ? c.grid : 2048;
-  // Pivot counts stay on the device: kernels read them through the counters (no host sync).
-  tc_pivot_wave_kernel<kPivotWaves, kSmallWords><<<grid, kPivotWaves * 64, 0, st>>>(
-      g, lists, counters + 0, cnt);
-  GARY_HIP(hipGetLastError());
-  tc_pivot_wave_kernel<kPivotWaves, kLargeWords><<<grid, kPivotWaves * 64, 0, st>>>(
-      g, lists + stride, counters + 1, cnt);
-  GARY_HIP(hipGetLastError());
-  tc_pivot_block_kernel<kHubBlock><<<512, kHubBlock, kHubWords * sizeof(int32_t), st>>>(
-      g, lists + 2 * stride, counters + 2, kHubWords, cnt);
+  tc_pivot_kernel<kPivotWaves, kPivotWords><<<grid, kPivotWaves * 64, 0, st>>>(
+      g, lists, lists + stride, counters, counters + 4, cnt);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
 }

@@ -138,3 +138,19 @@ def test_rmat20_full(request):
     ctx = _ctx(g)
     for strat in (gq.BITMAP, gq.HEURISTIC, gq.CACHED_BSEARCH):
         assert gq.triangleCountHom(ctx, strat) == want, strat
+
+
+def test_hub_fallback_star():
+    """A pivot with out-degree 5000 (id orientation keeps src > dst) exceeds every LDS set and
+    goes through the edge-by-edge fallback; 2,500 triangles (center, 2i, 2i+1)."""
+    c = 5999
+    src = [c] * 5000 + list(range(0, 5000, 2))
+    dst = list(range(5000)) + list(range(1, 5001, 2))
+    g = graph.csr_from_edges(6000, np.array(src, np.int32), np.array(dst, np.int32),
+                             graph.ORIENT_ID)
+    assert int(g.deg.max()) == 5000
+    ctx = _ctx(g)
+    want = oracle.tri_count(g.indPtr, g.colIdx)
+    assert want == 2500
+    for s in ALL:
+        assert gq.triangleCountHom(ctx, s) == want, s
