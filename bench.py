@@ -40,6 +40,10 @@ def main(argv=None) -> int:
     ap.add_argument("--strategy", default="learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--only", action="store_true",
+                    help="time the --strategy only (for profiler runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (each GPU box "
@@ -52,7 +56,8 @@ def main(argv=None) -> int:
     from gary_amd import graph
     from gary_amd import graph_query as gq
     from gary_amd.device import DeviceBuffer, DeviceGraphCtx
-    from gary_amd.timing import EventTimer
+    from gary_amd.profiling import read_traffic
+    from gary_amd.timing import EventTimer, kernel_timing, kernel_timing_read
 
     t0 = time.time()
     n, s, d = graph.rmat(a.scale, a.edgefactor)
@@ -62,27 +67,37 @@ def main(argv=None) -> int:
         f"host build {time.time() - t0:.2f}s")
     row, col = g.toCOO()
     dp = g.deg.astype(np.int64)
-    W = int((dp[row] + dp[col]).sum())
-    B_alg = 4 * W + 32 * g.nnz
-    del row, col
+    per_arc = 4 * (dp[row] + dp[col]) + 32  # merge-model bytes of each arc (u -> v)
+    B_alg = int(per_arc.sum())
+    # arcs whose pivot the dominant (main pivot) kernel handles: out-degree <= 2048 (larger
+    # pivots go to the hub kernel)
+    B_alg_main = int(per_arc[dp[row] <= 2048].sum())
+    del row, col, per_arc
 
     ctx = DeviceGraphCtx(g)
     ctx.populateDevice()
     cnt = DeviceBuffer(8)
     timer = EventTimer()
 
-    def run(strategy: int, steps: int, warmup: int, cfg=None):
+    kernel_ms = {}
+
+    def run(strategy: int, steps: int, warmup: int, cfg=None, name=""):
         for _ in range(warmup):
             cnt.zero()
             gq.triangleCountHomAsync(ctx, cnt, strategy, cfg)
         gq.synchronize()
         cnt.zero()
         gq.synchronize()
+        kernel_timing(True)
         timer.start()
         for _ in range(steps):
             gq.triangleCountHomAsync(ctx, cnt, strategy, cfg)
         ms = timer.stop() / steps
         gq.synchronize()
+        launches, kms = kernel_timing_read()
+        kernel_timing(False)
+        if launches:
+            kernel_ms[name] = kms / launches
         total = int(cnt.to_array(np.uint64, 1)[0])
         assert total % steps == 0
         return ms, total // steps
@@ -90,18 +105,27 @@ def main(argv=None) -> int:
     names = {"learned": gq.AUTO_LEARNED, "heuristic": gq.HEURISTIC, "bitmap": gq.BITMAP,
              "cached_bsearch": gq.CACHED_BSEARCH, "merge": gq.MERGE, "bsearch": gq.BSEARCH}
     head = names[a.strategy]
-    ms, tri = run(head, a.steps, a.warmup)
+    ms, tri = run(head, a.steps, a.warmup, name=a.strategy)
     log(f"{a.strategy}: {ms:.3f} ms/step, triangles={tri}")
     results = {a.strategy: ms}
     for nm, sid in names.items():
-        if nm == a.strategy:
+        if nm == a.strategy or a.only:
             continue
         k = max(1, min(a.steps, 3)) if nm in ("cached_bsearch", "merge", "bsearch") else a.steps
-        ms_s, tri_s = run(sid, k, 1)
+        ms_s, tri_s = run(sid, k, 1, name=nm)
         if tri_s != tri:
             raise SystemExit(f"strategy {nm} counted {tri_s} != {tri}")
         results[nm] = ms_s
         log(f"{nm}: {ms_s:.3f} ms/step")
+
+    if a.calibrate_copy:
+        from gary_amd._lib import check as _check, lib as _glib
+        nbytes = a.calibrate_copy << 20
+        src_b, dst_b = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+        src_b.zero()
+        _check(_glib.gary_memcpy_d2d(0, dst_b.ptr, src_b.ptr, nbytes))
+        gq.synchronize()
+        log(f"calibration copy: {nbytes} bytes device to device")
 
     cpu = None
     if not a.no_cpu:
@@ -122,7 +146,13 @@ def main(argv=None) -> int:
         log(f"cpu baseline: {t_cpu:.3f}s with {threads} threads")
 
     ms = results[a.strategy]
-    achieved = B_alg / (ms * 1e-3) / 1e9
+    pivot_path = a.strategy in ("learned", "heuristic", "bitmap")
+    k_ms = kernel_ms.get(a.strategy, ms)
+    k_bytes = B_alg_main if pivot_path else B_alg
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    traffic = None
+    if pivot_path and a.scale == 20 and a.edgefactor == 16:
+        traffic = read_traffic(ROOT / "profiles" / "tc_pivot_rmat20_traffic.md")
     out = {
         "metric": "edges/s triangle count RMAT-20 (exact)",
         "value": g.nnz / (ms * 1e-3),
@@ -140,11 +170,14 @@ def main(argv=None) -> int:
                                "symmetrized, dedup, no loops, degree-oriented int32 CSR",
                    "n": n, "m": g.nnz, "triangles": tri, "strategy": a.strategy},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_model": "4*W+32*m", "B_alg": B_alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "tc_pivot_kernel<4,4096>" if pivot_path else "tc_edge_kernel",
+                     "kernel_ms": k_ms, "bytes_model": "sum over arcs (u,v) of 4*(d+(u)+d+(v))+32",
+                     "algorithmic_bytes": k_bytes, "step_B_alg": B_alg},
+        "kernel_ms": kernel_ms,
         "strategies_ms": results,
         "learned_vs_plain": results["heuristic"] / results[a.strategy]
-        if a.strategy != "heuristic" else 1.0,
+        if "heuristic" in results and a.strategy != "heuristic" else 1.0,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

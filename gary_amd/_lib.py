@@ -75,6 +75,7 @@ _SIGS = {
     "gary_free": (C.c_int, [C.c_int, _P]),
     "gary_memcpy_h2d": (C.c_int, [C.c_int, _P, _P, C.c_size_t]),
     "gary_memcpy_d2h": (C.c_int, [C.c_int, _P, _P, C.c_size_t]),
+    "gary_memcpy_d2d": (C.c_int, [C.c_int, _P, _P, C.c_size_t]),
     "gary_memset": (C.c_int, [C.c_int, _P, C.c_int, C.c_size_t]),
 }
 

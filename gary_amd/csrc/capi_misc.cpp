@@ -92,6 +92,14 @@ int gary_memcpy_d2h(int device, void *dst, const void *src, size_t bytes) {
   return GARY_OK;
 }
 
+int gary_memcpy_d2d(int device, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return GARY_OK;
+  GARY_CHECK_ARG(dst && src, "gary_memcpy_d2d: null pointer");
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return GARY_OK;
+}
+
 int gary_memset(int device, void *dst, int value, size_t bytes) {
   if (bytes == 0) return GARY_OK;
   GARY_CHECK_ARG(dst, "gary_memset: null pointer");
@@ -106,6 +114,7 @@ int gary_memset(int device, void *dst, int value, size_t bytes) {
 #include <map>
 #include <mutex>
 #include <utility>
+#include <vector>
 
 #include "workspace.h"
 
@@ -144,7 +153,78 @@ int current_device(int *device) {
   return GARY_OK;
 }
 
+namespace {
+std::mutex g_tm_mu;
+bool g_tm_on = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> *g_tm_pairs =
+    new std::vector<std::pair<hipEvent_t, hipEvent_t>>();
+std::vector<hipEvent_t> *g_tm_pool = new std::vector<hipEvent_t>();
+hipEvent_t g_tm_open = nullptr;
+
+int take_event(hipEvent_t *e) {
+  if (!g_tm_pool->empty()) {
+    *e = g_tm_pool->back();
+    g_tm_pool->pop_back();
+    return GARY_OK;
+  }
+  GARY_HIP(hipEventCreate(e));
+  return GARY_OK;
+}
+}  // namespace
+
+int timing_begin(void *stream) {
+  std::lock_guard<std::mutex> lock(g_tm_mu);
+  if (!g_tm_on) return GARY_OK;
+  GARY_TRY(take_event(&g_tm_open));
+  GARY_HIP(hipEventRecord(g_tm_open, (hipStream_t)stream));
+  return GARY_OK;
+}
+
+int timing_end(void *stream) {
+  std::lock_guard<std::mutex> lock(g_tm_mu);
+  if (!g_tm_on || !g_tm_open) return GARY_OK;
+  hipEvent_t e = nullptr;
+  GARY_TRY(take_event(&e));
+  GARY_HIP(hipEventRecord(e, (hipStream_t)stream));
+  g_tm_pairs->push_back({g_tm_open, e});
+  g_tm_open = nullptr;
+  return GARY_OK;
+}
+
 }  // namespace gary
+
+extern "C" {
+
+int gary_kernel_timing(int enable) {
+  std::lock_guard<std::mutex> lock(gary::g_tm_mu);
+  for (auto &p : *gary::g_tm_pairs) {
+    gary::g_tm_pool->push_back(p.first);
+    gary::g_tm_pool->push_back(p.second);
+  }
+  gary::g_tm_pairs->clear();
+  gary::g_tm_on = enable != 0;
+  return GARY_OK;
+}
+
+int gary_kernel_timing_read(int *launches, float *total_ms) {
+  GARY_CHECK_ARG(launches && total_ms, "gary_kernel_timing_read: null pointer");
+  std::lock_guard<std::mutex> lock(gary::g_tm_mu);
+  float tot = 0;
+  for (auto &p : *gary::g_tm_pairs) {
+    GARY_HIP(hipEventSynchronize(p.second));
+    float ms = 0;
+    GARY_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    tot += ms;
+    gary::g_tm_pool->push_back(p.first);
+    gary::g_tm_pool->push_back(p.second);
+  }
+  *launches = (int)gary::g_tm_pairs->size();
+  *total_ms = tot;
+  gary::g_tm_pairs->clear();
+  return GARY_OK;
+}
+
+}  // extern "C"
 
 // ---- event timer ------------------------------------------------------------------------------
 struct gary_timer {

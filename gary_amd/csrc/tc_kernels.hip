@@ -410,6 +410,7 @@ __global__ void __launch_bounds__(WAVES * 64)
   const int wib = threadIdx.x / kWave;
   RowBatch &rb = s_rows[wib];
   const int32_t n_small = counts[0], n_big = counts[1];
+  if (n_small == 0 && n_big == 0) return;
   const uint32_t m = (uint32_t)g.csr.m;
   uint32_t c = 0;
   // ---- phase 1: big pivots, one workgroup each
@@ -581,11 +582,13 @@ int launch_edge_g(const gary_device_graph &g, const TcCfg &c, unsigned long long
   const int grid = c.grid ? c.grid : 4096;
   const size_t lds = METHOD == kIsectCached
                          ? (size_t)(c.block / G) * c.cache_depth * sizeof(int32_t) : 0;
+  GARY_TRY(timing_begin(st));
   if (c.partition == GARY_PARTITION_CHUNKED)
     tc_edge_kernel<G, METHOD, GARY_PARTITION_CHUNKED><<<grid, c.block, lds, st>>>(g, c.cache_depth, cnt);
   else
     tc_edge_kernel<G, METHOD, GARY_PARTITION_INTERLEAVED><<<grid, c.block, lds, st>>>(g, c.cache_depth, cnt);
   GARY_HIP(hipGetLastError());
+  GARY_TRY(timing_end(st));
   return GARY_OK;
 }
 
@@ -627,9 +630,11 @@ int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long 
       g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt);
   GARY_HIP(hipGetLastError());
   const int grid = c.grid ? c.grid : 2048;
+  GARY_TRY(timing_begin(st));
   tc_pivot_kernel<kPivotWaves, kPivotWords><<<grid, kPivotWaves * 64, 0, st>>>(
       g, lists, lists + stride, counters, counters + 4, cnt);
   GARY_HIP(hipGetLastError());
+  GARY_TRY(timing_end(st));
   return GARY_OK;
 }
 

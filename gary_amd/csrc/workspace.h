@@ -10,6 +10,11 @@ namespace gary {
 // share a key must be ordered by that stream; the caller holds no lock after return.
 int workspace_get(int device, void *stream, size_t bytes, void **ptr);
 
+// Kernel timing hooks around the dominant counting kernel (no-ops unless enabled through
+// gary_kernel_timing).
+int timing_begin(void *stream);
+int timing_end(void *stream);
+
 // Current device (hipGetDevice) with error reporting.
 int current_device(int *device);
 

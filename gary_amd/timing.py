@@ -36,3 +36,19 @@ class EventTimer:
                 lib.gary_timer_destroy(self._h)
         except Exception:
             pass
+
+
+_lib.bind("gary_kernel_timing", C.c_int, [C.c_int])
+_lib.bind("gary_kernel_timing_read", C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_float)])
+
+
+def kernel_timing(enable: bool) -> None:
+    """Turn per-launch event timing of the dominant counting kernel on/off (clears the record)."""
+    check(lib.gary_kernel_timing(1 if enable else 0))
+
+
+def kernel_timing_read() -> tuple[int, float]:
+    """(launches, summed ms) of the dominant counting kernel since the last read or enable."""
+    n, ms = C.c_int(0), C.c_float(0)
+    check(lib.gary_kernel_timing_read(C.byref(n), C.byref(ms)))
+    return n.value, float(ms.value)

@@ -125,11 +125,20 @@ int gary_timer_start(gary_timer *t, void *stream);
 /* Records the stop event on `stream`, waits for it and returns the elapsed milliseconds. */
 int gary_timer_stop(gary_timer *t, void *stream, float *ms);
 
+/* Opt-in HIP-event timing of the dominant counting kernel of each gary_tc_count call (the main
+ * pivot kernel for the pivot strategies, the edge kernel for the edge-centric ones): events are
+ * recorded on the launch stream around that kernel only. enable != 0 turns it on and clears the
+ * record; gary_kernel_timing_read waits for the recorded events, returns the number of launches
+ * and their summed milliseconds, and clears the record. */
+int gary_kernel_timing(int enable);
+int gary_kernel_timing_read(int *launches, float *total_ms);
+
 /* ---- device memory helpers (so hosts without a HIP toolchain can drive the ABI) --------- */
 int gary_malloc(int device, size_t bytes, void **ptr);
 int gary_free(int device, void *ptr);
 int gary_memcpy_h2d(int device, void *dst, const void *src, size_t bytes);
 int gary_memcpy_d2h(int device, void *dst, const void *src, size_t bytes);
+int gary_memcpy_d2d(int device, void *dst, const void *src, size_t bytes);
 int gary_memset(int device, void *dst, int value, size_t bytes);
 
 /* ---- search and intersection operators (include/ops/Search.cuh, Intersection.cuh) ------- */
