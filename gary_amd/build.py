@@ -61,6 +61,22 @@ def build_lib(jobs: int = 8) -> Path:
     return LIB
 
 
+CPP_TEST_SRC = ROOT / "tests" / "cpp" / "tri_query_test.cpp"
+CPP_TEST_BIN = ROOT / "tests" / "cpp" / "tri_query_test"
+
+
+def build_cpp_test() -> Path:
+    """The façade test: plain g++ host code linked against libgary.so and the oracle."""
+    cmd = ["g++", "-std=c++17", "-O2", f"-I{ROOT / 'include'}", str(CPP_TEST_SRC),
+           f"-L{PKG}", "-lgary", f"-L{ROOT / 'oracle'}", "-loracle",
+           "-Wl,-rpath,$ORIGIN/../../gary_amd", "-Wl,-rpath,$ORIGIN/../../oracle",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(CPP_TEST_BIN)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ façade test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return CPP_TEST_BIN
+
+
 def build_oracle() -> Path:
     r = subprocess.run(["make", "-B", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
@@ -76,6 +92,7 @@ def main(argv=None) -> int:
     print(build_lib(a.jobs))
     if not a.no_oracle:
         print(build_oracle())
+        print(build_cpp_test())
     return 0
 
 
