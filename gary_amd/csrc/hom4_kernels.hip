@@ -1,0 +1,498 @@
+// hom4_kernels.hip — 4-vertex homomorphism counts (K4, diamond, 4-cycle) on gfx950.
+//
+// The reference declares graph_query::rectCountHom but never defines it
+// (include/kernels/GraphQueryHom.cuh:48), and counts neither 4-cliques nor diamonds. Definitions
+// (G simple, A its adjacency matrix):
+//   hom(K4)      = 24 * #K4
+//   hom(diamond) = Σ over arcs (a,b) of t(a,b)^2,  t(a,b) = |N(a) ∩ N(b)|
+//   hom(C4)      = tr(A^4) = 8 * #C4 + 4 * Σ_v C(d(v),2) + 2m
+//
+// Input: a symmetric CSR whose vertex ids are ranked by (degree, id) (GARY_CSR_SYMMETRIC_RANKED),
+// so the upper part of each row is the degree orientation N+(u) and the lower part N-(u).
+//
+// K4, triangles and per-edge triangle counts come from one pivot pass: pivot u builds the local
+// graph G_u on N+(u) as bitset rows (row i = local out-neighbours of the i-th vertex of N+(u)),
+// found by streaming every N+(v) through an LDS set of N+(u) that also returns element
+// positions. Then T_u = #local arcs, #K4_u = Σ over local arcs (i,j) of popc(row_i & row_j), and
+// every triangle (u,v,w) adds one to t(u,v), t(u,w) and t(v,w): the first two are row and column
+// counts of G_u (one atomic per arc per pivot), the third one atomic per triangle.
+// C4 counts wedges u-v-w with u the highest-ranked vertex (v, w < u) in per-vertex counter maps:
+// #C4 = Σ_u Σ_w C(c_u(w), 2) (Chiba-Nishizeki style vertex ordering; every 4-cycle is counted
+// once, at its highest vertex and the opposite vertex).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.h"
+#include "device_util.h"
+#include "workspace.h"
+
+namespace gary {
+namespace {
+
+constexpr int kHomWaves = 4;
+constexpr int kHomBlock = kHomWaves * 64;
+constexpr int kSetWords = 4096;    // LDS words for the position-returning set of N+(u)
+constexpr int kRowLdsWords = 4096; // LDS words for the local bitset graph
+constexpr int kMaxHomDeg = 2048;   // largest out-degree handled (LDS column counters)
+
+__device__ __forceinline__ uint32_t hslot(int32_t w, int32_t log_t) {
+  return ((uint32_t)w * 0x9E3779B1u) >> (32 - log_t);
+}
+
+// Set of a sorted list L[0..d) that returns element positions.
+// Bitmap mode (log_t == 0): coarse[cw] | cprefix[cw] | exact[E] | eprefix[E], E = nblocks<<(s-5):
+//   position(w) = eprefix[word] + popc(exact[word] & below(w)).
+// Hash mode: keys[T] | idx[T] (linear probing, empty = -1).
+struct PosSet {
+  int32_t base;
+  uint32_t span;
+  int32_t s, cwords, ewords;  // bitmap
+  int32_t log_t;              // hash
+  int32_t words;
+};
+
+__device__ bool posset_plan(int32_t lo, int32_t hi, int32_t d, int32_t budget, PosSet &ps) {
+  ps.base = lo;
+  ps.span = (uint32_t)(hi - lo) + 1u;
+  int64_t best = INT64_MAX;
+  int32_t bs = 5;
+  for (int32_t s = 5; s <= 20; ++s) {
+    const int64_t cw = ((int64_t)ps.span + (1ll << (s + 5)) - 1) >> (s + 5);
+    const int64_t w = 2 * cw + 2 * ((int64_t)d << (s - 5));
+    if (w < best) { best = w; bs = s; }
+  }
+  if (best <= budget) {
+    ps.s = bs;
+    ps.cwords = (int32_t)(((int64_t)ps.span + (1ll << (bs + 5)) - 1) >> (bs + 5));
+    ps.ewords = d << (bs - 5);
+    ps.log_t = 0;
+    ps.words = (int32_t)best;
+    return true;
+  }
+  int32_t log_t = 5;
+  while ((1 << log_t) < 2 * d) ++log_t;
+  if (2 * (1 << log_t) <= budget) {
+    ps.log_t = log_t;
+    ps.s = ps.cwords = ps.ewords = 0;
+    ps.words = 2 << log_t;
+    return true;
+  }
+  return false;
+}
+
+// Workgroup-wide build (all threads of the block call it).
+__device__ void posset_build(int32_t *t, const PosSet &ps, const int32_t *__restrict__ L,
+                             int32_t d) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int32_t i = tid; i < ps.words; i += nthr) t[i] = ps.log_t ? -1 : 0;
+  __syncthreads();
+  if (ps.log_t) {
+    const uint32_t mask = (1u << ps.log_t) - 1;
+    int32_t *keys = t, *idx = t + (1 << ps.log_t);
+    for (int32_t i = tid; i < d; i += nthr) {
+      uint32_t h = hslot(L[i], ps.log_t);
+      while (atomicCAS(&keys[h], -1, L[i]) != -1) h = (h + 1) & mask;
+      idx[h] = i;
+    }
+    __syncthreads();
+    return;
+  }
+  const int32_t s = ps.s;
+  uint32_t *coarse = (uint32_t *)t;
+  int32_t *cpre = t + ps.cwords;
+  uint32_t *exact = (uint32_t *)(t + 2 * ps.cwords);
+  int32_t *epre = t + 2 * ps.cwords + ps.ewords;
+  for (int32_t i = tid; i < d; i += nthr) {
+    const uint32_t x = (uint32_t)(L[i] - ps.base);
+    if (x < ps.span) atomicOr(&coarse[x >> (s + 5)], 1u << ((x >> s) & 31));
+  }
+  __syncthreads();
+  if (tid < kWave) {
+    uint32_t carry = 0;
+    for (int32_t c0 = 0; c0 < ps.cwords; c0 += kWave) {
+      const int32_t i = c0 + tid;
+      const uint32_t v = i < ps.cwords ? (uint32_t)__popc(coarse[i]) : 0u;
+      const uint32_t inc = wave_inclusive_scan_u32(v);
+      if (i < ps.cwords) cpre[i] = (int32_t)(carry + inc - v);
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+    }
+  }
+  __syncthreads();
+  for (int32_t i = tid; i < d; i += nthr) {
+    const uint32_t x = (uint32_t)(L[i] - ps.base);
+    if (x >= ps.span) continue;
+    const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
+    const uint32_t r = (uint32_t)cpre[cw] + (uint32_t)__popc(coarse[cw] & ((1u << b) - 1));
+    atomicOr(&exact[(r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)], 1u << (x & 31));
+  }
+  __syncthreads();
+  if (tid < kWave) {
+    uint32_t carry = 0;
+    for (int32_t c0 = 0; c0 < ps.ewords; c0 += kWave) {
+      const int32_t i = c0 + tid;
+      const uint32_t v = i < ps.ewords ? (uint32_t)__popc(exact[i]) : 0u;
+      const uint32_t inc = wave_inclusive_scan_u32(v);
+      if (i < ps.ewords) epre[i] = (int32_t)(carry + inc - v);
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+    }
+  }
+  __syncthreads();
+}
+
+// Position of w in L, or -1.
+__device__ __forceinline__ int32_t posset_find(const int32_t *t, const PosSet &ps, int32_t w) {
+  const uint32_t x = (uint32_t)(w - ps.base);
+  if (x >= ps.span) return -1;
+  if (ps.log_t == 0) {
+    const int32_t s = ps.s;
+    const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
+    const uint32_t word = (uint32_t)t[cw];
+    if (!((word >> b) & 1u)) return -1;
+    const uint32_t r = (uint32_t)t[ps.cwords + cw] + (uint32_t)__popc(word & ((1u << b) - 1));
+    const uint32_t ei = (r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5);
+    const uint32_t e = (uint32_t)t[2 * ps.cwords + ei];
+    if (!((e >> (x & 31)) & 1u)) return -1;
+    return t[2 * ps.cwords + ps.ewords + ei] + __popc(e & ((1u << (x & 31)) - 1));
+  }
+  const uint32_t mask = (1u << ps.log_t) - 1;
+  uint32_t h = hslot(w, ps.log_t);
+  while (true) {
+    const int32_t k = t[h];
+    if (k == w) return t[(1 << ps.log_t) + h];
+    if (k == -1) return -1;
+    h = (h + 1) & mask;
+  }
+}
+
+// ---- upper / lower split of the ranked symmetric CSR -------------------------------------------
+
+// cut[u] = |{w in N(u) : w < u}|; up_start[u] = indPtr[u] + cut[u]; up_len[u] = deg[u] - cut[u].
+// Flags bad[0] if ids are not in non-decreasing degree order or a row holds u itself.
+// Also sums up_len - lo_len into *asym: zero for a symmetric graph.
+__global__ void hom_split_kernel(gary_device_graph g, int64_t *__restrict__ up_start,
+                                 int32_t *__restrict__ up_len, int32_t *__restrict__ lo_len,
+                                 int32_t *__restrict__ bad, unsigned long long *asym) {
+  const int32_t m = g.csr.m;
+  int64_t diff = 0;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < m;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = g.csr.indPtr[u];
+    const int32_t d = g.csr.deg[u];
+    const int32_t *N = g.csr.colIdx + p;
+    int32_t l = 0, r = d;
+    while (l < r) {
+      const int32_t mid = (l + r) >> 1;
+      if (N[mid] < (int32_t)u) l = mid + 1;
+      else r = mid;
+    }
+    if (l < d && N[l] == (int32_t)u) *bad = 1;
+    if (u + 1 < m && g.csr.deg[u + 1] < d) *bad = 1;
+    up_start[u] = p + l;
+    up_len[u] = d - l;
+    lo_len[u] = l;
+    diff += (int64_t)(d - l) - l;
+  }
+  const uint64_t s = wave_sum_u64((uint64_t)diff);
+  if (lane_id() == 0 && s) atomicAdd(asym, (unsigned long long)s);
+}
+
+// ---- pivot pass: triangles, K4, per-edge triangle counts ---------------------------------------
+
+struct HomTotals {
+  unsigned long long tri, k4;
+};
+
+// One workgroup per pivot (atomic cursor over all vertices). rows: local bitset graph, in LDS
+// when it fits, else in this block's slice of `row_scratch` (global memory).
+__global__ void __launch_bounds__(kHomBlock)
+    hom_local_kernel(gary_device_graph g, const int64_t *__restrict__ up_start,
+                     const int32_t *__restrict__ up_len, uint32_t *__restrict__ tcount,
+                     int32_t *__restrict__ cursor, uint32_t *__restrict__ row_scratch,
+                     int64_t scratch_words, HomTotals *tot, int32_t *__restrict__ unsupported) {
+  __shared__ int32_t set[kSetWords];
+  __shared__ uint32_t rows_lds[kRowLdsWords];
+  __shared__ int32_t colcnt[kMaxHomDeg];
+  __shared__ int32_t s_item;
+  const int lane = lane_id(), wib = threadIdx.x / kWave;
+  const int32_t m = g.csr.m;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  uint64_t tri = 0, k4 = 0;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
+    __syncthreads();
+    const int32_t u = s_item;
+    if (u >= m) break;
+    const int32_t d = up_len[u];
+    if (d < 2) {
+      __syncthreads();
+      continue;
+    }
+    const int64_t ubase = up_start[u];
+    const int32_t *L = col + ubase;
+    PosSet ps;
+    const int32_t W = (d + 63) >> 6;  // u64 words per local row
+    const int64_t row_words = (int64_t)d * W * 2;
+    if (d > kMaxHomDeg || !posset_plan(L[0], L[d - 1], d, kSetWords, ps) ||
+        (row_words > kRowLdsWords && row_words > scratch_words)) {
+      if (threadIdx.x == 0) *unsupported = d;
+      __syncthreads();
+      continue;
+    }
+    uint32_t *rows = row_words <= kRowLdsWords ? rows_lds
+                                               : row_scratch + (int64_t)blockIdx.x * scratch_words;
+    for (int64_t i = threadIdx.x; i < row_words; i += kHomBlock) rows[i] = 0;
+    for (int32_t i = threadIdx.x; i < d; i += kHomBlock) colcnt[i] = 0;
+    posset_build(set, ps, L, d);  // begins and ends with a workgroup barrier
+    // stream the rows: wave wib takes local rows i = wib, wib + 4, ...; lanes stride the row
+    for (int32_t i = wib; i < d; i += kHomWaves) {
+      const int32_t v = L[i];
+      if ((uint32_t)v >= (uint32_t)m) continue;
+      const int64_t vb = up_start[v];
+      const int32_t dv = up_len[v];
+      for (int32_t p = lane; p < dv; p += kWave) {
+        const int32_t j = posset_find(set, ps, col[vb + p]);
+        if (j >= 0) {
+          atomicOr(&rows[((int64_t)i * W + (j >> 6)) * 2 + ((j >> 5) & 1)], 1u << (j & 31));
+          atomicAdd(&colcnt[j], 1);
+          atomicAdd(&tcount[vb + p], 1u);  // t(v, w) from pivot u
+        }
+      }
+    }
+    __syncthreads();
+    // row counts (t(u, v_i)) + column counts (t(u, v_j)); triangles of u = Σ row counts
+    for (int32_t i = threadIdx.x; i < d; i += kHomBlock) {
+      uint32_t rc = 0;
+      for (int32_t w = 0; w < 2 * W; ++w) rc += (uint32_t)__popc(rows[(int64_t)i * W * 2 + w]);
+      tri += rc;
+      const uint32_t add = rc + (uint32_t)colcnt[i];
+      if (add) atomicAdd(&tcount[ubase + i], add);
+    }
+    // K4: for every local arc (i, j): popc(row_i & row_j)
+    for (int32_t i = wib; i < d; i += kHomWaves) {
+      const uint32_t *ri = rows + (int64_t)i * W * 2;
+      for (int32_t j = lane; j < d; j += kWave) {
+        if (!((ri[(j >> 5)] >> (j & 31)) & 1u)) continue;
+        const uint32_t *rj = rows + (int64_t)j * W * 2;
+        uint32_t c = 0;
+        for (int32_t w = 0; w < 2 * W; ++w) c += (uint32_t)__popc(ri[w] & rj[w]);
+        k4 += c;
+      }
+    }
+    __syncthreads();
+  }
+  const uint64_t st = wave_sum_u64(tri), sk = wave_sum_u64(k4);
+  if (lane == 0) {
+    if (st) atomicAdd(&tot->tri, (unsigned long long)st);
+    if (sk) atomicAdd(&tot->k4, (unsigned long long)sk);
+  }
+}
+
+// 128-bit accumulation into {lo, hi} from 64-bit wave partials.
+__device__ __forceinline__ void add_u128(unsigned long long *acc, uint64_t v) {
+  if (!v) return;
+  const unsigned long long old = atomicAdd(&acc[0], (unsigned long long)v);
+  if (old + v < old) atomicAdd(&acc[1], 1ull);
+}
+
+// Σ over upper arcs of t^2 (128-bit), and Σ_v C(d(v), 2) (128-bit).
+__global__ void hom_sums_kernel(gary_device_graph g, const int64_t *__restrict__ up_start,
+                                const int32_t *__restrict__ up_len,
+                                const uint32_t *__restrict__ tcount,
+                                unsigned long long *sum_t2, unsigned long long *sum_c2) {
+  const int32_t m = g.csr.m;
+  const int lane = lane_id();
+  uint64_t t2 = 0, c2 = 0;
+  for (int64_t u = global_wave_id(); u < m; u += num_global_waves()) {
+    const int64_t b = up_start[u];
+    const int32_t d = up_len[u];
+    for (int32_t i = lane; i < d; i += kWave) {
+      const uint64_t t = tcount[b + i];
+      t2 += t * t;
+    }
+    if (lane == 0) {
+      const uint64_t dd = (uint64_t)g.csr.deg[u];
+      c2 += dd * (dd - (dd > 0)) / 2;
+    }
+    // flush before a lane partial could wrap (t < 2^32, so one row adds < 2^64 / 64 per lane)
+    if ((u & 1023) == 1023) {
+      add_u128(sum_t2, wave_sum_u64(t2) * (lane == 0));
+      add_u128(sum_c2, wave_sum_u64(c2) * (lane == 0));
+      t2 = c2 = 0;
+    }
+  }
+  const uint64_t a = wave_sum_u64(t2), b = wave_sum_u64(c2);
+  if (lane == 0) {
+    add_u128(sum_t2, a);
+    add_u128(sum_c2, b);
+  }
+}
+
+// ---- 4-cycles ------------------------------------------------------------------------------------
+
+// First index of N(v) with value >= key.
+__device__ __forceinline__ int32_t lower_bound(const int32_t *__restrict__ N, int32_t d,
+                                               int32_t key) {
+  int32_t l = 0, r = d;
+  while (l < r) {
+    const int32_t mid = (l + r) >> 1;
+    if (N[mid] < key) l = mid + 1;
+    else r = mid;
+  }
+  return l;
+}
+
+// One workgroup per vertex u: counts wedges u-v-w (v, w < u) into this block's dense counter
+// slice (n entries, all zero between vertices), then reads and clears them, adding C(c, 2).
+__global__ void __launch_bounds__(kHomBlock)
+    c4_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
+              uint32_t *__restrict__ counters, int32_t *__restrict__ cursor,
+              unsigned long long *c4_acc) {
+  __shared__ int32_t s_item;
+  const int lane = lane_id(), wib = threadIdx.x / kWave;
+  const int32_t m = g.csr.m;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  uint32_t *cnt = counters + (int64_t)blockIdx.x * m;
+  uint64_t acc = 0;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
+    __syncthreads();
+    const int32_t u = s_item;
+    if (u >= m) break;
+    const int32_t nl = lo_len[u];
+    const int32_t *Nu = col + g.csr.indPtr[u];
+    if (nl >= 2) {
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int32_t k = wib; k < nl; k += kHomWaves) {
+          const int32_t v = Nu[k];
+          const int32_t *Nv = col + g.csr.indPtr[v];
+          const int32_t cut = lower_bound(Nv, g.csr.deg[v], u);
+          for (int32_t p = lane; p < cut; p += kWave) {
+            const int32_t w = Nv[p];
+            if (pass == 0) {
+              atomicAdd(&cnt[w], 1u);
+            } else {
+              const uint64_t c = atomicExch(&cnt[w], 0u);
+              acc += c * (c - (c > 0)) / 2;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+  }
+  const uint64_t s = wave_sum_u64(acc);
+  if (lane == 0) add_u128(c4_acc, s);
+}
+
+}  // namespace
+}  // namespace gary
+
+extern "C" {
+
+int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *out,
+                    void *stream) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(g && out, "gary_hom4_count: null pointer");
+  GARY_CHECK_ARG((patterns & ~GARY_HOM_ALL) == 0, "gary_hom4_count: bad pattern mask %d", patterns);
+  GARY_CHECK_ARG(g->csr.m >= 0 && g->csr.nnz >= 0, "gary_hom4_count: bad sizes");
+  *out = gary_hom4_counts{};
+  const int32_t m = g->csr.m;
+  if (m == 0 || g->csr.nnz == 0 || patterns == 0) return GARY_OK;
+  GARY_CHECK_ARG(g->csr.indPtr && g->csr.colIdx && g->csr.deg, "gary_hom4_count: null arrays");
+  hipStream_t st = (hipStream_t)stream;
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  int ncu = 0;
+  GARY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid_local = std::max(1, ncu * 2);
+  const int grid_c4 = std::max(1, ncu);
+  const int64_t scratch_words = (int64_t)kMaxHomDeg * (kMaxHomDeg / 64) * 2;
+  // workspace layout (bytes, 256-aligned pieces)
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_up_start = al(sizeof(int64_t) * (size_t)m);
+  const size_t b_up_len = al(sizeof(int32_t) * (size_t)m);
+  const size_t b_lo_len = al(sizeof(int32_t) * (size_t)m);
+  const size_t b_t = al(sizeof(uint32_t) * (size_t)g->csr.nnz);
+  const size_t b_misc = 256;  // bad flag, cursors, totals, 128-bit sums
+  const bool need_local = patterns & (GARY_HOM_K4 | GARY_HOM_DIAMOND);
+  const bool need_c4 = patterns & GARY_HOM_C4;
+  const size_t b_rows = need_local ? al(sizeof(uint32_t) * (size_t)scratch_words * grid_local) : 0;
+  const size_t b_cnt = need_c4 ? al(sizeof(uint32_t) * (size_t)m * grid_c4) : 0;
+  void *ws = nullptr;
+  GARY_TRY(workspace_get(dev, (void *)3,
+                         b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_cnt, &ws));
+  char *p = (char *)ws;
+  int64_t *up_start = (int64_t *)p; p += b_up_start;
+  int32_t *up_len = (int32_t *)p; p += b_up_len;
+  int32_t *lo_len = (int32_t *)p; p += b_lo_len;
+  uint32_t *tcount = (uint32_t *)p; p += b_t;
+  char *misc = p; p += b_misc;
+  uint32_t *rows = (uint32_t *)p; p += b_rows;
+  uint32_t *cnts = (uint32_t *)p;
+  int32_t *bad = (int32_t *)misc;               // [0] bad, [1] unsupported, [2..3] cursors
+  HomTotals *tot = (HomTotals *)(misc + 64);    // tri, k4
+  unsigned long long *sums = (unsigned long long *)(misc + 128);  // t2 lo/hi, c2 lo/hi, c4 lo/hi
+  GARY_HIP(hipMemsetAsync(misc, 0, b_misc, st));
+  GARY_HIP(hipMemsetAsync(tcount, 0, b_t, st));
+  if (need_c4) GARY_HIP(hipMemsetAsync(cnts, 0, b_cnt, st));
+  hom_split_kernel<<<std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
+      *g, up_start, up_len, lo_len, bad, (unsigned long long *)(misc + 192));
+  GARY_HIP(hipGetLastError());
+  if (need_local) {
+    hom_local_kernel<<<grid_local, kHomBlock, 0, st>>>(*g, up_start, up_len, tcount, bad + 2, rows,
+                                                        scratch_words, tot, bad + 1);
+    GARY_HIP(hipGetLastError());
+  }
+  hom_sums_kernel<<<1024, 256, 0, st>>>(*g, up_start, up_len, tcount, sums, sums + 2);
+  GARY_HIP(hipGetLastError());
+  if (need_c4) {
+    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, cnts, bad + 3, sums + 4);
+    GARY_HIP(hipGetLastError());
+  }
+  int32_t flags[2];
+  unsigned long long asym = 0;
+  HomTotals ht;
+  unsigned long long sh[6];
+  GARY_HIP(hipMemcpyAsync(flags, bad, sizeof flags, hipMemcpyDeviceToHost, st));
+  GARY_HIP(hipMemcpyAsync(&asym, misc + 192, sizeof asym, hipMemcpyDeviceToHost, st));
+  GARY_HIP(hipMemcpyAsync(&ht, tot, sizeof ht, hipMemcpyDeviceToHost, st));
+  GARY_HIP(hipMemcpyAsync(sh, sums, sizeof sh, hipMemcpyDeviceToHost, st));
+  GARY_HIP(hipStreamSynchronize(st));
+  if (flags[0] || asym)
+    return set_error(GARY_ERR_INVALID,
+                     "gary_hom4_count: the graph must be symmetric, loop-free and ranked "
+                     "(ids in non-decreasing degree order: GARY_CSR_SYMMETRIC_RANKED)");
+  if (flags[1])
+    return set_error(GARY_ERR_UNSUPPORTED,
+                     "gary_hom4_count: a pivot with out-degree %d exceeds the local-graph limit %d",
+                     flags[1], kMaxHomDeg);
+  using u128 = unsigned __int128;
+  const u128 lim = (u128)UINT64_MAX;
+  const u128 t2 = ((u128)sh[1] << 64) | sh[0];
+  const u128 c2 = ((u128)sh[3] << 64) | sh[2];
+  const u128 c4 = ((u128)sh[5] << 64) | sh[4];
+  const u128 diam = 2 * t2;
+  const u128 hk4 = (u128)ht.k4 * 24;
+  const u128 hc4 = 8 * c4 + 4 * c2 + (u128)g->csr.nnz;
+  out->tri = ht.tri;
+  out->k4 = ht.k4;
+  if (need_local) {
+    if (hk4 > lim || diam > lim)
+      return set_error(GARY_ERR_OVERFLOW, "gary_hom4_count: hom(K4) or hom(diamond) exceeds 64 bits");
+    out->hom_k4 = (uint64_t)hk4;
+    out->hom_diamond = (uint64_t)diam;
+  }
+  if (need_c4) {
+    if (hc4 > lim || c4 > lim)
+      return set_error(GARY_ERR_OVERFLOW, "gary_hom4_count: hom(C4) exceeds 64 bits");
+    out->hom_c4 = (uint64_t)hc4;
+    out->c4 = (uint64_t)c4;
+  }
+  return GARY_OK;
+  GARY_API_END
+}
+
+}  // extern "C"

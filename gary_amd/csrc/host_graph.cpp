@@ -227,8 +227,10 @@ int gary_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const in
   GARY_API_BEGIN
   GARY_CHECK_ARG(n >= 0 && ntuples >= 0, "gary_csr_from_edges: n=%d ntuples=%lld", n,
                  (long long)ntuples);
-  GARY_CHECK_ARG(mode >= GARY_CSR_ORIENT_DEGREE && mode <= GARY_CSR_ORIENT_ID,
+  GARY_CHECK_ARG(mode >= GARY_CSR_ORIENT_DEGREE && mode <= GARY_CSR_SYMMETRIC_RANKED,
                  "gary_csr_from_edges: bad mode %d", mode);
+  const bool ranked = mode == GARY_CSR_SYMMETRIC_RANKED;
+  if (ranked) mode = GARY_CSR_SYMMETRIC;
   GARY_CHECK_ARG(indPtr && nnz_out && (ntuples == 0 || (src && dst)),
                  "gary_csr_from_edges: null pointer");
   const int W = num_workers();
@@ -297,6 +299,23 @@ int gary_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const in
       }
     }
   });
+  // ranked: new id of v = position of (deg v, v) in ascending order (counting sort by degree)
+  std::vector<int32_t> newid;
+  if (ranked) {
+    int64_t maxd = 0;
+    for (int32_t v = 0; v < n; ++v) maxd = std::max(maxd, deg[(size_t)v].load(std::memory_order_relaxed));
+    std::vector<int64_t> start((size_t)maxd + 2, 0);
+    for (int32_t v = 0; v < n; ++v) start[(size_t)deg[(size_t)v].load(std::memory_order_relaxed) + 1]++;
+    for (int64_t k = 0; k <= maxd; ++k) start[(size_t)k + 1] += start[(size_t)k];
+    newid.resize((size_t)n);
+    for (int32_t v = 0; v < n; ++v)
+      newid[(size_t)v] = (int32_t)start[(size_t)deg[(size_t)v].load(std::memory_order_relaxed)]++;
+    // out-degree counts move with the vertices
+    std::vector<int64_t> cnt((size_t)n);
+    for (int32_t v = 0; v < n; ++v) cnt[(size_t)newid[(size_t)v]] = fill[(size_t)v].load(std::memory_order_relaxed);
+    for (int32_t v = 0; v < n; ++v) fill[(size_t)v].store(cnt[(size_t)v], std::memory_order_relaxed);
+  }
+  auto relabel = [&](uint32_t x) -> uint32_t { return ranked ? (uint32_t)newid[x] : x; };
   indPtr[0] = 0;
   for (int32_t v = 0; v < n; ++v) {
     indPtr[v + 1] = indPtr[v] + fill[(size_t)v].load(std::memory_order_relaxed);
@@ -307,8 +326,9 @@ int gary_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const in
     for (int64_t i = b; i < e; ++i) {
       uint32_t lo = (uint32_t)(keys[(size_t)i] >> vb), hi = (uint32_t)(keys[(size_t)i] & lomask);
       if (mode == GARY_CSR_SYMMETRIC) {
-        colIdx[fill[lo].fetch_add(1, std::memory_order_relaxed)] = (int32_t)hi;
-        colIdx[fill[hi].fetch_add(1, std::memory_order_relaxed)] = (int32_t)lo;
+        const uint32_t a = relabel(lo), b = relabel(hi);
+        colIdx[fill[a].fetch_add(1, std::memory_order_relaxed)] = (int32_t)b;
+        colIdx[fill[b].fetch_add(1, std::memory_order_relaxed)] = (int32_t)a;
       } else if (lo_first(lo, hi)) {
         colIdx[fill[lo].fetch_add(1, std::memory_order_relaxed)] = (int32_t)hi;
       } else {

@@ -17,6 +17,7 @@ from ._lib import check, lib, ptr
 ORIENT_DEGREE = _lib.CSR_ORIENT_DEGREE
 SYMMETRIC = _lib.CSR_SYMMETRIC
 ORIENT_ID = _lib.CSR_ORIENT_ID
+SYMMETRIC_RANKED = 3
 
 
 @dataclass
@@ -86,7 +87,7 @@ def csr_from_edges(n: int, src: np.ndarray, dst: np.ndarray, mode: int = ORIENT_
     src = np.ascontiguousarray(src, dtype=np.int32)
     dst = np.ascontiguousarray(dst, dtype=np.int32)
     nt = int(src.shape[0])
-    cap = 2 * nt if mode == SYMMETRIC else nt
+    cap = 2 * nt if mode in (SYMMETRIC, SYMMETRIC_RANKED) else nt
     indPtr = np.empty(n + 1, np.int64)
     colIdx = np.empty(max(cap, 1), np.int32)
     nnz = C.c_int64(0)

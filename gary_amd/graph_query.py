@@ -72,3 +72,28 @@ def triangleCountLegacy(g, d_count: DeviceBuffer, stream: int | None = None) -> 
 
 def synchronize(device: int = 0) -> None:
     check(lib.gary_device_synchronize(device))
+
+
+HOM_K4, HOM_DIAMOND, HOM_C4, HOM_ALL = 1, 2, 4, 7
+
+
+class Hom4Counts(C.Structure):
+    _fields_ = [("tri", C.c_uint64), ("k4", C.c_uint64), ("hom_k4", C.c_uint64),
+                ("hom_diamond", C.c_uint64), ("hom_c4", C.c_uint64), ("c4", C.c_uint64)]
+
+
+_bind("gary_hom4_count", C.c_int, [C.POINTER(DeviceGraph), C.c_int, C.POINTER(Hom4Counts), _P])
+
+
+def hom4Count(g, patterns: int = HOM_ALL, stream: int | None = None) -> dict:
+    """4-vertex homomorphism counts of a ranked symmetric graph (graph.SYMMETRIC_RANKED):
+    dict with tri, k4, hom_k4, hom_diamond, hom_c4, c4."""
+    dg = _graph(g)
+    out = Hom4Counts()
+    check(lib.gary_hom4_count(C.byref(dg), patterns, C.byref(out), stream))
+    return {k: int(getattr(out, k)) for k, _ in Hom4Counts._fields_}
+
+
+def rectCountHom(g) -> int:
+    """hom(C4) (the reference declares graph_query::rectCountHom without defining it)."""
+    return hom4Count(g, HOM_C4)["hom_c4"]

@@ -89,12 +89,15 @@ int gary_gen_chung_lu(int32_t n, int64_t ntuples, double gamma, double max_w, ui
 enum {
   GARY_CSR_ORIENT_DEGREE = 0, /* u -> v iff (deg u, u) < (deg v, v); one arc per undirected edge */
   GARY_CSR_SYMMETRIC = 1,     /* both arcs of every undirected edge */
-  GARY_CSR_ORIENT_ID = 2      /* u -> v iff u > v (the reference's GraphCOO::orientation) */
+  GARY_CSR_ORIENT_ID = 2,     /* u -> v iff u > v (the reference's GraphCOO::orientation) */
+  GARY_CSR_SYMMETRIC_RANKED = 3 /* both arcs; vertex ids relabeled by (degree, old id) rank, so
+                                   id order is degree order and the upper part of each row is
+                                   the degree orientation (input of gary_hom4_count) */
 };
 
 /* Symmetrize, drop self-loops, dedup, then build a CSR in `mode`. Lists are sorted ascending.
  * indPtr has n+1 entries; colIdx must hold `cap` entries (ntuples suffices for the oriented
- * modes, 2*ntuples for GARY_CSR_SYMMETRIC). *nnz_out receives the stored edge count. */
+ * modes, 2*ntuples for the symmetric ones). *nnz_out receives the stored edge count. */
 int gary_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const int32_t *dst,
                         int mode, int64_t *indPtr, int32_t *colIdx, int64_t cap,
                         int64_t *nnz_out);
@@ -219,6 +222,23 @@ int gary_tc_count_sync(const gary_device_graph *g, int strategy, const gary_laun
  * Unlike the reference it counts in 64 bits and fails with GARY_ERR_OVERFLOW instead of
  * wrapping when the sum would not fit. Synchronizes `stream`. */
 int gary_tc_count_legacy(const gary_device_graph *g, uint32_t *d_count, void *stream);
+
+/* ---- 4-vertex homomorphism counts (graph_query::rectCountHom, declared but never defined at
+ *      include/kernels/GraphQueryHom.cuh:48; K4 and diamond have no reference counterpart) --- */
+enum { GARY_HOM_K4 = 1, GARY_HOM_DIAMOND = 2, GARY_HOM_C4 = 4, GARY_HOM_ALL = 7 };
+typedef struct gary_hom4_counts {
+  uint64_t tri;         /* #triangles (with GARY_HOM_K4 or GARY_HOM_DIAMOND) */
+  uint64_t k4;          /* #4-cliques */
+  uint64_t hom_k4;      /* 24 * k4 */
+  uint64_t hom_diamond; /* sum over arcs (a,b) of |N(a) ∩ N(b)|^2 */
+  uint64_t hom_c4;      /* tr(A^4) = 8 * c4 + 4 * sum_v C(d(v),2) + 2m */
+  uint64_t c4;          /* #4-cycles */
+} gary_hom4_counts;
+/* Counts on a symmetric, loop-free CSR whose ids are ranked by (degree, id)
+ * (GARY_CSR_SYMMETRIC_RANKED); any other input fails with GARY_ERR_INVALID. Synchronous on
+ * `stream`; values that exceed 64 bits fail with GARY_ERR_OVERFLOW. */
+int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *out,
+                    void *stream);
 
 #ifdef __cplusplus
 }

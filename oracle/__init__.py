@@ -107,7 +107,7 @@ def intersect_emit(a, b) -> np.ndarray:
 
 def csr_from_edges(n: int, src, dst, mode: int = 0) -> tuple[np.ndarray, np.ndarray]:
     src, dst = _a(src, np.int32), _a(dst, np.int32)
-    cap = 2 * src.size if mode == 1 else src.size
+    cap = 2 * src.size if mode in (1, 3) else src.size
     indPtr = np.empty(n + 1, np.int64)
     colIdx = np.empty(max(cap, 1), np.int32)
     nnz = C.c_int64(0)

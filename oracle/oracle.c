@@ -133,7 +133,9 @@ static int cmp_i32(const void *x, const void *y) {
 int or_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const int32_t *dst,
                       int32_t mode, int64_t *indPtr, int32_t *colIdx, int64_t cap,
                       int64_t *nnz_out) {
-  if (n < 0 || ntuples < 0 || mode < 0 || mode > 2) return -1;
+  if (n < 0 || ntuples < 0 || mode < 0 || mode > 3) return -1;
+  const int ranked = mode == 3;
+  if (ranked) mode = 1;
   /* symmetrize (Graph.cpp:19-32) folded into a canonical (lo, hi) key; loops dropped */
   uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(ntuples ? ntuples : 1));
   int64_t k = 0;
@@ -154,6 +156,21 @@ int or_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const int3
   for (int64_t e = 0; e < m; ++e) {
     deg[keys[e] >> 32]++;
     deg[keys[e] & 0xffffffffu]++;
+  }
+  if (ranked) { /* relabel: new id = rank of (deg, id); rewrite the keys canonically */
+    uint64_t *dk = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+    for (int32_t v = 0; v < n; ++v) dk[v] = ((uint64_t)deg[v] << 32) | (uint32_t)v;
+    qsort(dk, (size_t)n, sizeof(uint64_t), cmp_u64);
+    int32_t *nid = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int64_t *ndeg = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int32_t r = 0; r < n; ++r) nid[(uint32_t)(dk[r] & 0xffffffffu)] = r;
+    for (int32_t v = 0; v < n; ++v) ndeg[nid[v]] = deg[v];
+    for (int64_t e = 0; e < m; ++e) {
+      int32_t a = nid[keys[e] >> 32], b = nid[keys[e] & 0xffffffffu];
+      keys[e] = a < b ? ((uint64_t)(uint32_t)a << 32) | (uint32_t)b : ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
+    }
+    memcpy(deg, ndeg, sizeof(int64_t) * ((size_t)n + 1));
+    free(dk); free(nid); free(ndeg);
   }
   int64_t *outdeg = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
   for (int64_t e = 0; e < m; ++e) {

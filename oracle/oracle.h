@@ -51,7 +51,8 @@ int64_t or_intersect_emit(const int32_t *a, int64_t na, const int32_t *b, int64_
 /* COO -> CSR with the semantics of src/Graph.cpp:19-91 (symmetrize, toCSR_) and
  * src/Graph.cpp:34-61 (orientation), plus self-loop removal and dedup, which the reference
  * leaves out. mode: 0 = orient by (degree, id) low -> high, 1 = symmetric, 2 = orient by id
- * keeping src > dst (Graph.cpp:34-61). Returns 0, or -1 when cap is too small / bad input. */
+ * keeping src > dst (Graph.cpp:34-61), 3 = symmetric with vertices relabeled by their
+ * (degree, id) rank. Returns 0, or -1 when cap is too small / bad input. */
 int or_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const int32_t *dst,
                       int32_t mode, int64_t *indPtr, int32_t *colIdx, int64_t cap,
                       int64_t *nnz_out);

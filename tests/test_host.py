@@ -58,7 +58,7 @@ def test_generators_deterministic_and_valid():
     assert deg[:10].mean() > 20 * deg[-1000:].mean()  # heavy head
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_csr_build_equals_oracle(golden_graphs, mode):
     for rec in golden_graphs:
         n = rec["n"]
@@ -68,7 +68,7 @@ def test_csr_build_equals_oracle(golden_graphs, mode):
         assert np.array_equal(g.indPtr, ip) and np.array_equal(g.colIdx, ci), rec["name"]
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_csr_build_equals_oracle_rmat(mode):
     n, s, d = graph.rmat(14, 16, seed=77)
     g = graph.csr_from_edges(n, s, d, mode)
@@ -93,3 +93,16 @@ def test_header_documents_reference_interfaces():
     for ref in ("include/DeviceGraph.cuh", "include/ops/Search.cuh", "include/ops/Intersection.cuh",
                 "include/kernels/GraphQueryHom.cuh", "src/GraphQueryHom.cu"):
         assert ref in text
+
+
+def test_ranked_symmetric_properties(golden_graphs):
+    # ids in non-decreasing degree order; hom counts invariant under the relabel
+    import oracle as O
+    for rec in golden_graphs:
+        n = rec["n"]
+        src, dst = np.array(rec["src"], np.int32), np.array(rec["dst"], np.int32)
+        g = graph.csr_from_edges(n, src, dst, graph.SYMMETRIC_RANKED)
+        assert np.all(np.diff(g.deg) >= 0), rec["name"]
+        h = O.hom4(g.indPtr, g.colIdx)
+        assert h["hom_c4"] == rec["hom_c4"] and h["hom_diamond"] == rec["hom_diamond"]
+        assert h["k4"] == rec["k4"] and h["tri"] == rec["triangles"]
