@@ -395,6 +395,8 @@ __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
 // Phase 2: each wave takes `kSmallBatch` pivots of `small` at a time and works alone with a
 //   1/WAVES share of the table memory.
 // counts[0] = #small, counts[1] = #big; cursors[0..1] start at zero.
+// Per-graph mode (gid_of != nullptr, batched disjoint unions): each wave adds its partial count
+// to gcount[gid_of[u]] whenever the pivot's graph changes, instead of to *count.
 constexpr int kSmallBatch = 16;
 constexpr int kUnroll = 2;
 
@@ -402,7 +404,8 @@ template <int WAVES, int TABLE_WORDS>
 __global__ void __launch_bounds__(WAVES * 64)
     tc_pivot_kernel(gary_device_graph g, const int32_t *__restrict__ small,
                     const int32_t *__restrict__ big, const int32_t *__restrict__ counts,
-                    int32_t *__restrict__ cursors, unsigned long long *count) {
+                    int32_t *__restrict__ cursors, unsigned long long *count,
+                    const int32_t *__restrict__ gid_of, unsigned long long *gcount) {
   __shared__ int32_t table[TABLE_WORDS];
   __shared__ RowBatch s_rows[WAVES];
   __shared__ int32_t s_item;
@@ -430,11 +433,16 @@ __global__ void __launch_bounds__(WAVES * 64)
       } else {
         c += pivot_fallback(g, L, d, wib, WAVES, lane);
       }
+      if (gid_of) {
+        wave_atomic_add(&gcount[gid_of[u]], c);
+        c = 0;
+      }
     }
     __syncthreads();  // the set and s_item are rewritten for the next pivot
   }
   // ---- phase 2: small pivots, one wave each
   int32_t *wt = table + wib * (TABLE_WORDS / WAVES);
+  int32_t cur = -1;  // graph of the wave's partial count (per-graph mode)
   while (true) {
     int32_t base = 0;
     if (lane == 0) base = atomicAdd(&cursors[0], kSmallBatch);
@@ -446,6 +454,14 @@ __global__ void __launch_bounds__(WAVES * 64)
       if ((uint32_t)u >= m) continue;
       const int32_t d = g.csr.deg[u];
       if (d < 2) continue;
+      if (gid_of) {
+        const int32_t gu = gid_of[u];
+        if (gu != cur) {
+          if (cur >= 0) wave_atomic_add(&gcount[cur], c);
+          c = 0;
+          cur = gu;
+        }
+      }
       const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
       PivotSet s;
       if (!pivot_plan(L[0], L[d - 1], d, TABLE_WORDS / WAVES, s)) {
@@ -457,7 +473,15 @@ __global__ void __launch_bounds__(WAVES * 64)
       wave_sync();  // the set is rebuilt for the next pivot
     }
   }
-  wave_atomic_add(count, c);
+  if (!gid_of) wave_atomic_add(count, c);
+  else if (cur >= 0) wave_atomic_add(&gcount[cur], c);
+}
+
+// Zeroes `n` words (graph-capturable stand-in for a memset).
+__global__ void fill_zero_kernel(uint32_t *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 // ======================================================================================
@@ -603,39 +627,58 @@ int launch_edge(const gary_device_graph &g, const TcCfg &c, unsigned long long *
   }
 }
 
+}  // namespace
+
 // Pivot path: bin pivots by out-degree on the device, then the hub launch (d > bin_hub, 8-wave
 // workgroups with 32 KiB sets) and the main persistent launch (big pivots per workgroup, small
-// pivots per wave). Workspace = 3 lists of m int32 + counters + cursors; no host round trip.
-int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long *cnt,
-                 hipStream_t st) {
+// pivots per wave). `lists` holds 3 * stride int32, `counters` 8 int32 (zeroed here by a kernel,
+// so the sequence can be captured in a graph). No host round trip, no allocation.
+int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, int grid,
+                      int32_t *lists, size_t stride, int32_t *counters,
+                      const int32_t *gid_of, unsigned long long *gcount, int32_t ngraphs,
+                      unsigned long long *cnt, hipStream_t st, bool timed) {
   const int32_t m = g.csr.m;
   if (m == 0 || g.csr.nnz == 0) return GARY_OK;
   GARY_CHECK_ARG(((uintptr_t)g.csr.colIdx & 15) == 0,
                  "pivot strategies read colIdx in 16-byte chunks: colIdx must be 16-byte aligned");
+  // counters: [0] #small, [1] #big, [2] always 0, [3] #hub; cursors: [4..5] main, [6..7] hub
+  fill_zero_kernel<<<1, 64, 0, st>>>((uint32_t *)counters, 8);
+  GARY_HIP(hipGetLastError());
+  if (gcount) {
+    fill_zero_kernel<<<std::max<int64_t>(1, std::min<int64_t>((ngraphs * 2 + 255) / 256, 1024)), 256,
+                       0, st>>>((uint32_t *)gcount, (int64_t)ngraphs * 2);
+    GARY_HIP(hipGetLastError());
+  }
+  const int bgrid = (int)std::min<int64_t>(((int64_t)m + 4095) / 4096, 1024);
+  bin_pivots_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, bin_small, bin_hub, lists,
+                                           (int64_t)stride, counters);
+  GARY_HIP(hipGetLastError());
+  // hubs: small list empty (counters[2] == 0), big list = bin 2
+  tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount);
+  GARY_HIP(hipGetLastError());
+  if (timed) GARY_TRY(timing_begin(st));
+  tc_pivot_kernel<kPivotWaves, kPivotWords><<<grid ? grid : 2048, kPivotWaves * 64, 0, st>>>(
+      g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount);
+  GARY_HIP(hipGetLastError());
+  if (timed) GARY_TRY(timing_end(st));
+  return GARY_OK;
+}
+
+namespace {
+
+int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long *cnt,
+                 hipStream_t st) {
+  const int32_t m = g.csr.m;
+  if (m == 0 || g.csr.nnz == 0) return GARY_OK;
   int dev = 0;
   GARY_TRY(current_device(&dev));
   void *ws = nullptr;
   const size_t stride = ((size_t)m + 63) & ~(size_t)63;
   GARY_TRY(workspace_get(dev, st, sizeof(int32_t) * (3 * stride + 64), &ws));
   int32_t *lists = (int32_t *)ws;
-  // counters: [0] #small, [1] #big, [2] always 0, [3] #hub; cursors: [4..5] main, [6..7] hub
-  int32_t *counters = lists + 3 * stride;
-  GARY_HIP(hipMemsetAsync(counters, 0, sizeof(int32_t) * 8, st));
-  const int bgrid = (int)std::min<int64_t>(((int64_t)m + 4095) / 4096, 1024);
-  bin_pivots_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, c.bin_small, c.bin_hub, lists,
-                                           (int64_t)stride, counters);
-  GARY_HIP(hipGetLastError());
-  // hubs: small list empty (counters[2] == 0), big list = bin 2
-  tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
-      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt);
-  GARY_HIP(hipGetLastError());
-  const int grid = c.grid ? c.grid : 2048;
-  GARY_TRY(timing_begin(st));
-  tc_pivot_kernel<kPivotWaves, kPivotWords><<<grid, kPivotWaves * 64, 0, st>>>(
-      g, lists, lists + stride, counters, counters + 4, cnt);
-  GARY_HIP(hipGetLastError());
-  GARY_TRY(timing_end(st));
-  return GARY_OK;
+  return launch_pivot_core(g, c.bin_small, c.bin_hub, c.grid, lists, stride, lists + 3 * stride,
+                           nullptr, nullptr, 0, cnt, st, true);
 }
 
 }  // namespace

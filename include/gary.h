@@ -223,6 +223,26 @@ int gary_tc_count_sync(const gary_device_graph *g, int strategy, const gary_laun
  * wrapping when the sum would not fit. Synchronizes `stream`. */
 int gary_tc_count_legacy(const gary_device_graph *g, uint32_t *d_count, void *stream);
 
+/* ---- batched triangle counting (one persistent launch over many graphs, hipGraph) -------- */
+typedef struct gary_batch gary_batch;
+/* The batch is the disjoint union of its graphs in one oriented CSR: graph g owns the vertex ids
+ * [vbase[g], vbase[g+1]) with vbase the prefix sum of graph_vertices, and every arc stays inside
+ * its graph (checked). indPtr has Σ graph_vertices + 1 entries. All device memory is allocated
+ * here; counting allocates nothing. */
+int gary_batch_create(int device, int32_t ngraphs, const int32_t *graph_vertices, int64_t nnz,
+                      const int64_t *indPtr, const int32_t *colIdx, gary_batch **out);
+int gary_batch_destroy(gary_batch *b);
+/* Bin thresholds / grid of the pivot path; drops the captured graph. */
+int gary_batch_set_cfg(gary_batch *b, const gary_launch_cfg *cfg);
+int gary_batch_device_graph(gary_batch *b, gary_device_graph *out);
+/* Counts every graph, overwriting the batch's per-graph counters. use_graph != 0 replays the
+ * hipGraph captured at the first such call (one hipGraphLaunch); 0 enqueues the kernels. */
+int gary_batch_tc_launch(gary_batch *b, int use_graph, void *stream);
+/* Copies the per-graph counts of the last launch (waits for `stream`). */
+int gary_batch_counts(gary_batch *b, uint64_t *h_counts, void *stream);
+/* gary_batch_tc_launch(b, 1, NULL) + gary_batch_counts(b, h_counts, NULL). */
+int gary_batch_tc_count(gary_batch *b, uint64_t *h_counts);
+
 /* ---- 4-vertex homomorphism counts (graph_query::rectCountHom, declared but never defined at
  *      include/kernels/GraphQueryHom.cuh:48; K4 and diamond have no reference counterpart) --- */
 enum { GARY_HOM_K4 = 1, GARY_HOM_DIAMOND = 2, GARY_HOM_C4 = 4, GARY_HOM_ALL = 7 };

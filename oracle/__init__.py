@@ -37,6 +37,7 @@ def _load() -> C.CDLL:
         "or_intersect_emit": (I64, [P, I64, P, I64, P]),
         "or_csr_from_edges": (C.c_int, [I32, I64, P, P, I32, P, P, I64, P]),
         "or_tri_count": (U64, [I32, P, P, I32]),
+        "or_tri_count_range": (U64, [I32, I32, P, P, I32]),
         "or_omp_threads": (I32, [I32]),
         "or_hom4": (C.c_int, [I32, P, P, I32, P]),
     }
@@ -121,6 +122,12 @@ def csr_from_edges(n: int, src, dst, mode: int = 0) -> tuple[np.ndarray, np.ndar
 def tri_count(indPtr, colIdx, threads: int = 0) -> int:
     indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
     return int(lib.or_tri_count(indPtr.size - 1, indPtr.ctypes.data, colIdx.ctypes.data, threads))
+
+
+def tri_count_range(indPtr, colIdx, vbegin: int, vend: int, threads: int = 0) -> int:
+    indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
+    return int(lib.or_tri_count_range(vbegin, vend, indPtr.ctypes.data, colIdx.ctypes.data,
+                                      threads))
 
 
 def omp_threads(threads: int = 0) -> int:

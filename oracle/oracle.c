@@ -222,6 +222,11 @@ int32_t or_omp_threads(int32_t threads) {
 }
 
 uint64_t or_tri_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t threads) {
+  return or_tri_count_range(0, m, indPtr, colIdx, threads);
+}
+
+uint64_t or_tri_count_range(int32_t vbegin, int32_t vend, const int64_t *indPtr,
+                            const int32_t *colIdx, int32_t threads) {
   /* TriQueryTest.cu:17-37 with a 64-bit count */
   uint64_t count = 0;
 #ifdef _OPENMP
@@ -230,7 +235,7 @@ uint64_t or_tri_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, i
   (void)threads;
 #endif
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : count)
-  for (int32_t v = 0; v < m; ++v) {
+  for (int32_t v = vbegin; v < vend; ++v) {
     const int32_t *vAdj = colIdx + indPtr[v];
     int64_t dv = indPtr[v + 1] - indPtr[v];
     uint64_t local = 0;

@@ -61,6 +61,11 @@ int or_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const int3
  * u in N(v): |set_intersection(N(u), N(v))|. threads <= 0 uses the OpenMP default. */
 uint64_t or_tri_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t threads);
 
+/* cpuTriCount restricted to source vertices v in [vbegin, vend): the triangles whose lowest
+ * vertex lies in the range (per-graph counts of a disjoint union). */
+uint64_t or_tri_count_range(int32_t vbegin, int32_t vend, const int64_t *indPtr,
+                            const int32_t *colIdx, int32_t threads);
+
 /* Number of OpenMP threads a parallel region of `threads` (<= 0 = default) really uses. */
 int32_t or_omp_threads(int32_t threads);
 
