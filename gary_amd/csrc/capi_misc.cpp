@@ -134,8 +134,8 @@ int workspace_get(int device, void *stream, size_t bytes, void **ptr) {
   Buf &b = (*g_ws)[{device, stream}];
   if (b.bytes < bytes) {
     if (b.ptr) {
-      // the previous buffer may still be in use by work queued on this stream
-      GARY_HIP(hipStreamSynchronize((hipStream_t)stream));
+      // the previous buffer may still be in use by queued work (keys are not always streams)
+      GARY_HIP(hipDeviceSynchronize());
       GARY_HIP(hipFree(b.ptr));
       b.ptr = nullptr;
       b.bytes = 0;

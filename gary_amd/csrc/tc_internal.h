@@ -15,11 +15,42 @@ constexpr int kIsectBsearch = 0;  // plain binary search of the shorter list in 
 constexpr int kIsectCached = 1;   // two-phase search with a per-group LDS index cache
 constexpr int kIsectMerge = 2;    // merge-path merge of both lists
 
+// Per-arc routes of the selector-driven path (one byte per arc).
+constexpr uint8_t kRouteBitmap = 0;  // streamed through the pivot's LDS set
+constexpr uint8_t kRouteCached = 1;  // edge-centric two-phase cached binary search
+constexpr uint8_t kRouteMerge = 2;   // edge-centric merge path
+constexpr uint8_t kRouteSkip = 3;    // no work (empty intersection, or excluded for timing)
+constexpr int kRouteBins = 256;      // (min(15, log2 d+(u)), min(15, log2 d+(v)))
+
+// Device-resident strategy table: route per bin and predicted cost per arc (ns) per bin.
+struct RouteTable {
+  uint8_t route[kRouteBins];
+  float cost[kRouteBins];
+  uint32_t rowmask;  // bit i: some bin (i, *) routes to the bitmap
+  uint32_t pad[3];
+};
+
+// Counts with every arc routed by `table` (device pointer). Cost-based pivot split: pivots whose
+// predicted bitmap cost exceeds cost_big (ns) get a workgroup; 0 keeps the degree split.
+int launch_routed(const gary_device_graph &g, const RouteTable *table, int bin_small, int bin_hub,
+                  int grid, float cost_big, unsigned long long *cnt, hipStream_t st);
+
+// Edge-centric passes of the routed path over compacted arc lists (counts on the device).
+int launch_edge_lists(const gary_device_graph &g, const uint32_t *cached, const int32_t *n_cached,
+                      const uint32_t *merge, const int32_t *n_merge, unsigned long long *cnt,
+                      hipStream_t st);
+
+// HEURISTIC / AUTO_LEARNED strategies (selector.hip).
+int launch_selected(const gary_device_graph &g, bool learned, int bin_small, int bin_hub,
+                    int grid, unsigned long long *cnt, hipStream_t st);
+
 // Pivot path with caller-provided buffers (see tc_kernels.hip); gid_of/gcount select
 // per-graph accumulation into gcount[ngraphs] (zeroed by the launch).
 int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, int grid,
                       int32_t *lists, size_t stride, int32_t *counters,
                       const int32_t *gid_of, unsigned long long *gcount, int32_t ngraphs,
-                      unsigned long long *cnt, hipStream_t st, bool timed);
+                      unsigned long long *cnt, hipStream_t st, bool timed,
+                      const uint8_t *route = nullptr, const float *pcost = nullptr,
+                      float cost_big = 0.f, const RouteTable *table = nullptr);
 
 }  // namespace gary

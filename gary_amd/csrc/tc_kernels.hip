@@ -107,7 +107,9 @@ __device__ __forceinline__ uint32_t isect_group(const int32_t *__restrict__ A, i
 
 template <int G, int METHOD, int PART>
 __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32_t n_cache,
-                                                      unsigned long long *count) {
+                                                      unsigned long long *count,
+                                                      const uint8_t *__restrict__ route,
+                                                      uint8_t want) {
   extern __shared__ int32_t smem[];
   const int glane = lane_id() & (G - 1);
   const int gpb = blockDim.x / G;
@@ -129,7 +131,34 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
   }
   uint32_t c = 0;
   for (; e < e_end; e += step) {
+    if (route && route[e] != want) continue;
     const int32_t src = g.coo.rowIdx[e], dst = g.coo.colIdx[e];
+    if ((uint32_t)src >= m || (uint32_t)dst >= m) continue;
+    const int32_t *A = g.csr.colIdx + g.csr.indPtr[src];
+    const int32_t *B = g.csr.colIdx + g.csr.indPtr[dst];
+    c += isect_group<G, METHOD>(A, g.csr.deg[src], B, g.csr.deg[dst], cache, n_cache, glane);
+  }
+  wave_atomic_add(count, c);
+}
+
+// List-driven variant: one group per arc of `arcs[0 .. *n_arcs)` (arc = index into the CSR).
+template <int G, int METHOD>
+__global__ void __launch_bounds__(256) tc_edge_list_kernel(gary_device_graph g, int32_t n_cache,
+                                                           const uint32_t *__restrict__ arcs,
+                                                           const int32_t *__restrict__ n_arcs,
+                                                           unsigned long long *count) {
+  extern __shared__ int32_t smem[];
+  const int glane = lane_id() & (G - 1);
+  const int gpb = blockDim.x / G;
+  const int gib = threadIdx.x / G;
+  const int64_t ngroups = (int64_t)gridDim.x * gpb;
+  int32_t *cache = smem + (size_t)gib * n_cache;
+  const int64_t E = *n_arcs;
+  const uint32_t m = (uint32_t)g.csr.m;
+  uint32_t c = 0;
+  for (int64_t k = (int64_t)blockIdx.x * gpb + gib; k < E; k += ngroups) {
+    const uint32_t e = arcs[k];
+    const int32_t src = g.coo.rowIdx[e], dst = g.csr.colIdx[e];
     if ((uint32_t)src >= m || (uint32_t)dst >= m) continue;
     const int32_t *A = g.csr.colIdx + g.csr.indPtr[src];
     const int32_t *B = g.csr.colIdx + g.csr.indPtr[dst];
@@ -298,7 +327,8 @@ template <int U>
 __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
                                                 const int32_t *__restrict__ L, int32_t d,
                                                 int32_t r0, int32_t stride, const int32_t *table,
-                                                const PivotSet &ps, RowBatch &rb, int lane) {
+                                                const PivotSet &ps, RowBatch &rb, int lane,
+                                                const uint8_t *__restrict__ Lroute) {
   const uint32_t m = (uint32_t)g.csr.m;
   const int64_t nnz = g.csr.nnz;
   const int32_t *__restrict__ col = g.csr.colIdx;
@@ -307,7 +337,7 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
     const int32_t r = b + lane * stride;
     int32_t dv = 0;
     int64_t pv = 0;
-    if (r < d) {
+    if (r < d && (!Lroute || Lroute[r] == kRouteBitmap)) {
       const int32_t v = L[r];
       if ((uint32_t)v < m) {
         dv = g.csr.deg[v];
@@ -377,10 +407,12 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
 // Edge-by-edge fallback for a pivot whose set does not fit in LDS: rows r0, r0+stride, ...
 __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
                                                    const int32_t *__restrict__ L, int32_t d,
-                                                   int32_t r0, int32_t stride, int lane) {
+                                                   int32_t r0, int32_t stride, int lane,
+                                                   const uint8_t *__restrict__ Lroute) {
   const uint32_t m = (uint32_t)g.csr.m;
   uint32_t c = 0;
   for (int32_t k = r0; k < d; k += stride) {
+    if (Lroute && Lroute[k] != kRouteBitmap) continue;
     const int32_t v = L[k];
     if ((uint32_t)v >= m) continue;
     c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + g.csr.indPtr[v], g.csr.deg[v],
@@ -405,7 +437,8 @@ __global__ void __launch_bounds__(WAVES * 64)
     tc_pivot_kernel(gary_device_graph g, const int32_t *__restrict__ small,
                     const int32_t *__restrict__ big, const int32_t *__restrict__ counts,
                     int32_t *__restrict__ cursors, unsigned long long *count,
-                    const int32_t *__restrict__ gid_of, unsigned long long *gcount) {
+                    const int32_t *__restrict__ gid_of, unsigned long long *gcount,
+                    const uint8_t *__restrict__ route) {
   __shared__ int32_t table[TABLE_WORDS];
   __shared__ RowBatch s_rows[WAVES];
   __shared__ int32_t s_item;
@@ -429,9 +462,10 @@ __global__ void __launch_bounds__(WAVES * 64)
       PivotSet s;
       if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
         pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
-        c += stream_rows<kUnroll>(g, L, d, wib, WAVES, table, s, rb, lane);
+        c += stream_rows<kUnroll>(g, L, d, wib, WAVES, table, s, rb, lane,
+                                  route ? route + g.csr.indPtr[u] : nullptr);
       } else {
-        c += pivot_fallback(g, L, d, wib, WAVES, lane);
+        c += pivot_fallback(g, L, d, wib, WAVES, lane, route ? route + g.csr.indPtr[u] : nullptr);
       }
       if (gid_of) {
         wave_atomic_add(&gcount[gid_of[u]], c);
@@ -465,11 +499,12 @@ __global__ void __launch_bounds__(WAVES * 64)
       const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
       PivotSet s;
       if (!pivot_plan(L[0], L[d - 1], d, TABLE_WORDS / WAVES, s)) {
-        c += pivot_fallback(g, L, d, 0, 1, lane);
+        c += pivot_fallback(g, L, d, 0, 1, lane, route ? route + g.csr.indPtr[u] : nullptr);
         continue;
       }
       pivot_build<false>(wt, s, L, d, lane, kWave);
-      c += stream_rows<kUnroll>(g, L, d, 0, 1, wt, s, rb, lane);
+      c += stream_rows<kUnroll>(g, L, d, 0, 1, wt, s, rb, lane,
+                                route ? route + g.csr.indPtr[u] : nullptr);
       wave_sync();  // the set is rebuilt for the next pivot
     }
   }
@@ -492,17 +527,31 @@ __global__ void fill_zero_kernel(uint32_t *__restrict__ p, int64_t n) {
 // Vertices with out-degree < 2 are dropped (no triangle has them as its lowest vertex). Each
 // workgroup bins a contiguous vertex range: LDS counters first, then one global atomic per bin
 // reserves the workgroup's slots, then a second pass writes the ids.
+// Routed mode (rowmask != 0xffffffff): pivots whose degree row routes no bin to the bitmap are
+// dropped; with predicted costs (pcost) the small/big split is by cost (cost_big, ns) instead of
+// by out-degree. Hubs always go by degree.
 __device__ __forceinline__ int pivot_bin(const int32_t *__restrict__ deg, int64_t v, int64_t end,
-                                         int32_t t_small, int32_t t_big) {
+                                         int32_t t_small, int32_t t_big, uint32_t rowmask,
+                                         const float *__restrict__ pcost, float cost_big) {
   if (v >= end) return -1;
   const int32_t d = deg[v];
   if (d < 2) return -1;
-  return d <= t_small ? 0 : (d <= t_big ? 1 : 2);
+  if (!((rowmask >> min(15, 31 - __clz(d))) & 1u)) return -1;
+  if (d > t_big) return 2;
+  if (pcost && cost_big > 0.f) {
+    const float cst = pcost[v];
+    if (!(cst > 0.f)) return -1;
+    return cst > cost_big ? 1 : 0;
+  }
+  return d <= t_small ? 0 : 1;
 }
 
 __global__ void __launch_bounds__(256)
     bin_pivots_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small, int32_t t_big,
-                      int32_t *__restrict__ lists, int64_t stride, int32_t *__restrict__ counters) {
+                      int32_t *__restrict__ lists, int64_t stride, int32_t *__restrict__ counters,
+                      const RouteTable *__restrict__ table, const float *__restrict__ pcost,
+                      float cost_big) {
+  const uint32_t rowmask = table ? table->rowmask : 0xffffffffu;
   __shared__ int32_t s_cnt[3], s_base[3];
   const int lane = lane_id();
   const int64_t chunk = (((int64_t)m + gridDim.x - 1) / gridDim.x + 63) & ~63ll;
@@ -510,7 +559,7 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
-    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big);
+    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       const uint64_t mask = __ballot(bin == b);
@@ -526,7 +575,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
-    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big);
+    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       const uint64_t mask = __ballot(bin == b);
@@ -608,9 +657,11 @@ int launch_edge_g(const gary_device_graph &g, const TcCfg &c, unsigned long long
                          ? (size_t)(c.block / G) * c.cache_depth * sizeof(int32_t) : 0;
   GARY_TRY(timing_begin(st));
   if (c.partition == GARY_PARTITION_CHUNKED)
-    tc_edge_kernel<G, METHOD, GARY_PARTITION_CHUNKED><<<grid, c.block, lds, st>>>(g, c.cache_depth, cnt);
+    tc_edge_kernel<G, METHOD, GARY_PARTITION_CHUNKED><<<grid, c.block, lds, st>>>(g, c.cache_depth, cnt,
+                                                                                nullptr, 0);
   else
-    tc_edge_kernel<G, METHOD, GARY_PARTITION_INTERLEAVED><<<grid, c.block, lds, st>>>(g, c.cache_depth, cnt);
+    tc_edge_kernel<G, METHOD, GARY_PARTITION_INTERLEAVED><<<grid, c.block, lds, st>>>(g, c.cache_depth, cnt,
+                                                                                    nullptr, 0);
   GARY_HIP(hipGetLastError());
   GARY_TRY(timing_end(st));
   return GARY_OK;
@@ -636,7 +687,9 @@ int launch_edge(const gary_device_graph &g, const TcCfg &c, unsigned long long *
 int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, int grid,
                       int32_t *lists, size_t stride, int32_t *counters,
                       const int32_t *gid_of, unsigned long long *gcount, int32_t ngraphs,
-                      unsigned long long *cnt, hipStream_t st, bool timed) {
+                      unsigned long long *cnt, hipStream_t st, bool timed,
+                      const uint8_t *route, const float *pcost, float cost_big,
+                      const RouteTable *table) {
   const int32_t m = g.csr.m;
   if (m == 0 || g.csr.nnz == 0) return GARY_OK;
   GARY_CHECK_ARG(((uintptr_t)g.csr.colIdx & 15) == 0,
@@ -651,17 +704,35 @@ int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, in
   }
   const int bgrid = (int)std::min<int64_t>(((int64_t)m + 4095) / 4096, 1024);
   bin_pivots_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, bin_small, bin_hub, lists,
-                                           (int64_t)stride, counters);
+                                           (int64_t)stride, counters, table, pcost, cost_big);
   GARY_HIP(hipGetLastError());
   // hubs: small list empty (counters[2] == 0), big list = bin 2
   tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
-      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount);
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount, route);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_begin(st));
   tc_pivot_kernel<kPivotWaves, kPivotWords><<<grid ? grid : 2048, kPivotWaves * 64, 0, st>>>(
-      g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount);
+      g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_end(st));
+  return GARY_OK;
+}
+
+namespace {
+
+int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long *cnt,
+                 hipStream_t st);
+
+}  // namespace
+
+int launch_edge_lists(const gary_device_graph &g, const uint32_t *cached, const int32_t *n_cached,
+                      const uint32_t *merge, const int32_t *n_merge, unsigned long long *cnt,
+                      hipStream_t st) {
+  tc_edge_list_kernel<16, kIsectCached><<<4096, 256, (256 / 16) * 32 * sizeof(int32_t), st>>>(
+      g, 32, cached, n_cached, cnt);
+  GARY_HIP(hipGetLastError());
+  tc_edge_list_kernel<16, kIsectMerge><<<4096, 256, 0, st>>>(g, 0, merge, n_merge, cnt);
+  GARY_HIP(hipGetLastError());
   return GARY_OK;
 }
 
@@ -699,7 +770,8 @@ int gary_tc_count(const gary_device_graph *g, int strategy, const gary_launch_cf
   GARY_CHECK_ARG(strategy >= 0 && strategy < GARY_TC_NUM_STRATEGIES,
                  "gary_tc_count: unknown strategy %d", strategy);
   const bool edge = strategy == GARY_TC_MERGE || strategy == GARY_TC_CACHED_BSEARCH ||
-                    strategy == GARY_TC_BSEARCH;
+                    strategy == GARY_TC_BSEARCH || strategy == GARY_TC_HEURISTIC ||
+                    strategy == GARY_TC_AUTO_LEARNED;
   GARY_TRY(check_graph(g, edge));
   TcCfg c;
   if (strategy == GARY_TC_MERGE) c.group = 16;
@@ -711,9 +783,11 @@ int gary_tc_count(const gary_device_graph *g, int strategy, const gary_launch_cf
     case GARY_TC_BSEARCH: return launch_edge<kIsectBsearch>(*g, c, cnt, st);
     case GARY_TC_CACHED_BSEARCH: return launch_edge<kIsectCached>(*g, c, cnt, st);
     case GARY_TC_MERGE: return launch_edge<kIsectMerge>(*g, c, cnt, st);
-    case GARY_TC_BITMAP:
     case GARY_TC_HEURISTIC:
     case GARY_TC_AUTO_LEARNED:
+      return launch_selected(*g, strategy == GARY_TC_AUTO_LEARNED, c.bin_small, c.bin_hub, c.grid,
+                             cnt, st);
+    case GARY_TC_BITMAP:
     default: return launch_pivot(*g, c, cnt, st);
   }
   GARY_API_END

@@ -5,9 +5,10 @@
 
 namespace gary {
 
-// Returns (in *ptr) a device buffer of at least `bytes` bytes for (device, stream). The buffer
-// is reused by every later call with the same key and is held until the process ends. Calls that
-// share a key must be ordered by that stream; the caller holds no lock after return.
+// Returns (in *ptr) a device buffer of at least `bytes` bytes for the key (device, stream), where
+// `stream` is a stream handle or any other distinct tag. The buffer is reused by every later call
+// with the same key and is held until the process ends; growing it waits for the device. Calls
+// that share a key must be ordered (same stream); the caller holds no lock after return.
 int workspace_get(int device, void *stream, size_t bytes, void **ptr);
 
 // Kernel timing hooks around the dominant counting kernel (no-ops unless enabled through

@@ -223,6 +223,56 @@ int gary_tc_count_sync(const gary_device_graph *g, int strategy, const gary_laun
  * wrapping when the sum would not fit. Synchronizes `stream`. */
 int gary_tc_count_legacy(const gary_device_graph *g, uint32_t *d_count, void *stream);
 
+/* ---- strategy selection: routed counting, heuristic, learned MLP selector ------------------ */
+/* Arcs (u,v) fall in bin 16*min(15, log2 d+(u)) + min(15, log2 d+(v)). A route table assigns each
+ * of the 256 bins one route; counts are exact for every table. */
+enum {
+  GARY_ROUTE_BITMAP = 0,         /* streamed through the pivot's LDS set */
+  GARY_ROUTE_CACHED_BSEARCH = 1, /* edge-centric two-phase cached binary search */
+  GARY_ROUTE_MERGE = 2,          /* edge-centric merge path */
+  GARY_ROUTE_SKIP = 3,           /* not counted (per-bin timing) */
+  GARY_ROUTE_BINS = 256
+};
+/* Counts with a caller-given table (h_routes[256], optional predicted ns per arc h_costs[256]);
+ * pivots whose predicted bitmap cost exceeds cost_big ns get a workgroup (0: degree split).
+ * Requires the COO (populate). Accumulates into *d_count. */
+int gary_tc_count_routed(const gary_device_graph *g, const uint8_t *h_routes,
+                         const float *h_costs, float cost_big, const gary_launch_cfg *cfg,
+                         uint64_t *d_count, void *stream);
+/* The plain degree-threshold table used by GARY_TC_HEURISTIC. */
+int gary_selector_heuristic_table(uint8_t *h_routes);
+
+/* MLP selector: 6 bin features -> 32 -> 32 (ReLU) -> 3 predicted log2(ns per arc) for
+ * {bitmap, cached binary search, merge}. Parameters: W1[32][6] b1[32] W2[32][32] b2[32]
+ * W3[3][32] b3[3] (GARY_MLP_PARAMS floats). */
+#define GARY_MLP_IN 6
+#define GARY_MLP_HIDDEN 32
+#define GARY_MLP_OUT 3
+#define GARY_MLP_PARAMS (GARY_MLP_HIDDEN * GARY_MLP_IN + GARY_MLP_HIDDEN + \
+                         GARY_MLP_HIDDEN * GARY_MLP_HIDDEN + GARY_MLP_HIDDEN + \
+                         GARY_MLP_OUT * GARY_MLP_HIDDEN + GARY_MLP_OUT)
+/* Features of the 256 bins of a graph (h_features[256][6]); h_arcs[256] = arcs per bin
+ * (sampled estimate), may be NULL. Requires the COO. */
+int gary_selector_bin_features(const gary_device_graph *g, float *h_features, float *h_arcs);
+/* Forward pass on the device: h_y[n][3] = MLP(h_x[n][6]). */
+int gary_mlp_forward(const float *h_params, const float *h_x, int32_t n, float *h_y);
+/* Mean over n samples of 0.5 * Σ_o mask[o] (y_o - t_o)^2 and its gradient (device kernels). */
+int gary_mlp_loss_grad(const float *h_params, const float *h_x, const float *h_t,
+                       const float *h_mask, int32_t n, float *h_loss, float *h_grad);
+/* Full-batch Adam on the device-resident model; seed != 0 re-initialises it first. */
+int gary_selector_train(const float *h_x, const float *h_t, const float *h_mask, int32_t n,
+                        int32_t epochs, float lr, uint64_t seed, float *h_loss);
+/* Model parameters in/out; `trained` enables GARY_TC_AUTO_LEARNED (else it runs the heuristic
+ * table); `allowed` is a bit mask of the routes the model may pick. */
+int gary_selector_set_params(const float *h_params, int trained, float cost_big, uint32_t allowed);
+int gary_selector_get_params(float *h_params);
+/* Fixed cost (ns) of using each route at all, as up to 4 points (h_nnz[k] increasing,
+ * h_costs[k][3]) interpolated linearly in log2(1 + nnz); the learned table picks the route subset
+ * with the least predicted total. */
+int gary_selector_set_fixed_costs(int32_t npoints, const float *h_nnz, const float *h_costs);
+/* The table GARY_TC_AUTO_LEARNED would use on this graph (fused features + MLP inference). */
+int gary_selector_learned_table(const gary_device_graph *g, uint8_t *h_routes, float *h_costs);
+
 /* ---- batched triangle counting (one persistent launch over many graphs, hipGraph) -------- */
 typedef struct gary_batch gary_batch;
 /* The batch is the disjoint union of its graphs in one oriented CSR: graph g owns the vertex ids

@@ -1,0 +1,92 @@
+"""Selector: the FP64 MLP restatement against finite differences (CPU); the device forward /
+backward kernels against it, routed counts for arbitrary tables, training (GPU)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp_ref
+from gary_amd import graph, graph_query as gq, selector
+from gary_amd.device import DeviceGraphCtx
+
+
+def _rand_problem(seed, n=40):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.4, mlp_ref.NPARAMS)
+    X = rng.normal(0, 1, (n, mlp_ref.IN))
+    T = rng.normal(0, 1, (n, mlp_ref.OUT))
+    M = (rng.random((n, mlp_ref.OUT)) < 0.7).astype(np.float64)
+    return p, X, T, M
+
+
+def test_mlp_ref_gradient_matches_finite_differences():
+    p, X, T, M = _rand_problem(1)
+    _, g = mlp_ref.loss_grad(p, X, T, M)
+    eps = 1e-6
+    rng = np.random.default_rng(2)
+    for k in rng.choice(mlp_ref.NPARAMS, 80, replace=False):
+        dp = np.zeros_like(p)
+        dp[k] = eps
+        lp, _ = mlp_ref.loss_grad(p + dp, X, T, M)
+        lm, _ = mlp_ref.loss_grad(p - dp, X, T, M)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), k
+
+
+def test_mlp_layout_matches_library():
+    assert mlp_ref.NPARAMS == selector.NPARAMS
+
+
+def test_heuristic_table_shape():
+    t = selector.heuristic_table()
+    assert t.shape == (256,) and set(np.unique(t)) <= {0, 1, 2}
+    assert t[6 * 16 + 3] == selector.BITMAP     # pivots with >= 64 out-arcs: bitmap
+    assert t[2 * 16 + 2] == selector.MERGE      # both small
+    assert t[1 * 16 + 9] == selector.CACHED     # skewed
+
+
+@pytest.mark.gpu
+def test_device_mlp_matches_fp64_reference():
+    p, X, T, M = _rand_problem(3, n=200)
+    y_ref, _ = mlp_ref.forward(p, X)
+    y = selector.mlp_forward(p, X)
+    np.testing.assert_allclose(y, y_ref, rtol=1e-4, atol=1e-4)
+    loss_ref, g_ref = mlp_ref.loss_grad(p, X, T, M)
+    loss, g = selector.mlp_loss_grad(p, X, T, M)
+    assert abs(loss - loss_ref) <= 1e-4 * max(1.0, abs(loss_ref))
+    np.testing.assert_allclose(g, g_ref, rtol=2e-3, atol=2e-5)
+
+
+@pytest.mark.gpu
+def test_device_training_reduces_loss():
+    rng = np.random.default_rng(4)
+    X = rng.uniform(0, 1, (300, 6)).astype(np.float32)
+    T = np.stack([X[:, 0] * 3 - X[:, 1], X[:, 2] ** 2, np.sin(3 * X[:, 3])], 1).astype(np.float32)
+    M = np.ones_like(T)
+    l0 = selector.train(X, T, M, epochs=1, lr=1e-3, seed=11)
+    l1 = selector.train(X, T, M, epochs=1500, lr=3e-3, seed=0)
+    assert l1 < 0.1 * l0
+    # the trained parameters reproduce through the FP64 restatement
+    p = selector.get_params()
+    y_ref, _ = mlp_ref.forward(p, X)
+    np.testing.assert_allclose(selector.mlp_forward(p, X), y_ref, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.gpu
+def test_routed_counts_exact_for_any_table():
+    n, s, d = graph.rmat(14, 16, seed=31)
+    g = graph.oriented(n, s, d)
+    want = oracle.tri_count(g.indPtr, g.colIdx)
+    ctx = DeviceGraphCtx(g)
+    ctx.populateDevice()
+    rng = np.random.default_rng(5)
+    tables = [np.full(256, r, np.uint8) for r in (0, 1, 2)] + [selector.heuristic_table()]
+    tables += [rng.integers(0, 3, 256).astype(np.uint8) for _ in range(6)]
+    for t in tables:
+        assert selector.count_with_table(ctx, t) == want
+        costs = rng.uniform(0.1, 10, 256).astype(np.float32)
+        assert selector.count_with_table(ctx, t, costs, cost_big=float(rng.uniform(1, 500))) == want
+    assert gq.triangleCountHom(ctx, gq.HEURISTIC) == want
+    # skip everything: zero
+    assert selector.count_with_table(ctx, np.full(256, 3, np.uint8)) == 0
