@@ -68,6 +68,9 @@ __global__ void __launch_bounds__(256)
   __shared__ float s_cost[kRouteBins];
   __shared__ uint8_t s_r[kRouteChunk];
   __shared__ int32_t s_cnt[2], s_base[2];
+  // all-bitmap tables: the pivot kernels ignore `route`, so only the predicted costs are needed
+  const bool mixed = table->mixed != 0;
+  if (!mixed && !pcost) return;
   for (int i = threadIdx.x; i < kRouteBins; i += blockDim.x) {
     s_route[i] = table->route[i];
     s_cost[i] = table->cost[i] > 0.f ? table->cost[i] : 1.f;
@@ -111,6 +114,7 @@ __global__ void __launch_bounds__(256)
       if (u >= 0 && (lane == kWave - 1 || un != u) && cst > 0.f) atomicAdd(&pcost[u], cst);
     }
   }
+  if (!mixed) return;
   __syncthreads();
   if (threadIdx.x < 2) {
     s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]) : 0;
@@ -136,8 +140,9 @@ __global__ void __launch_bounds__(256)
 
 // ---- bin features (sampled) -------------------------------------------------------------------
 // sums[bin] = {arcs seen, overlap samples, Σ |N+(u) ∩ N+(v)| / min(d+(u), d+(v))}. Every arc whose
-// index is a multiple of `hist_every` is counted (the count is scaled back), and every multiple of
-// `ovl_every` also measures its overlap by binary search of the shorter list in the longer.
+// index is a multiple of `hist_every` is counted (the count is scaled back); every multiple of
+// `ovl_every` also measures its overlap, one wave per sampled arc: the lanes stride over the
+// shorter list and binary-search the longer one, so no thread walks a whole list alone.
 struct BinSums {
   float arcs, samples, ovl;
 };
@@ -157,24 +162,36 @@ __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int
     if ((uint32_t)u >= (uint32_t)m || (uint32_t)v >= (uint32_t)m) continue;
     const int32_t a = g.csr.deg[u], b = g.csr.deg[v];
     if (a < 2 || b < 1) continue;
+    atomicAdd(&s_arcs[ilog2c(a) * 16 + ilog2c(b)], (float)hist_every);
+  }
+  const int lane = lane_id();
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t novl = (nnz + ovl_every - 1) / ovl_every;
+  for (int64_t k = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; k < novl;
+       k += nwaves) {
+    const int64_t e = k * ovl_every;
+    const int32_t u = g.coo.rowIdx[e], v = g.coo.colIdx[e];
+    if ((uint32_t)u >= (uint32_t)m || (uint32_t)v >= (uint32_t)m) continue;  // wave-uniform
+    const int32_t a = g.csr.deg[u], b = g.csr.deg[v];
+    if (a < 2 || b < 1) continue;
     const int bin = ilog2c(a) * 16 + ilog2c(b);
-    atomicAdd(&s_arcs[bin], (float)hist_every);
-    if (e % ovl_every == 0) {
-      const int32_t *A = g.csr.colIdx + g.csr.indPtr[u], *B = g.csr.colIdx + g.csr.indPtr[v];
-      int32_t na = a, nb = b;
-      if (na > nb) { const int32_t *t = A; A = B; B = t; int32_t tn = na; na = nb; nb = tn; }
-      int32_t hit = 0;
-      for (int32_t i = 0; i < na; ++i) {
-        int32_t l = 0, r = nb - 1;
-        const int32_t key = A[i];
-        while (l <= r) {
-          const int32_t mid = (l + r) >> 1;
-          const int32_t x = B[mid];
-          if (x == key) { ++hit; break; }
-          if (x < key) l = mid + 1;
-          else r = mid - 1;
-        }
+    const int32_t *A = g.csr.colIdx + g.csr.indPtr[u], *B = g.csr.colIdx + g.csr.indPtr[v];
+    int32_t na = a, nb = b;
+    if (na > nb) { const int32_t *t = A; A = B; B = t; int32_t tn = na; na = nb; nb = tn; }
+    uint32_t hit = 0;
+    for (int32_t i = lane; i < na; i += kWave) {
+      int32_t l = 0, r = nb - 1;
+      const int32_t key = A[i];
+      while (l <= r) {
+        const int32_t mid = (l + r) >> 1;
+        const int32_t x = B[mid];
+        if (x == key) { ++hit; break; }
+        if (x < key) l = mid + 1;
+        else r = mid - 1;
       }
+    }
+    hit = wave_sum_u32(hit);
+    if (lane == 0) {
       atomicAdd(&s_samp[bin], 1.f);
       atomicAdd(&s_ovl[bin], (float)hit / (float)na);
     }
@@ -343,13 +360,16 @@ __device__ __forceinline__ float fixed_cost(const FixedCosts &fc, float lx, int 
 
 __global__ void __launch_bounds__(kRouteBins)
     mlp_table_kernel(const float *__restrict__ P, const BinSums *__restrict__ sums, float log_nnz,
-                     float nnz, uint32_t allowed, FixedCosts fixed,
+                     float nnz, uint32_t allowed, FixedCosts fixed, float margin,
                      RouteTable *__restrict__ table) {
   __shared__ float sP[kParams];
   __shared__ float s_tot[8];
   for (int i = threadIdx.x; i < kParams; i += blockDim.x) sP[i] = P[i];
   if (threadIdx.x < 8) s_tot[threadIdx.x] = 0.f;
-  if (threadIdx.x == 0) table->rowmask = 0;
+  if (threadIdx.x == 0) {
+    table->rowmask = 0;
+    table->mixed = 0;
+  }
   __syncthreads();
   const int b = threadIdx.x;
   float x[kIn], h1[kHid], h2[kHid], y[kOut], c[kOut];
@@ -379,6 +399,10 @@ __global__ void __launch_bounds__(kRouteBins)
       bestR = R;
     }
   }
+  if ((allowed & 1u) && bestR != 1u) {  // leave the bitmap only for a predicted gain > margin
+    const float tb = s_tot[1] + fmaxf(0.f, fixed_cost(fixed, log_nnz, 0));
+    if (bestT > (1.f - margin) * tb) bestR = 1u;
+  }
   int best = 0;
   float by = INFINITY;
   for (int o = 0; o < kOut; ++o)
@@ -394,6 +418,7 @@ __global__ void __launch_bounds__(kRouteBins)
     for (int k = 0; k < 64; ++k)
       if ((bm >> k) & 1ull) rows |= 1u << ((b + k) >> 4);
     atomicOr(&table->rowmask, rows);
+    if (bm != ~0ull) atomicOr(&table->mixed, 1u);
   }
 }
 
@@ -408,6 +433,7 @@ struct Model {
   int64_t steps = 0;
   float cost_big = 0.f;  // ns; pivots above get a workgroup (0 = degree split)
   uint32_t allowed = 7;
+  float margin = 0.f;  // predicted relative gain needed to route arcs away from the bitmap
 };
 
 struct DeviceState {
@@ -424,7 +450,10 @@ void set_rowmask(RouteTable &t) {
   t.rowmask = 0;
   for (int b = 0; b < kRouteBins; ++b)
     if (t.route[b] == kRouteBitmap) t.rowmask |= 1u << (b >> 4);
-  t.pad[0] = t.pad[1] = t.pad[2] = 0;
+  t.mixed = 0;
+  for (int b = 0; b < kRouteBins; ++b)
+    if (t.route[b] != kRouteBitmap) t.mixed = 1;
+  t.pad[0] = t.pad[1] = 0;
 }
 
 void heuristic_table_host(RouteTable &t) {
@@ -488,7 +517,7 @@ __global__ void fill_zero_kernel_sel(uint32_t *__restrict__ p, int32_t n) {
 int compute_bin_sums(const gary_device_graph &g, BinSums *sums, hipStream_t st) {
   GARY_HIP(hipMemsetAsync(sums, 0, sizeof(BinSums) * kRouteBins, st));
   if (g.csr.nnz == 0) return GARY_OK;
-  bin_features_kernel<<<512, 256, 0, st>>>(g, 8, 512, sums);
+  bin_features_kernel<<<1024, 256, 0, st>>>(g, 8, 2048, sums);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
 }
@@ -545,7 +574,8 @@ int launch_selected(const gary_device_graph &g, bool learned, int bin_small, int
   GARY_TRY(compute_bin_sums(g, s->sums, st));
   mlp_table_kernel<<<1, kRouteBins, 0, st>>>(s->model.P, s->sums,
                                              log2f(1.f + (float)g.csr.nnz), (float)g.csr.nnz,
-                                             s->model.allowed, fixed_of(s->model), s->scratch);
+                                             s->model.allowed, fixed_of(s->model),
+                                             s->model.margin, s->scratch);
   GARY_HIP(hipGetLastError());
   return launch_routed(g, s->scratch, bin_small, bin_hub, grid, s->model.cost_big, cnt, st);
 }
@@ -613,11 +643,12 @@ int gary_selector_bin_features(const gary_device_graph *g, float *h_features, fl
 }
 
 int gary_selector_set_params(const float *h_params, int trained, float cost_big,
-                             uint32_t allowed) {
+                             uint32_t allowed, float margin) {
   GARY_API_BEGIN
   using namespace gary;
   GARY_CHECK_ARG(h_params, "gary_selector_set_params: null pointer");
   GARY_CHECK_ARG(allowed >= 1 && allowed <= 7, "gary_selector_set_params: allowed mask %u", allowed);
+  GARY_CHECK_ARG(margin >= 0.f && margin < 1.f, "gary_selector_set_params: margin %g", margin);
   int dev = 0;
   GARY_TRY(current_device(&dev));
   std::lock_guard<std::mutex> lock(g_mu);
@@ -631,6 +662,7 @@ int gary_selector_set_params(const float *h_params, int trained, float cost_big,
   s->model.trained = trained != 0;
   s->model.cost_big = cost_big;
   s->model.allowed = allowed;
+  s->model.margin = margin;
   return GARY_OK;
   GARY_API_END
 }
@@ -795,7 +827,7 @@ int gary_selector_learned_table(const gary_device_graph *g, uint8_t *h_routes, f
   GARY_TRY(compute_bin_sums(*g, s->sums, nullptr));
   mlp_table_kernel<<<1, kRouteBins>>>(s->model.P, s->sums, log2f(1.f + (float)g->csr.nnz),
                                       (float)g->csr.nnz, s->model.allowed, fixed_of(s->model),
-                                      s->scratch);
+                                      s->model.margin, s->scratch);
   GARY_HIP(hipGetLastError());
   RouteTable t;
   GARY_HIP(hipMemcpy(&t, s->scratch, sizeof t, hipMemcpyDeviceToHost));

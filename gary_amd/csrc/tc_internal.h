@@ -27,7 +27,8 @@ struct RouteTable {
   uint8_t route[kRouteBins];
   float cost[kRouteBins];
   uint32_t rowmask;  // bit i: some bin (i, *) routes to the bitmap
-  uint32_t pad[3];
+  uint32_t mixed;    // nonzero: some bin routes elsewhere than the bitmap
+  uint32_t pad[2];
 };
 
 // Counts with every arc routed by `table` (device pointer). Cost-based pivot split: pivots whose

@@ -438,8 +438,9 @@ __global__ void __launch_bounds__(WAVES * 64)
                     const int32_t *__restrict__ big, const int32_t *__restrict__ counts,
                     int32_t *__restrict__ cursors, unsigned long long *count,
                     const int32_t *__restrict__ gid_of, unsigned long long *gcount,
-                    const uint8_t *__restrict__ route) {
+                    const uint8_t *__restrict__ route, const RouteTable *__restrict__ rtable) {
   __shared__ int32_t table[TABLE_WORDS];
+  if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   __shared__ RowBatch s_rows[WAVES];
   __shared__ int32_t s_item;
   const int lane = lane_id();
@@ -708,11 +709,11 @@ int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, in
   GARY_HIP(hipGetLastError());
   // hubs: small list empty (counters[2] == 0), big list = bin 2
   tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
-      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount, route);
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount, route, table);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_begin(st));
   tc_pivot_kernel<kPivotWaves, kPivotWords><<<grid ? grid : 2048, kPivotWaves * 64, 0, st>>>(
-      g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route);
+      g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_end(st));
   return GARY_OK;

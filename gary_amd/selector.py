@@ -15,6 +15,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
+from . import graph_query as _gq  # noqa: F401  (binds gary_tc_count)
 from ._lib import DeviceGraph, LaunchCfg, check, lib, ptr
 from .device import DeviceBuffer, DeviceGraphCtx
 from .timing import EventTimer
@@ -34,7 +35,7 @@ _lib.bind("gary_selector_bin_features", C.c_int, [C.POINTER(DeviceGraph), _P, _P
 _lib.bind("gary_mlp_forward", C.c_int, [_P, _P, C.c_int32, _P])
 _lib.bind("gary_mlp_loss_grad", C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P])
 _lib.bind("gary_selector_train", C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _F, C.c_uint64, _P])
-_lib.bind("gary_selector_set_params", C.c_int, [_P, C.c_int, _F, C.c_uint32])
+_lib.bind("gary_selector_set_params", C.c_int, [_P, C.c_int, _F, C.c_uint32, _F])
 _lib.bind("gary_selector_get_params", C.c_int, [_P])
 _lib.bind("gary_selector_set_fixed_costs", C.c_int, [C.c_int32, _P, _P])
 _lib.bind("gary_selector_learned_table", C.c_int, [C.POINTER(DeviceGraph), _P, _P])
@@ -99,6 +100,20 @@ def time_table(g, routes, costs=None, cost_big=0.0, repeats=5, cfg=None) -> floa
     t.start()
     for _ in range(repeats):
         count_with_table(g, routes, costs, cost_big, cfg, d)
+    return t.stop() / repeats
+
+
+def time_strategy(g, strategy: int, repeats: int = 5, cfg: LaunchCfg | None = None) -> float:
+    """Milliseconds per count with a gary_tc_count strategy (features and table included)."""
+    dg = _dg(g)
+    d = DeviceBuffer(8)
+    d.zero()
+    pc = C.byref(cfg) if cfg else None
+    check(lib.gary_tc_count(C.byref(dg), strategy, pc, d.ptr, None))
+    t = EventTimer()
+    t.start()
+    for _ in range(repeats):
+        check(lib.gary_tc_count(C.byref(dg), strategy, pc, d.ptr, None))
     return t.stop() / repeats
 
 
@@ -168,10 +183,11 @@ def get_params() -> np.ndarray:
     return p
 
 
-def set_params(p, trained: bool = True, cost_big: float = 0.0, allowed: int = 7) -> None:
+def set_params(p, trained: bool = True, cost_big: float = 0.0, allowed: int = 7,
+               margin: float = 0.0) -> None:
     p = np.ascontiguousarray(p, np.float32)
     assert p.size == NPARAMS
-    check(lib.gary_selector_set_params(ptr(p), 1 if trained else 0, cost_big, allowed))
+    check(lib.gary_selector_set_params(ptr(p), 1 if trained else 0, cost_big, allowed, margin))
 
 
 def learned_table(g) -> tuple[np.ndarray, np.ndarray]:
@@ -225,7 +241,7 @@ def load(path: Path = MODEL_PATH) -> dict | None:
         return None
     doc = json.loads(p.read_text())
     set_params(np.array(doc["params"], np.float32), True, float(doc.get("cost_big", 0.0)),
-               int(doc.get("allowed", 7)))
+               int(doc.get("allowed", 7)), float(doc.get("margin", 0.0)))
     if doc.get("fixed_nnz"):
         set_fixed_costs(doc["fixed_nnz"], doc["fixed_costs"])
     return doc

@@ -264,7 +264,10 @@ int gary_selector_train(const float *h_x, const float *h_t, const float *h_mask,
                         int32_t epochs, float lr, uint64_t seed, float *h_loss);
 /* Model parameters in/out; `trained` enables GARY_TC_AUTO_LEARNED (else it runs the heuristic
  * table); `allowed` is a bit mask of the routes the model may pick. */
-int gary_selector_set_params(const float *h_params, int trained, float cost_big, uint32_t allowed);
+/* margin: the learned table leaves the all-bitmap routing only when another route subset is
+ * predicted to save more than this fraction of its time (0 <= margin < 1). */
+int gary_selector_set_params(const float *h_params, int trained, float cost_big, uint32_t allowed,
+                             float margin);
 int gary_selector_get_params(float *h_params);
 /* Fixed cost (ns) of using each route at all, as up to 4 points (h_nnz[k] increasing,
  * h_costs[k][3]) interpolated linearly in log2(1 + nnz); the learned table picks the route subset
