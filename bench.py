@@ -6,8 +6,13 @@ launch of the counting path, including the per-call pivot binning). Prints one J
 edges/s of the headline strategy, per-strategy times, the HBM roofline fraction of the counting
 pass under the merge byte model B_alg = 4*W + 32*m (W = sum over arcs (u,v) of d+(u) + d+(v)),
 and the reference's CPU algorithm (cpuTriCount restated in the oracle) timed on this host.
+Beside the headline: the learned selector against the plain degree-threshold heuristic on R-MAT
+20 and on the Chung-Lu graph (4,194,304 vertices, ~64M edges), and the batched form (2,000
+graphs of uneven size, one hipGraph replay per step, per-graph counts). Launch knobs come from
+the autotuner's file (gary_amd/tuned.json) where it has an entry for the workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 20] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 20] [--no-cpu] [--no-batch]
+                    [--no-c3] [--strategy learned|heuristic|bitmap|...] [--only]
 """
 from __future__ import annotations
 
@@ -44,6 +49,9 @@ def main(argv=None) -> int:
                     help="time the --strategy only (for profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 2,000-graph leg")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the selector leg on the Chung-Lu graph (4,194,304 vertices)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (each GPU box "
@@ -53,11 +61,16 @@ def main(argv=None) -> int:
 
     import numpy as np
 
-    from gary_amd import graph
+    from gary_amd import autotune, graph, selector
     from gary_amd import graph_query as gq
     from gary_amd.device import DeviceBuffer, DeviceGraphCtx
     from gary_amd.profiling import read_traffic
     from gary_amd.timing import EventTimer, kernel_timing, kernel_timing_read
+
+    model = selector.load()
+    if model is None and a.strategy == "learned":
+        raise SystemExit(f"no trained selector at {selector.MODEL_PATH}")
+    wl = f"rmat{a.scale}" if a.edgefactor == 16 else None
 
     t0 = time.time()
     n, s, d = graph.rmat(a.scale, a.edgefactor)
@@ -82,6 +95,8 @@ def main(argv=None) -> int:
     kernel_ms = {}
 
     def run(strategy: int, steps: int, warmup: int, cfg=None, name=""):
+        if cfg is None and wl:
+            cfg = autotune.cfg_for(wl, name)
         for _ in range(warmup):
             cnt.zero()
             gq.triangleCountHomAsync(ctx, cnt, strategy, cfg)
@@ -127,6 +142,87 @@ def main(argv=None) -> int:
         gq.synchronize()
         log(f"calibration copy: {nbytes} bytes device to device")
 
+    def pivot_bytes(gx):
+        """Σ over arcs (u,v) of 4*(d+(u) + d+(v)) + 32, and the part whose pivot has
+        out-degree <= 2048 (the main pivot kernel's share)."""
+        dpx = gx.deg.astype(np.int64)
+        rowx = np.repeat(np.arange(gx.m, dtype=np.int64), dpx)
+        du = dpx[rowx]
+        per = 4 * (du + dpx[gx.colIdx]) + 32
+        return int(per.sum()), int(per[du <= 2048].sum())
+
+    selector_legs = {}
+    if not a.only:
+        selector_legs[f"rmat{a.scale}"] = {
+            "heuristic_ms": results.get("heuristic"), "learned_ms": results.get("learned"),
+            "bitmap_ms": results.get("bitmap")}
+    if not a.no_c3 and not a.only:
+        t0 = time.time()
+        gc = graph.oriented(*graph.chung_lu_4m())
+        log(f"chung_lu_4m: m={gc.nnz} host build {time.time() - t0:.2f}s")
+        cctx = DeviceGraphCtx(gc)
+        cctx.populateDevice()
+        leg = {}
+        ref = None
+        for nm in ("heuristic", "learned", "bitmap"):
+            cfg = autotune.cfg_for("c3", nm)
+            for _ in range(2):
+                cnt.zero()
+                gq.triangleCountHomAsync(cctx, cnt, names[nm], cfg)
+            gq.synchronize()
+            cnt.zero()
+            k = max(1, min(a.steps, 10))
+            timer.start()
+            for _ in range(k):
+                gq.triangleCountHomAsync(cctx, cnt, names[nm], cfg)
+            leg[f"{nm}_ms"] = timer.stop() / k
+            tot = int(cnt.to_array(np.uint64, 1)[0]) // k
+            if ref is not None and tot != ref:
+                raise SystemExit(f"chung_lu_4m: {nm} counted {tot} != {ref}")
+            ref = tot
+            log(f"chung_lu_4m {nm}: {leg[f'{nm}_ms']:.3f} ms")
+        leg["triangles"] = ref
+        leg["m"] = gc.nnz
+        selector_legs["chung_lu_4m"] = leg
+        del cctx, gc
+    for leg in selector_legs.values():
+        if leg.get("heuristic_ms") and leg.get("learned_ms"):
+            leg["learned_vs_plain"] = leg["heuristic_ms"] / leg["learned_ms"]
+
+    batched = None
+    if not a.no_batch and not a.only:
+        from gary_amd import batch as gb
+        t0 = time.time()
+        gB, vbase = gb.build_union(gb.batch_specs(2000))
+        log(f"batch2000: m={gB.nnz} host build {time.time() - t0:.2f}s")
+        bB_all, _ = pivot_bytes(gB)
+        B = gb.Batch(gB, vbase)
+        B.set_cfg(autotune.cfg_for("batch2000", "bitmap"))
+        per_graph = B.count()
+        one = gq.triangleCountHom(B.deviceGraph(), gq.BITMAP)
+        if int(per_graph.sum()) != one:
+            raise SystemExit(f"batch2000: per-graph sum {int(per_graph.sum())} != union count {one}")
+        for _ in range(a.warmup):
+            B.launch(use_graph=True)
+        gq.synchronize()
+        timer.start()
+        for _ in range(a.steps):
+            B.launch(use_graph=True)
+        ms_b = timer.stop() / a.steps
+        if not np.array_equal(B.counts(), per_graph):
+            raise SystemExit("batch2000: replayed counts differ")
+        ach_b = bB_all / (ms_b * 1e-3) / 1e9
+        batched = {"workload": "2,000 seeded graphs, n ~ U[64, 20000], avg degree ~ U[4, 64], "
+                               "half ER half R-MAT(.57,.19,.19,.05), degree-oriented int32 CSR, "
+                               "one hipGraph replay per step, uint64 count per graph",
+                   "graphs": 2000, "m": gB.nnz, "ms_per_launch": ms_b,
+                   "edges_per_s": gB.nnz / (ms_b * 1e-3), "triangles": one,
+                   "roofline": {"achieved": ach_b, "frac": ach_b / HBM_PEAK_GBS,
+                                "algorithmic_bytes": bB_all}}
+        log(f"batch2000: {ms_b:.3f} ms per launch, {gB.nnz / (ms_b * 1e-3):.3e} edges/s")
+        B.close()
+        del gB
+
     cpu = None
     if not a.no_cpu:
         import oracle
@@ -154,7 +250,7 @@ def main(argv=None) -> int:
     if pivot_path and a.scale == 20 and a.edgefactor == 16:
         traffic = read_traffic(ROOT / "profiles" / "tc_pivot_rmat20_traffic.md")
     out = {
-        "metric": "edges/s triangle count RMAT-20 (exact)",
+        "metric": "edges/s triangle count RMAT-20 (exact); learned-vs-plain GPU speedup, %HBM",
         "value": g.nnz / (ms * 1e-3),
         "unit": "edges/s",
         "n_gpus": 1,
@@ -178,6 +274,11 @@ def main(argv=None) -> int:
         "strategies_ms": results,
         "learned_vs_plain": results["heuristic"] / results[a.strategy]
         if "heuristic" in results and a.strategy != "heuristic" else 1.0,
+        "selector": selector_legs,
+        "batched": batched,
+        "tuned_cfg": {k: (None if v is None else {f: getattr(v, f) for f in autotune.FIELDS})
+                      for k, v in ((nm, autotune.cfg_for(wl, nm) if wl else None)
+                                   for nm in results)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -79,11 +79,13 @@ gary_device_graph view(const gary_batch *b) {
 
 int enqueue(gary_batch *b, hipStream_t st) {
   const gary_device_graph g = view(b);
-  const int bin_small = b->cfg.bin_small ? b->cfg.bin_small : 64;
-  const int bin_hub = b->cfg.bin_hub ? b->cfg.bin_hub : 2048;
-  return gary::launch_pivot_core(g, bin_small, bin_hub, b->cfg.grid, b->lists, b->stride,
-                                 b->lists + 3 * b->stride, b->gid_of, b->gcount, b->ngraphs,
-                                 nullptr, st, false);
+  gary::PivotCfg pc;
+  if (b->cfg.bin_small) pc.bin_small = b->cfg.bin_small;
+  if (b->cfg.bin_hub) pc.bin_hub = b->cfg.bin_hub;
+  pc.grid = b->cfg.grid;
+  if (b->cfg.block) pc.waves = b->cfg.block == 128 ? 2 : b->cfg.block == 512 ? 8 : 4;
+  return gary::launch_pivot_core(g, pc, b->lists, b->stride, b->lists + 3 * b->stride, b->gid_of,
+                                 b->gcount, b->ngraphs, nullptr, st, false);
 }
 
 int capture(gary_batch *b) {
@@ -203,7 +205,10 @@ int gary_batch_set_cfg(gary_batch *b, const gary_launch_cfg *cfg) {
   GARY_CHECK_ARG(b, "gary_batch_set_cfg: null batch");
   gary_launch_cfg c = cfg ? *cfg : gary_launch_cfg{};
   const int bs = c.bin_small ? c.bin_small : 64, bh = c.bin_hub ? c.bin_hub : 2048;
-  GARY_CHECK_ARG(bs >= 2 && bs <= 512 && bh >= bs && bh <= 2048 && c.grid >= 0,
+  const int waves = c.block == 128 ? 2 : c.block == 512 ? 8 : 4;
+  GARY_CHECK_ARG(c.block == 0 || c.block == 128 || c.block == 256 || c.block == 512,
+                 "gary_batch_set_cfg: block=%d (0, 128, 256 or 512)", c.block);
+  GARY_CHECK_ARG(bs >= 2 && bs <= 2048 / waves && bh >= bs && bh <= 2048 && c.grid >= 0,
                  "gary_batch_set_cfg: bins small=%d hub=%d grid=%d out of range", bs, bh, c.grid);
   GARY_TRY(use_device(b->device));
   if (b->exec) {  // the captured graph bakes in the old configuration

@@ -524,8 +524,8 @@ int compute_bin_sums(const gary_device_graph &g, BinSums *sums, hipStream_t st) 
 
 }  // namespace
 
-int launch_routed(const gary_device_graph &g, const RouteTable *table, int bin_small, int bin_hub,
-                  int grid, float cost_big, unsigned long long *cnt, hipStream_t st) {
+int launch_routed(const gary_device_graph &g, const RouteTable *table, const PivotCfg &pc,
+                  float cost_big, unsigned long long *cnt, hipStream_t st) {
   const int32_t m = g.csr.m;
   if (m == 0 || g.csr.nnz == 0) return GARY_OK;
   GARY_CHECK_ARG(g.coo.rowIdx, "routed counting needs the COO rowIdx (call populate first)");
@@ -555,29 +555,28 @@ int launch_routed(const gary_device_graph &g, const RouteTable *table, int bin_s
   route_kernel<<<(int)((g.csr.nnz + kRouteChunk - 1) / kRouteChunk), 256, 0, st>>>(
       g, table, route, by_cost ? pcost : nullptr, cached, merge, counts);
   GARY_HIP(hipGetLastError());
-  GARY_TRY(launch_pivot_core(g, bin_small, bin_hub, grid, lists, stride, lists + 3 * stride,
-                             nullptr, nullptr, 0, cnt, st, true, route, by_cost ? pcost : nullptr,
-                             cost_big, table));
+  GARY_TRY(launch_pivot_core(g, pc, lists, stride, lists + 3 * stride, nullptr, nullptr, 0, cnt,
+                             st, true, route, by_cost ? pcost : nullptr, cost_big, table));
   return launch_edge_lists(g, cached, counts, merge, counts + 1, cnt, st);
 }
 
 // HEURISTIC and AUTO_LEARNED strategies of gary_tc_count.
-int launch_selected(const gary_device_graph &g, bool learned, int bin_small, int bin_hub,
-                    int grid, unsigned long long *cnt, hipStream_t st) {
+int launch_selected(const gary_device_graph &g, bool learned, const PivotCfg &pc,
+                    unsigned long long *cnt, hipStream_t st) {
   int dev = 0;
   GARY_TRY(current_device(&dev));
   std::lock_guard<std::mutex> lock(g_mu);
   DeviceState *s = nullptr;
   GARY_TRY(state_for(dev, &s));
   if (!learned || !s->model.trained)
-    return launch_routed(g, s->heuristic, bin_small, bin_hub, grid, 0.f, cnt, st);
+    return launch_routed(g, s->heuristic, pc, 0.f, cnt, st);
   GARY_TRY(compute_bin_sums(g, s->sums, st));
   mlp_table_kernel<<<1, kRouteBins, 0, st>>>(s->model.P, s->sums,
                                              log2f(1.f + (float)g.csr.nnz), (float)g.csr.nnz,
                                              s->model.allowed, fixed_of(s->model),
                                              s->model.margin, s->scratch);
   GARY_HIP(hipGetLastError());
-  return launch_routed(g, s->scratch, bin_small, bin_hub, grid, s->model.cost_big, cnt, st);
+  return launch_routed(g, s->scratch, pc, s->model.cost_big, cnt, st);
 }
 
 }  // namespace gary
@@ -608,7 +607,12 @@ int gary_tc_count_routed(const gary_device_graph *g, const uint8_t *h_routes,
   hipStream_t st = (hipStream_t)stream;
   GARY_HIP(hipMemcpyAsync(s->scratch, &t, sizeof t, hipMemcpyHostToDevice, st));
   GARY_HIP(hipStreamSynchronize(st));  // `t` lives on this stack frame
-  return launch_routed(*g, s->scratch, bs, bh, grid, cost_big, (unsigned long long *)d_count, st);
+  PivotCfg pc;
+  pc.bin_small = bs;
+  pc.bin_hub = bh;
+  pc.grid = grid;
+  if (cfg && cfg->block) pc.waves = cfg->block == 128 ? 2 : cfg->block == 512 ? 8 : 4;
+  return launch_routed(*g, s->scratch, pc, cost_big, (unsigned long long *)d_count, st);
   GARY_API_END
 }
 

@@ -23,6 +23,13 @@ constexpr uint8_t kRouteSkip = 3;    // no work (empty intersection, or excluded
 constexpr int kRouteBins = 256;      // (min(15, log2 d+(u)), min(15, log2 d+(v)))
 
 // Device-resident strategy table: route per bin and predicted cost per arc (ns) per bin.
+// Pivot-path launch knobs: pivots with out-degree <= bin_small share a workgroup (one per wave),
+// those up to bin_hub get a workgroup of `waves` waves, larger ones the hub kernel; grid = 0
+// takes the default (2048 workgroups).
+struct PivotCfg {
+  int bin_small = 64, bin_hub = 2048, grid = 0, waves = 4;
+};
+
 struct RouteTable {
   uint8_t route[kRouteBins];
   float cost[kRouteBins];
@@ -33,8 +40,8 @@ struct RouteTable {
 
 // Counts with every arc routed by `table` (device pointer). Cost-based pivot split: pivots whose
 // predicted bitmap cost exceeds cost_big (ns) get a workgroup; 0 keeps the degree split.
-int launch_routed(const gary_device_graph &g, const RouteTable *table, int bin_small, int bin_hub,
-                  int grid, float cost_big, unsigned long long *cnt, hipStream_t st);
+int launch_routed(const gary_device_graph &g, const RouteTable *table, const PivotCfg &pc,
+                  float cost_big, unsigned long long *cnt, hipStream_t st);
 
 // Edge-centric passes of the routed path over compacted arc lists (counts on the device).
 int launch_edge_lists(const gary_device_graph &g, const uint32_t *cached, const int32_t *n_cached,
@@ -42,12 +49,12 @@ int launch_edge_lists(const gary_device_graph &g, const uint32_t *cached, const 
                       hipStream_t st);
 
 // HEURISTIC / AUTO_LEARNED strategies (selector.hip).
-int launch_selected(const gary_device_graph &g, bool learned, int bin_small, int bin_hub,
-                    int grid, unsigned long long *cnt, hipStream_t st);
+int launch_selected(const gary_device_graph &g, bool learned, const PivotCfg &pc,
+                    unsigned long long *cnt, hipStream_t st);
 
 // Pivot path with caller-provided buffers (see tc_kernels.hip); gid_of/gcount select
 // per-graph accumulation into gcount[ngraphs] (zeroed by the launch).
-int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, int grid,
+int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                       int32_t *lists, size_t stride, int32_t *counters,
                       const int32_t *gid_of, unsigned long long *gcount, int32_t ngraphs,
                       unsigned long long *cnt, hipStream_t st, bool timed,

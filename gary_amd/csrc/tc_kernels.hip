@@ -604,11 +604,21 @@ struct TcCfg {
   int bin_small = 64, bin_hub = 2048;
 };
 
-constexpr int kPivotWaves = 4;
-constexpr int kPivotWords = 4096;  // 16 KiB: big-pivot set, or 4 x 1024-word small-pivot sets
+constexpr int kPivotWords = 4096;  // 16 KiB: a big pivot's set, or one small-pivot set per wave
 constexpr int kHubWaves = 8;
 constexpr int kHubWords = 8192;    // 32 KiB hub set
 constexpr int kHubGrid = 512;
+
+// Pivot path: `block` picks the waves per workgroup of the main pivot kernel (128, 256 or 512
+// threads; other values take 256).
+PivotCfg pivot_cfg(const TcCfg &c) {
+  PivotCfg p;
+  p.bin_small = c.bin_small;
+  p.bin_hub = c.bin_hub;
+  p.grid = c.grid;
+  p.waves = c.block == 128 ? 2 : c.block == 512 ? 8 : 4;
+  return p;
+}
 
 int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
   if (in) {
@@ -620,8 +630,8 @@ int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
     if (in->bin_small) c.bin_small = in->bin_small;
     if (in->bin_hub) c.bin_hub = in->bin_hub;
   }
-  GARY_CHECK_ARG(c.block >= 64 && c.block <= 256 && c.block % 64 == 0,
-                 "launch cfg: block=%d must be a multiple of 64 in [64, 256]", c.block);
+  GARY_CHECK_ARG(c.block >= 64 && c.block <= 512 && c.block % 64 == 0,
+                 "launch cfg: block=%d must be a multiple of 64 in [64, 512]", c.block);
   GARY_CHECK_ARG(c.group == 8 || c.group == 16 || c.group == 32 || c.group == 64,
                  "launch cfg: group=%d must be 8, 16, 32 or 64", c.group);
   GARY_CHECK_ARG(c.cache_depth >= 1 && c.cache_depth <= 4096 &&
@@ -630,8 +640,7 @@ int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
   GARY_CHECK_ARG(c.partition == GARY_PARTITION_INTERLEAVED || c.partition == GARY_PARTITION_CHUNKED,
                  "launch cfg: bad partition %d", c.partition);
   GARY_CHECK_ARG(c.grid >= 0, "launch cfg: grid=%d", c.grid);
-  GARY_CHECK_ARG(c.bin_small >= 2 && c.bin_small <= kPivotWords / kPivotWaves / 2 &&
-                     c.bin_hub >= c.bin_small && c.bin_hub <= kPivotWords / 2,
+  GARY_CHECK_ARG(c.bin_small >= 2 && c.bin_hub >= c.bin_small && c.bin_hub <= kPivotWords / 2,
                  "launch cfg: bins small=%d hub=%d out of range", c.bin_small, c.bin_hub);
   const size_t lds = (size_t)(c.block / c.group) * c.cache_depth * sizeof(int32_t);
   GARY_CHECK_ARG(lds <= 65536, "launch cfg: index cache needs %zu bytes of LDS", lds);
@@ -671,6 +680,7 @@ int launch_edge_g(const gary_device_graph &g, const TcCfg &c, unsigned long long
 template <int METHOD>
 int launch_edge(const gary_device_graph &g, const TcCfg &c, unsigned long long *cnt,
                 hipStream_t st) {
+  GARY_CHECK_ARG(c.block <= 256, "edge-centric strategies: block=%d exceeds 256", c.block);
   switch (c.group) {
     case 8: return launch_edge_g<8, METHOD>(g, c, cnt, st);
     case 16: return launch_edge_g<16, METHOD>(g, c, cnt, st);
@@ -685,7 +695,7 @@ int launch_edge(const gary_device_graph &g, const TcCfg &c, unsigned long long *
 // workgroups with 32 KiB sets) and the main persistent launch (big pivots per workgroup, small
 // pivots per wave). `lists` holds 3 * stride int32, `counters` 8 int32 (zeroed here by a kernel,
 // so the sequence can be captured in a graph). No host round trip, no allocation.
-int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, int grid,
+int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                       int32_t *lists, size_t stride, int32_t *counters,
                       const int32_t *gid_of, unsigned long long *gcount, int32_t ngraphs,
                       unsigned long long *cnt, hipStream_t st, bool timed,
@@ -704,7 +714,13 @@ int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, in
     GARY_HIP(hipGetLastError());
   }
   const int bgrid = (int)std::min<int64_t>(((int64_t)m + 4095) / 4096, 1024);
-  bin_pivots_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, bin_small, bin_hub, lists,
+  GARY_CHECK_ARG(pc.waves == 2 || pc.waves == 4 || pc.waves == 8,
+                 "pivot path: %d waves per workgroup (2, 4 or 8)", pc.waves);
+  GARY_CHECK_ARG(pc.bin_small >= 2 && pc.bin_small <= kPivotWords / pc.waves / 2 &&
+                     pc.bin_hub >= pc.bin_small && pc.bin_hub <= kPivotWords / 2,
+                 "pivot path: bins small=%d hub=%d out of range for %d waves", pc.bin_small,
+                 pc.bin_hub, pc.waves);
+  bin_pivots_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, lists,
                                            (int64_t)stride, counters, table, pcost, cost_big);
   GARY_HIP(hipGetLastError());
   // hubs: small list empty (counters[2] == 0), big list = bin 2
@@ -712,8 +728,16 @@ int launch_pivot_core(const gary_device_graph &g, int bin_small, int bin_hub, in
       g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount, route, table);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_begin(st));
-  tc_pivot_kernel<kPivotWaves, kPivotWords><<<grid ? grid : 2048, kPivotWaves * 64, 0, st>>>(
-      g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
+  const int grid = pc.grid ? pc.grid : 2048;
+  if (pc.waves == 2)
+    tc_pivot_kernel<2, kPivotWords><<<grid, 2 * 64, 0, st>>>(
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
+  else if (pc.waves == 8)
+    tc_pivot_kernel<8, kPivotWords><<<grid, 8 * 64, 0, st>>>(
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
+  else
+    tc_pivot_kernel<4, kPivotWords><<<grid, 4 * 64, 0, st>>>(
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_end(st));
   return GARY_OK;
@@ -749,8 +773,8 @@ int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long 
   const size_t stride = ((size_t)m + 63) & ~(size_t)63;
   GARY_TRY(workspace_get(dev, st, sizeof(int32_t) * (3 * stride + 64), &ws));
   int32_t *lists = (int32_t *)ws;
-  return launch_pivot_core(g, c.bin_small, c.bin_hub, c.grid, lists, stride, lists + 3 * stride,
-                           nullptr, nullptr, 0, cnt, st, true);
+  return launch_pivot_core(g, pivot_cfg(c), lists, stride, lists + 3 * stride, nullptr, nullptr,
+                           0, cnt, st, true);
 }
 
 }  // namespace
@@ -786,8 +810,7 @@ int gary_tc_count(const gary_device_graph *g, int strategy, const gary_launch_cf
     case GARY_TC_MERGE: return launch_edge<kIsectMerge>(*g, c, cnt, st);
     case GARY_TC_HEURISTIC:
     case GARY_TC_AUTO_LEARNED:
-      return launch_selected(*g, strategy == GARY_TC_AUTO_LEARNED, c.bin_small, c.bin_hub, c.grid,
-                             cnt, st);
+      return launch_selected(*g, strategy == GARY_TC_AUTO_LEARNED, pivot_cfg(c), cnt, st);
     case GARY_TC_BITMAP:
     default: return launch_pivot(*g, c, cnt, st);
   }

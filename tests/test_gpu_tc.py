@@ -77,6 +77,26 @@ def test_er_4096_pivot_bins(er_ctx, bins):
     assert gq.triangleCountHom(ctx, gq.BITMAP, cfg) == want
 
 
+@pytest.mark.parametrize("block", [128, 256, 512])
+@pytest.mark.parametrize("strategy", [gq.BITMAP, gq.HEURISTIC, gq.AUTO_LEARNED])
+def test_pivot_waves(rmat16, block, strategy):
+    """The main pivot kernel with 2, 4 and 8 waves per workgroup, with the per-wave small-pivot
+    sets at their largest and smallest bins."""
+    g, ctx, want = rmat16
+    for bins in ((2048 // (block // 64), 2048), (2, 2), (16, 256)):
+        cfg = gq.launch_cfg(block=block, grid=333, bin_small=bins[0], bin_hub=bins[1])
+        assert gq.triangleCountHom(ctx, strategy, cfg) == want, (block, bins)
+
+
+def test_pivot_cfg_limits(er_ctx):
+    g, ctx, _ = er_ctx
+    from gary_amd._lib import GaryError
+    with pytest.raises(GaryError):  # per-wave sets of 512 words hold at most 256 ids
+        gq.triangleCountHom(ctx, gq.BITMAP, gq.launch_cfg(block=512, bin_small=512))
+    with pytest.raises(GaryError):  # edge-centric kernels are bounded to 256 threads
+        gq.triangleCountHom(ctx, gq.CACHED_BSEARCH, gq.launch_cfg(block=512))
+
+
 @pytest.fixture(scope="module")
 def rmat16():
     n, s, d = graph.rmat(16, 16, seed=12345)
