@@ -343,41 +343,59 @@ __device__ __forceinline__ int32_t lower_bound(const int32_t *__restrict__ N, in
 }
 
 // One workgroup per vertex u: counts wedges u-v-w (v, w < u) into this block's dense counter
-// slice (n entries, all zero between vertices), then reads and clears them, adding C(c, 2).
+// slice (m entries, all zero between vertices), listing each endpoint w the first time its
+// counter leaves zero; then reads and clears the listed counters, adding C(c, 2). The second
+// pass touches the distinct endpoints only (a few percent of the wedges at hub vertices).
 __global__ void __launch_bounds__(kHomBlock)
     c4_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
-              uint32_t *__restrict__ counters, int32_t *__restrict__ cursor,
-              unsigned long long *c4_acc) {
-  __shared__ int32_t s_item;
+              uint32_t *__restrict__ counters, int32_t *__restrict__ touched_all,
+              int32_t *__restrict__ cursor, unsigned long long *c4_acc) {
+  __shared__ int32_t s_item, s_nt;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t m = g.csr.m;
   const int32_t *__restrict__ col = g.csr.colIdx;
   uint32_t *cnt = counters + (int64_t)blockIdx.x * m;
+  int32_t *touched = touched_all + (int64_t)blockIdx.x * m;
   uint64_t acc = 0;
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(cursor, 1);
+      s_nt = 0;
+    }
     __syncthreads();
     const int32_t u = s_item;
     if (u >= m) break;
     const int32_t nl = lo_len[u];
     const int32_t *Nu = col + g.csr.indPtr[u];
     if (nl >= 2) {
-      for (int pass = 0; pass < 2; ++pass) {
-        for (int32_t k = wib; k < nl; k += kHomWaves) {
-          const int32_t v = Nu[k];
-          const int32_t *Nv = col + g.csr.indPtr[v];
-          const int32_t cut = lower_bound(Nv, g.csr.deg[v], u);
-          for (int32_t p = lane; p < cut; p += kWave) {
-            const int32_t w = Nv[p];
-            if (pass == 0) {
-              atomicAdd(&cnt[w], 1u);
-            } else {
-              const uint64_t c = atomicExch(&cnt[w], 0u);
-              acc += c * (c - (c > 0)) / 2;
-            }
+      for (int32_t k = wib; k < nl; k += kHomWaves) {
+        const int32_t v = Nu[k];
+        const int32_t *Nv = col + g.csr.indPtr[v];
+        const int32_t cut = lower_bound(Nv, g.csr.deg[v], u);
+        for (int32_t p0 = 0; p0 < cut; p0 += kWave) {
+          const int32_t p = p0 + lane;
+          int32_t w = -1;
+          bool first = false;
+          if (p < cut) {
+            w = Nv[p];
+            first = atomicAdd(&cnt[w], 1u) == 0u;
+          }
+          const uint64_t fm = __ballot(first);
+          if (fm) {  // wave-uniform
+            int32_t base = 0;
+            if (lane == 0) base = atomicAdd(&s_nt, __popcll(fm));
+            base = __shfl(base, 0, kWave);
+            if (first) touched[base + lanes_below(fm)] = w;
           }
         }
-        __syncthreads();
+      }
+      __syncthreads();
+      const int32_t nt = s_nt;
+      for (int32_t k = threadIdx.x; k < nt; k += kHomBlock) {
+        // read-and-clear at L2, where the counting atomics landed (a plain load could hit a
+        // line this CU cached before those atomics)
+        const uint64_t c = atomicExch(&cnt[touched[k]], 0u);
+        acc += c * (c - 1) / 2;
       }
     }
     __syncthreads();
@@ -421,9 +439,11 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   const bool need_c4 = patterns & GARY_HOM_C4;
   const size_t b_rows = need_local ? al(sizeof(uint32_t) * (size_t)scratch_words * grid_local) : 0;
   const size_t b_cnt = need_c4 ? al(sizeof(uint32_t) * (size_t)m * grid_c4) : 0;
+  const size_t b_touched = need_c4 ? al(sizeof(int32_t) * (size_t)m * grid_c4) : 0;
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)3,
-                         b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_cnt, &ws));
+                         b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_cnt +
+                             b_touched, &ws));
   char *p = (char *)ws;
   int64_t *up_start = (int64_t *)p; p += b_up_start;
   int32_t *up_len = (int32_t *)p; p += b_up_len;
@@ -431,7 +451,8 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   uint32_t *tcount = (uint32_t *)p; p += b_t;
   char *misc = p; p += b_misc;
   uint32_t *rows = (uint32_t *)p; p += b_rows;
-  uint32_t *cnts = (uint32_t *)p;
+  uint32_t *cnts = (uint32_t *)p; p += b_cnt;
+  int32_t *touched = (int32_t *)p;
   int32_t *bad = (int32_t *)misc;               // [0] bad, [1] unsupported, [2..3] cursors
   HomTotals *tot = (HomTotals *)(misc + 64);    // tri, k4
   unsigned long long *sums = (unsigned long long *)(misc + 128);  // t2 lo/hi, c2 lo/hi, c4 lo/hi
@@ -449,7 +470,7 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   hom_sums_kernel<<<1024, 256, 0, st>>>(*g, up_start, up_len, tcount, sums, sums + 2);
   GARY_HIP(hipGetLastError());
   if (need_c4) {
-    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, cnts, bad + 3, sums + 4);
+    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, cnts, touched, bad + 3, sums + 4);
     GARY_HIP(hipGetLastError());
   }
   int32_t flags[2];

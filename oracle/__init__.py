@@ -40,6 +40,7 @@ def _load() -> C.CDLL:
         "or_tri_count_range": (U64, [I32, I32, P, P, I32]),
         "or_omp_threads": (I32, [I32]),
         "or_hom4": (C.c_int, [I32, P, P, I32, P]),
+        "or_hom4_ranked": (C.c_int, [I32, P, P, I32, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -139,6 +140,20 @@ def hom4(indPtr, colIdx, threads: int = 0) -> dict:
     r = Hom4()
     rc = lib.or_hom4(indPtr.size - 1, indPtr.ctypes.data, colIdx.ctypes.data, threads,
                      C.byref(r))
+    if rc != 0:
+        raise OverflowError("hom4 count overflows 64 bits")
+    return {k: int(getattr(r, k)) for k, _ in Hom4._fields_}
+
+
+def hom4_ranked(indPtr, colIdx, threads: int = 0) -> dict:
+    """hom4 by orientation and wedges, for ranked symmetric CSRs (graph.SYMMETRIC_RANKED) of any
+    size; equal to hom4() wherever both run."""
+    indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
+    r = Hom4()
+    rc = lib.or_hom4_ranked(indPtr.size - 1, indPtr.ctypes.data, colIdx.ctypes.data, threads,
+                            C.byref(r))
+    if rc == -1:
+        raise ValueError("hom4_ranked: ids are not ranked by degree")
     if rc != 0:
         raise OverflowError("hom4 count overflows 64 bits")
     return {k: int(getattr(r, k)) for k, _ in Hom4._fields_}

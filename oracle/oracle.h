@@ -89,6 +89,14 @@ typedef struct or_hom4_result {
 int or_hom4(int32_t n, const int64_t *indPtr, const int32_t *colIdx, int32_t threads,
             or_hom4_result *out);
 
+/* The same six values for a symmetric CSR whose ids are ranked by (degree, id), by orientation:
+ * triangles, 4-cliques and per-edge triangle counts from N+(u) = {w in N(u) : w > u}, and
+ * 4-cycles from wedges u-v-w whose highest vertex is u (#C4 = Σ_u Σ_w C(c_u(w), 2)). Cost
+ * O(m * degeneracy) instead of Σ deg^2, for the full-size graphs. Returns -1 when the ids are not
+ * ranked, -4 on overflow. */
+int or_hom4_ranked(int32_t n, const int64_t *indPtr, const int32_t *colIdx, int32_t threads,
+                   or_hom4_result *out);
+
 #ifdef __cplusplus
 }
 #endif

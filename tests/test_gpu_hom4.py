@@ -44,6 +44,28 @@ def test_hom4_vs_oracle(which):
     assert gq.hom4Count(ctx, gq.HOM_K4)["k4"] == want["k4"]
 
 
+@pytest.mark.parametrize("scale", [20, 22])
+def test_hom4_full_size_fixture(scale):
+    """R-MAT scales 20 and 22 (edgefactor 16, seeded): all six values equal the oracle's
+    (tests/golden/hom4_full.json, written by make_hom4_full.py), and the triangle count equals
+    the triangle-counting path's on the degree-oriented graph."""
+    import json
+    from conftest import GOLDEN
+    doc = json.loads((GOLDEN / "hom4_full.json").read_text())
+    rec = doc.get(f"rmat{scale}")
+    if rec is None:
+        pytest.skip(f"no fixture for rmat{scale}")
+    n, s, d = graph.rmat(scale, 16, seed=rec["seed"])
+    g, ctx = _ranked_ctx(n, s, d)
+    assert g.nnz == rec["nnz"]
+    h = gq.hom4Count(ctx)
+    assert h == {k: rec[k] for k in KEYS}
+    del ctx
+    go = graph.oriented(n, s, d)
+    octx = DeviceGraphCtx(go)
+    assert gq.triangleCountHom(octx, gq.BITMAP) == rec["tri"]
+
+
 def test_hom4_rejects_unranked_or_oriented():
     n, s, d = graph.rmat(10, 8, seed=5)
     sym = graph.csr_from_edges(n, s, d, graph.SYMMETRIC)  # not ranked

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import GOLDEN
 
 
 def _sorted_unique(rng, n, lo, hi):
@@ -150,6 +151,45 @@ def test_hom4_golden(golden_graphs):
         assert h["hom_diamond"] == rec["hom_diamond"], rec["name"]
         assert h["hom_c4"] == rec["hom_c4"], rec["name"]
         assert h["c4"] == rec["c4"], rec["name"]
+
+
+def test_hom4_ranked_equals_trace_oracle(golden_graphs):
+    """The orientation/wedge restatement against the trace-based one (pinned above by brute
+    force and networkx) on every golden graph and on seeded ER / R-MAT graphs."""
+    from gary_amd import graph
+    cases = [(rec["n"], np.array(rec["src"], np.int32), np.array(rec["dst"], np.int32))
+             for rec in golden_graphs]
+    cases.append(graph.er_4096())
+    for sc in (8, 11, 13):
+        cases.append(graph.rmat(sc, 16, seed=77 + sc))
+    for n, s, d in cases:
+        g = graph.csr_from_edges(n, s, d, graph.SYMMETRIC_RANKED)
+        assert oracle.hom4_ranked(g.indPtr, g.colIdx) == oracle.hom4(g.indPtr, g.colIdx), n
+
+
+def test_hom4_ranked_rejects_unranked():
+    # a star whose centre has the smallest id: degrees not non-decreasing in id
+    ip, ci = oracle.csr_from_edges(6, np.zeros(5, np.int32), np.arange(1, 6), 1)
+    with pytest.raises(ValueError):
+        oracle.hom4_ranked(ip, ci)
+
+
+def test_hom4_full_fixture_triangles():
+    """The full-size hom4 fixture's triangle counts equal the triangle oracle's (a different
+    code path) on the same seeded graphs; only scale 20 here, to keep the CPU suite short."""
+    import json
+    from gary_amd import graph
+    path = GOLDEN / "hom4_full.json"
+    if not path.exists():
+        pytest.skip("hom4_full.json not generated")
+    doc = json.loads(path.read_text())
+    rec = doc["rmat20"]
+    n, s, d = graph.rmat(20, 16, seed=rec["seed"])
+    g = graph.oriented(n, s, d)
+    assert oracle.tri_count(g.indPtr, g.colIdx) == rec["tri"]
+    sym = graph.csr_from_edges(n, s, d, graph.SYMMETRIC_RANKED)
+    assert sym.nnz == rec["nnz"]
+    assert rec["hom_c4"] >= 8 * rec["c4"] and rec["hom_k4"] == 24 * rec["k4"]
 
 
 def test_golden_brute_force_consistency(golden_graphs):
