@@ -9,8 +9,9 @@ B_alg = 4*W + 32*m (W = sum over arcs (u,v) of d+(u) + d+(v)), and the reference
 (cpuTriCount restated in the oracle) timed on this host.
 
 Further legs (--legs, default all):
-  c3     the learned selector against the plain degree-threshold heuristic (and bitmap-only) on
-         the Chung-Lu graph (4,194,304 vertices, ~64M edges, exponent 2.1, max weight 50,000);
+  c3     the learned selector against the plain degree-threshold heuristic (and bitmap-only, and
+         the reference dispatcher's cases as routes) on the Chung-Lu graph (4,194,304 vertices,
+         ~64M edges, exponent 2.1, max weight 50,000);
   batch  the batched form: 2,000 graphs of uneven size, one hipGraph replay per step,
          per-graph uint64 counts;
   hom4   K4 / diamond / 4-cycle homomorphism counts on R-MAT scale 22 (wall clock per call:
@@ -94,6 +95,16 @@ class Bench:
         assert total % steps == 0
         return ms, total // steps, (kms / launches if launches else None)
 
+    def time_dispatch(self, ctx, want: int) -> float:
+        """ms per count with the reference dispatcher's cases as routes (no bitmap route)."""
+        tab = self.selector.dispatch_table()
+        got = self.selector.count_with_table(ctx, tab)
+        if got != want:
+            raise SystemExit(f"dispatch table counted {got} != {want}")
+        ms = self.selector.time_table(ctx, tab, repeats=3)
+        log(f"reference-dispatch table: {ms:.3f} ms")
+        return ms
+
     # ---- legs ------------------------------------------------------------------------------
     def headline(self) -> dict:
         a, gq, graph = self.a, self.gq, self.graph
@@ -125,6 +136,7 @@ class Bench:
             if kms is not None:
                 kernel_ms[nm] = kms
             log(f"{nm}: {ms:.3f} ms/step" + (f", triangles={tri}" if nm == a.strategy else ""))
+        self.dispatch_ms = self.time_dispatch(ctx, tri)
         pivot_path = a.strategy in ("learned", "heuristic", "bitmap")
         k_ms = kernel_ms.get(a.strategy, results[a.strategy])
         k_bytes = B_alg_main if pivot_path else B_alg
@@ -174,7 +186,9 @@ class Bench:
             leg[f"{nm}_ms"] = ms
             log(f"chung_lu_4m {nm}: {ms:.3f} ms")
         leg["triangles"] = ref
+        leg["reference_dispatch_ms"] = self.time_dispatch(ctx, ref)
         leg["learned_vs_plain"] = leg["heuristic_ms"] / leg["learned_ms"]
+        leg["learned_vs_reference_dispatch"] = leg["reference_dispatch_ms"] / leg["learned_ms"]
         return leg
 
     def batch(self) -> dict:
@@ -299,7 +313,9 @@ def main(argv=None) -> int:
         r = out["strategies_ms"]
         selector_legs[f"rmat{a.scale}"] = {
             "heuristic_ms": r["heuristic"], "learned_ms": r["learned"],
-            "bitmap_ms": r["bitmap"], "learned_vs_plain": r["heuristic"] / r["learned"]}
+            "bitmap_ms": r["bitmap"], "reference_dispatch_ms": b.dispatch_ms,
+            "learned_vs_plain": r["heuristic"] / r["learned"],
+            "learned_vs_reference_dispatch": b.dispatch_ms / r["learned"]}
     if "c3" in legs:
         selector_legs["chung_lu_4m"] = b.c3()
     if selector_legs:

@@ -139,10 +139,11 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- bin features (sampled) -------------------------------------------------------------------
-// sums[bin] = {arcs seen, overlap samples, Σ |N+(u) ∩ N+(v)| / min(d+(u), d+(v))}. Every arc whose
-// index is a multiple of `hist_every` is counted (the count is scaled back); every multiple of
-// `ovl_every` also measures its overlap, one wave per sampled arc: the lanes stride over the
-// shorter list and binary-search the longer one, so no thread walks a whole list alone.
+// sums[bin] = {arcs seen, overlap samples, Σ |N+(u) ∩ N+(v)| / min(d+(u), d+(v))}. Arcs are
+// sampled in runs of 64 consecutive ones (one wave, coalesced COO reads): every `hist_every`-th
+// run is counted (scaled back by hist_every). Every arc whose index is a multiple of `ovl_every`
+// also measures its overlap, one wave per sampled arc: the lanes stride over the shorter list and
+// binary-search the longer one, so no thread walks a whole list alone.
 struct BinSums {
   float arcs, samples, ovl;
 };
@@ -154,21 +155,21 @@ __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int
   __syncthreads();
   const int32_t m = g.csr.m;
   const int64_t nnz = g.csr.nnz;
-  const int64_t nsamp = (nnz + hist_every - 1) / hist_every;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nsamp;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = k * hist_every;
+  const int lane = lane_id();
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t wave0 = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t nruns = (nnz + kWave - 1) / kWave;
+  for (int64_t k = wave0 * hist_every; k < nruns; k += nwaves * hist_every) {
+    const int64_t e = k * kWave + lane;
+    if (e >= nnz) continue;
     const int32_t u = g.coo.rowIdx[e], v = g.coo.colIdx[e];
     if ((uint32_t)u >= (uint32_t)m || (uint32_t)v >= (uint32_t)m) continue;
     const int32_t a = g.csr.deg[u], b = g.csr.deg[v];
     if (a < 2 || b < 1) continue;
     atomicAdd(&s_arcs[ilog2c(a) * 16 + ilog2c(b)], (float)hist_every);
   }
-  const int lane = lane_id();
-  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
   const int64_t novl = (nnz + ovl_every - 1) / ovl_every;
-  for (int64_t k = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; k < novl;
-       k += nwaves) {
+  for (int64_t k = wave0; k < novl; k += nwaves) {
     const int64_t e = k * ovl_every;
     const int32_t u = g.coo.rowIdx[e], v = g.coo.colIdx[e];
     if ((uint32_t)u >= (uint32_t)m || (uint32_t)v >= (uint32_t)m) continue;  // wave-uniform

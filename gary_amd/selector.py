@@ -51,6 +51,23 @@ def heuristic_table() -> np.ndarray:
     return t
 
 
+def dispatch_table() -> np.ndarray:
+    """The reference dispatcher's four cases as a route table, without the bitmap route
+    (include/ops/Intersection.cuh:182-187): both lists small (< 64) -> merge; one small, one
+    large, or both large and skewed (>= 8x) -> binary search (cached); both large and even ->
+    merge."""
+    t = np.zeros(NBINS, np.uint8)
+    for b in range(NBINS):
+        i, j = b >> 4, b & 15
+        if i <= 5 and j <= 5:
+            t[b] = MERGE
+        elif abs(j - i) >= 3:
+            t[b] = CACHED
+        else:
+            t[b] = MERGE
+    return t
+
+
 def bin_of(du: int, dv: int) -> int:
     return min(15, int(du).bit_length() - 1) * 16 + min(15, int(dv).bit_length() - 1)
 
