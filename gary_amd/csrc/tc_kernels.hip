@@ -433,7 +433,7 @@ constexpr int kSmallBatch = 16;
 constexpr int kUnroll = 2;
 
 template <int WAVES, int TABLE_WORDS>
-__global__ void __launch_bounds__(WAVES * 64)
+__global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8)))
     tc_pivot_kernel(gary_device_graph g, const int32_t *__restrict__ small,
                     const int32_t *__restrict__ big, const int32_t *__restrict__ counts,
                     int32_t *__restrict__ cursors, unsigned long long *count,
@@ -442,7 +442,9 @@ __global__ void __launch_bounds__(WAVES * 64)
   __shared__ int32_t table[TABLE_WORDS];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   __shared__ RowBatch s_rows[WAVES];
-  __shared__ int32_t s_item;
+  // No separate LDS word for the pivot index: at <4,4096> the kernel's LDS is then exactly
+  // 20 KiB, and eight workgroups fit a CU (160 KiB) instead of seven. The index is passed
+  // through table[0] between pivots, when the set is dead.
   const int lane = lane_id();
   const int wib = threadIdx.x / kWave;
   RowBatch &rb = s_rows[wib];
@@ -452,9 +454,10 @@ __global__ void __launch_bounds__(WAVES * 64)
   uint32_t c = 0;
   // ---- phase 1: big pivots, one workgroup each
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&cursors[1], 1);
+    if (threadIdx.x == 0) table[0] = atomicAdd(&cursors[1], 1);
     __syncthreads();
-    const int32_t p = s_item;
+    const int32_t p = table[0];
+    __syncthreads();  // every thread holds p before the next set overwrites table[0]
     if (p >= n_big) break;
     const int32_t u = big[p];
     const int32_t d = (uint32_t)u < m ? g.csr.deg[u] : 0;
@@ -473,7 +476,7 @@ __global__ void __launch_bounds__(WAVES * 64)
         c = 0;
       }
     }
-    __syncthreads();  // the set and s_item are rewritten for the next pivot
+    __syncthreads();  // the set (and table[0]) are rewritten for the next pivot
   }
   // ---- phase 2: small pivots, one wave each
   int32_t *wt = table + wib * (TABLE_WORDS / WAVES);
