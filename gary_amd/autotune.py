@@ -112,7 +112,7 @@ def tune_graph(ctx, strategy: str, repeats: int = 5) -> dict:
         return t
 
     space = PIVOT_SPACE if strategy in PIVOT_STRATEGIES else EDGE_SPACE
-    start = {"block": 256, "grid": 2048, "bin_small": 64, "bin_hub": 2048} \
+    start = {"block": 256, "grid": 2048, "bin_small": 128, "bin_hub": 2048} \
         if strategy in PIVOT_STRATEGIES else \
         {"block": 256, "group": 16 if strategy == "merge" else 64, "cache_depth": 64,
          "partition": gq.INTERLEAVED, "grid": 0}
@@ -147,7 +147,7 @@ def tune_batch(b, repeats: int = 5) -> dict:
         return t
 
     best, t, evals = _descend(measure, PIVOT_SPACE,
-                              {"block": 256, "grid": 2048, "bin_small": 64, "bin_hub": 2048})
+                              {"block": 256, "grid": 2048, "bin_small": 128, "bin_hub": 2048})
     b.set_cfg(None)
     return {"cfg": best, "ms": t, "default_ms": t_def, "evals": evals}
 

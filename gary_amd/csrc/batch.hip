@@ -164,7 +164,7 @@ int gary_batch_create(int device, int32_t ngraphs, const int32_t *graph_vertices
   BATCH_HIP(hipMalloc(&b->colIdx, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
   BATCH_HIP(hipMalloc(&b->deg, sizeof(int32_t) * (size_t)std::max<int64_t>(nv, 1)));
   BATCH_HIP(hipMalloc(&b->gid_of, sizeof(int32_t) * (size_t)std::max<int64_t>(nv, 1)));
-  BATCH_HIP(hipMalloc(&b->lists, sizeof(int32_t) * (3 * b->stride + 64)));
+  BATCH_HIP(hipMalloc(&b->lists, sizeof(int32_t) * (3 * b->stride + gary::kPivotCounterWords)));
   BATCH_HIP(hipMalloc(&b->gcount, sizeof(unsigned long long) * (size_t)ngraphs));
   BATCH_HIP(hipMalloc(&d_vbase, sizeof(int32_t) * ((size_t)ngraphs + 1)));
   BATCH_HIP(hipMalloc(&bad, sizeof(int32_t)));
@@ -204,7 +204,7 @@ int gary_batch_destroy(gary_batch *b) {
 int gary_batch_set_cfg(gary_batch *b, const gary_launch_cfg *cfg) {
   GARY_CHECK_ARG(b, "gary_batch_set_cfg: null batch");
   gary_launch_cfg c = cfg ? *cfg : gary_launch_cfg{};
-  const int bs = c.bin_small ? c.bin_small : 64, bh = c.bin_hub ? c.bin_hub : 2048;
+  const int bs = c.bin_small ? c.bin_small : gary::kDefaultBinSmall, bh = c.bin_hub ? c.bin_hub : 2048;
   const int waves = c.block == 128 ? 2 : c.block == 512 ? 8 : 4;
   GARY_CHECK_ARG(c.block == 0 || c.block == 128 || c.block == 256 || c.block == 512,
                  "gary_batch_set_cfg: block=%d (0, 128, 256 or 512)", c.block);

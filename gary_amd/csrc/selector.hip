@@ -536,7 +536,7 @@ int launch_routed(const gary_device_graph &g, const RouteTable *table, const Piv
   const size_t stride = ((size_t)m + 63) & ~(size_t)63;
   const size_t b_route = ((size_t)g.csr.nnz + 255) & ~(size_t)255;
   const size_t b_pcost = sizeof(float) * stride;
-  const size_t b_lists = sizeof(int32_t) * (3 * stride + 64);
+  const size_t b_lists = sizeof(int32_t) * (3 * stride + kPivotCounterWords);
   const size_t b_arcs = (sizeof(uint32_t) * (size_t)g.csr.nnz + 255) & ~(size_t)255;
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)((uintptr_t)st ^ 0x5E1Eull),
@@ -597,7 +597,7 @@ int gary_tc_count_routed(const gary_device_graph *g, const uint8_t *h_routes,
   std::memcpy(t.route, h_routes, kRouteBins);
   for (int b = 0; b < kRouteBins; ++b) t.cost[b] = h_costs ? h_costs[b] : 1.f;
   set_rowmask(t);
-  const int bs = cfg && cfg->bin_small ? cfg->bin_small : 64;
+  const int bs = cfg && cfg->bin_small ? cfg->bin_small : kDefaultBinSmall;
   const int bh = cfg && cfg->bin_hub ? cfg->bin_hub : 2048;
   const int grid = cfg ? cfg->grid : 0;
   int dev = 0;

@@ -26,8 +26,15 @@ constexpr int kRouteBins = 256;      // (min(15, log2 d+(u)), min(15, log2 d+(v)
 // Pivot-path launch knobs: pivots with out-degree <= bin_small share a workgroup (one per wave),
 // those up to bin_hub get a workgroup of `waves` waves, larger ones the hub kernel; grid = 0
 // takes the default (2048 workgroups).
+// int32 words of pivot-path counters that follow the three pivot lists (3 * stride words).
+constexpr int kPivotCounterWords = 128;
+
+// Default out-degree split between per-wave and per-workgroup pivots (measured best on
+// R-MAT 20 with the heavy-first pivot order).
+constexpr int kDefaultBinSmall = 128;
+
 struct PivotCfg {
-  int bin_small = 64, bin_hub = 2048, grid = 0, waves = 4;
+  int bin_small = kDefaultBinSmall, bin_hub = 2048, grid = 0, waves = 4;
 };
 
 struct RouteTable {

@@ -424,12 +424,17 @@ __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
 // Persistent pivot kernel.
 // Phase 1: each workgroup takes the next pivot of `big` from an atomic cursor, builds one
 //   shared set of TABLE_WORDS words and its WAVES waves split the rows.
-// Phase 2: each wave takes `kSmallBatch` pivots of `small` at a time and works alone with a
+// Phase 2: each wave takes a few pivots of `small` at a time and works alone with a
 //   1/WAVES share of the table memory.
 // counts[0] = #small, counts[1] = #big; cursors[0..1] start at zero.
 // Per-graph mode (gid_of != nullptr, batched disjoint unions): each wave adds its partial count
 // to gcount[gid_of[u]] whenever the pivot's graph changes, instead of to *count.
-constexpr int kSmallBatch = 16;
+// The small list is ordered heaviest first; its first split[0] pivots (out-degree >=
+// 2^kLightLog) are claimed kSmallBatchHeavy at a time, the rest kSmallBatchLight at a time.
+constexpr int kSmallBatchHeavy = 4;
+constexpr int kSmallBatchLight = 16;
+constexpr int kLightLog = 4;
+static_assert(kLightLog >= 1 && kLightLog <= 15, "light-pivot threshold");
 constexpr int kUnroll = 2;
 
 template <int WAVES, int TABLE_WORDS>
@@ -438,7 +443,8 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     const int32_t *__restrict__ big, const int32_t *__restrict__ counts,
                     int32_t *__restrict__ cursors, unsigned long long *count,
                     const int32_t *__restrict__ gid_of, unsigned long long *gcount,
-                    const uint8_t *__restrict__ route, const RouteTable *__restrict__ rtable) {
+                    const uint8_t *__restrict__ route, const RouteTable *__restrict__ rtable,
+                    const int32_t *__restrict__ split) {
   __shared__ int32_t table[TABLE_WORDS];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   __shared__ RowBatch s_rows[WAVES];
@@ -481,12 +487,19 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
   // ---- phase 2: small pivots, one wave each
   int32_t *wt = table + wib * (TABLE_WORDS / WAVES);
   int32_t cur = -1;  // graph of the wave's partial count (per-graph mode)
+  // claim k covers pivots [k*H, k*H + H) while k < ceil(n_heavy / H), then light claims of L
+  const int32_t n_heavy = min(split[0], n_small);
+  const int32_t heavy_claims = (n_heavy + kSmallBatchHeavy - 1) / kSmallBatchHeavy;
   while (true) {
-    int32_t base = 0;
-    if (lane == 0) base = atomicAdd(&cursors[0], kSmallBatch);
-    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, kWave));
+    int32_t k = 0;
+    if (lane == 0) k = atomicAdd(&cursors[0], 1);
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
+    const int32_t base = k < heavy_claims
+                             ? k * kSmallBatchHeavy
+                             : n_heavy + (k - heavy_claims) * kSmallBatchLight;
     if (base >= n_small) break;
-    const int32_t end = min(base + kSmallBatch, n_small);
+    const int32_t end = min(base + (k < heavy_claims ? kSmallBatchHeavy : kSmallBatchLight),
+                            k < heavy_claims ? n_heavy : n_small);
     for (int32_t p = base; p < end; ++p) {
       const int32_t u = small[p];
       if ((uint32_t)u >= m) continue;
@@ -528,68 +541,130 @@ __global__ void fill_zero_kernel(uint32_t *__restrict__ p, int64_t n) {
 // ======================================================================================
 
 // bins: 0 = small (2 <= d <= t_small), 1 = big (t_small < d <= t_big), 2 = hub (d > t_big).
-// Vertices with out-degree < 2 are dropped (no triangle has them as its lowest vertex). Each
-// workgroup bins a contiguous vertex range: LDS counters first, then one global atomic per bin
-// reserves the workgroup's slots, then a second pass writes the ids.
+// Vertices with out-degree < 2 are dropped (no triangle has them as its lowest vertex).
 // Routed mode (rowmask != 0xffffffff): pivots whose degree row routes no bin to the bitmap are
 // dropped; with predicted costs (pcost) the small/big split is by cost (cost_big, ns) instead of
 // by out-degree. Hubs always go by degree.
-__device__ __forceinline__ int pivot_bin(const int32_t *__restrict__ deg, int64_t v, int64_t end,
+// Within each list, pivots are ordered heaviest first by bucket 15 - min(15, log2 weight)
+// (weight = out-degree, or the predicted cost in cost mode), so the persistent kernels take the
+// long pivots early and the tail of the launch is made of short ones.
+constexpr int kBuckets = 16;
+constexpr int kKeys = 3 * kBuckets;
+// counter words: [0] #small, [1] #big, [2] always 0, [3] #hub, [4..5] main cursors,
+// [6..7] hub cursors, [8, 8+kKeys) histogram, [8+kKeys, 8+2*kKeys) scatter cursors
+constexpr int kHistOff = 8;
+constexpr int kCurOff = 8 + kKeys;
+// [kSplitOff] heavy/light split of the small list; [+2] the same for the hub launch (whose
+// small list is empty: always 0)
+constexpr int kSplitOff = kCurOff + kKeys;
+static_assert(kSplitOff + 4 <= kPivotCounterWords, "pivot counter words");
+
+// ordered == false (per-graph counting of batched unions) puts every pivot in the lightest
+// bucket: the lists then keep vertex order, so consecutive pivots of a wave mostly belong to one
+// graph and its per-graph partial count is flushed rarely.
+__device__ __forceinline__ int pivot_key(const int32_t *__restrict__ deg, int64_t v, int64_t end,
                                          int32_t t_small, int32_t t_big, uint32_t rowmask,
-                                         const float *__restrict__ pcost, float cost_big) {
+                                         const float *__restrict__ pcost, float cost_big,
+                                         bool ordered) {
   if (v >= end) return -1;
   const int32_t d = deg[v];
   if (d < 2) return -1;
   if (!((rowmask >> min(15, 31 - __clz(d))) & 1u)) return -1;
-  if (d > t_big) return 2;
+  const int lg = ordered ? min(15, 31 - __clz(d)) : 0;
+  if (d > t_big) return 2 * kBuckets + (15 - lg);
   if (pcost && cost_big > 0.f) {
     const float cst = pcost[v];
     if (!(cst > 0.f)) return -1;
-    return cst > cost_big ? 1 : 0;
+    const int lc = ordered ? min(15, max(0, (int)log2f(cst))) : 0;
+    return (cst > cost_big ? kBuckets : 0) + (15 - lc);
   }
-  return d <= t_small ? 0 : 1;
+  return (d <= t_small ? 0 : kBuckets) + (15 - lg);
+}
+
+// Calls f(key, mask, leader) once per distinct key of the wave's active lanes (key >= 0).
+template <class F>
+__device__ __forceinline__ void for_each_key(int key, F &&f) {
+  uint64_t todo = __ballot(key >= 0);
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int k = __shfl(key, leader, kWave);
+    const uint64_t mask = __ballot(key == k);
+    f(k, mask, leader);
+    todo &= ~mask;
+  }
 }
 
 __global__ void __launch_bounds__(256)
-    bin_pivots_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small, int32_t t_big,
-                      int32_t *__restrict__ lists, int64_t stride, int32_t *__restrict__ counters,
-                      const RouteTable *__restrict__ table, const float *__restrict__ pcost,
-                      float cost_big) {
+    bin_hist_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small, int32_t t_big,
+                    int32_t *__restrict__ counters, const RouteTable *__restrict__ table,
+                    const float *__restrict__ pcost, float cost_big, bool ordered) {
   const uint32_t rowmask = table ? table->rowmask : 0xffffffffu;
-  __shared__ int32_t s_cnt[3], s_base[3];
+  __shared__ int32_t s_hist[kKeys];
   const int lane = lane_id();
+  for (int i = threadIdx.x; i < kKeys; i += blockDim.x) s_hist[i] = 0;
+  __syncthreads();
   const int64_t chunk = (((int64_t)m + gridDim.x - 1) / gridDim.x + 63) & ~63ll;
   const int64_t beg = (int64_t)blockIdx.x * chunk, end = min((int64_t)m, beg + chunk);
-  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
-  __syncthreads();
   for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
-    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big);
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const uint64_t mask = __ballot(bin == b);
-      if (mask && lane == __ffsll((unsigned long long)mask) - 1)
-        atomicAdd(&s_cnt[b], __popcll(mask));
+    const int key =
+        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered);
+    for_each_key(key, [&](int k, uint64_t mask, int leader) {
+      if (lane == leader) atomicAdd(&s_hist[k], __popcll(mask));
+    });
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kKeys; i += blockDim.x)
+    if (s_hist[i]) atomicAdd(&counters[kHistOff + i], s_hist[i]);
+}
+
+// Same vertex ranges as bin_hist_kernel; bucket starts come from the finished histogram.
+__global__ void __launch_bounds__(256)
+    bin_scatter_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small,
+                       int32_t t_big, int32_t *__restrict__ lists, int64_t stride,
+                       int32_t *__restrict__ counters, const RouteTable *__restrict__ table,
+                       const float *__restrict__ pcost, float cost_big, bool ordered) {
+  const uint32_t rowmask = table ? table->rowmask : 0xffffffffu;
+  __shared__ int32_t s_cnt[kKeys], s_base[kKeys], s_start[kKeys];
+  const int lane = lane_id();
+  if (threadIdx.x < 3) {  // bucket starts of list threadIdx.x (heaviest bucket first)
+    const int b = threadIdx.x;
+    int32_t run = 0;
+    for (int k = 0; k < kBuckets; ++k) {
+      s_start[b * kBuckets + k] = run;
+      run += counters[kHistOff + b * kBuckets + k];
+    }
+    if (blockIdx.x == 0) {
+      counters[b == 2 ? 3 : b] = run;
+      if (b == 0) counters[kSplitOff] = s_start[15 - kLightLog + 1];  // buckets with lg >= kLightLog
     }
   }
+  for (int i = threadIdx.x; i < kKeys; i += blockDim.x) s_cnt[i] = 0;
   __syncthreads();
-  if (threadIdx.x < 3) {
-    s_base[threadIdx.x] = s_cnt[threadIdx.x]
-        ? atomicAdd(&counters[threadIdx.x == 2 ? 3 : threadIdx.x], s_cnt[threadIdx.x]) : 0;
-    s_cnt[threadIdx.x] = 0;
+  const int64_t chunk = (((int64_t)m + gridDim.x - 1) / gridDim.x + 63) & ~63ll;
+  const int64_t beg = (int64_t)blockIdx.x * chunk, end = min((int64_t)m, beg + chunk);
+  for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
+    const int key =
+        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered);
+    for_each_key(key, [&](int k, uint64_t mask, int leader) {
+      if (lane == leader) atomicAdd(&s_cnt[k], __popcll(mask));
+    });
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kKeys; i += blockDim.x) {
+    s_base[i] = s_cnt[i] ? s_start[i] + atomicAdd(&counters[kCurOff + i], s_cnt[i]) : 0;
+    s_cnt[i] = 0;
   }
   __syncthreads();
   for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
-    const int bin = pivot_bin(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big);
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const uint64_t mask = __ballot(bin == b);
-      if (!mask) continue;
-      const int leader = __ffsll((unsigned long long)mask) - 1;
+    const int key =
+        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered);
+    for_each_key(key, [&](int k, uint64_t mask, int leader) {
       int32_t pos0 = 0;
-      if (lane == leader) pos0 = atomicAdd(&s_cnt[b], __popcll(mask));
+      if (lane == leader) pos0 = atomicAdd(&s_cnt[k], __popcll(mask));
       pos0 = __shfl(pos0, leader, kWave);
-      if (bin == b) lists[b * stride + s_base[b] + pos0 + lanes_below(mask)] = (int32_t)(base + lane);
-    }
+      if (key == k)
+        lists[(k / kBuckets) * stride + s_base[k] + pos0 + lanes_below(mask)] = (int32_t)(base + lane);
+    });
   }
 }
 
@@ -604,7 +679,7 @@ namespace {
 
 struct TcCfg {
   int block = 256, grid = 0, group = 64, cache_depth = 64, partition = GARY_PARTITION_INTERLEAVED;
-  int bin_small = 64, bin_hub = 2048;
+  int bin_small = kDefaultBinSmall, bin_hub = 2048;
 };
 
 constexpr int kPivotWords = 4096;  // 16 KiB: a big pivot's set, or one small-pivot set per wave
@@ -708,8 +783,9 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
   if (m == 0 || g.csr.nnz == 0) return GARY_OK;
   GARY_CHECK_ARG(((uintptr_t)g.csr.colIdx & 15) == 0,
                  "pivot strategies read colIdx in 16-byte chunks: colIdx must be 16-byte aligned");
-  // counters: [0] #small, [1] #big, [2] always 0, [3] #hub; cursors: [4..5] main, [6..7] hub
-  fill_zero_kernel<<<1, 64, 0, st>>>((uint32_t *)counters, 8);
+  // counters: [0] #small, [1] #big, [2] always 0, [3] #hub; cursors: [4..5] main, [6..7] hub;
+  // then the binning histogram and scatter cursors
+  fill_zero_kernel<<<1, 256, 0, st>>>((uint32_t *)counters, kPivotCounterWords);
   GARY_HIP(hipGetLastError());
   if (gcount) {
     fill_zero_kernel<<<std::max<int64_t>(1, std::min<int64_t>((ngraphs * 2 + 255) / 256, 1024)), 256,
@@ -723,24 +799,33 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                      pc.bin_hub >= pc.bin_small && pc.bin_hub <= kPivotWords / 2,
                  "pivot path: bins small=%d hub=%d out of range for %d waves", pc.bin_small,
                  pc.bin_hub, pc.waves);
-  bin_pivots_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, lists,
-                                           (int64_t)stride, counters, table, pcost, cost_big);
+  const bool ordered = gid_of == nullptr;
+  bin_hist_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, counters, table,
+                                         pcost, cost_big, ordered);
+  GARY_HIP(hipGetLastError());
+  bin_scatter_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, lists,
+                                            (int64_t)stride, counters, table, pcost, cost_big,
+                                            ordered);
   GARY_HIP(hipGetLastError());
   // hubs: small list empty (counters[2] == 0), big list = bin 2
   tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
-      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount, route, table);
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount, route, table,
+      counters + kSplitOff + 2);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_begin(st));
   const int grid = pc.grid ? pc.grid : 2048;
   if (pc.waves == 2)
     tc_pivot_kernel<2, kPivotWords><<<grid, 2 * 64, 0, st>>>(
-        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table,
+        counters + kSplitOff);
   else if (pc.waves == 8)
     tc_pivot_kernel<8, kPivotWords><<<grid, 8 * 64, 0, st>>>(
-        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table,
+        counters + kSplitOff);
   else
     tc_pivot_kernel<4, kPivotWords><<<grid, 4 * 64, 0, st>>>(
-        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table);
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table,
+        counters + kSplitOff);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_end(st));
   return GARY_OK;
@@ -774,7 +859,7 @@ int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long 
   GARY_TRY(current_device(&dev));
   void *ws = nullptr;
   const size_t stride = ((size_t)m + 63) & ~(size_t)63;
-  GARY_TRY(workspace_get(dev, st, sizeof(int32_t) * (3 * stride + 64), &ws));
+  GARY_TRY(workspace_get(dev, st, sizeof(int32_t) * (3 * stride + kPivotCounterWords), &ws));
   int32_t *lists = (int32_t *)ws;
   return launch_pivot_core(g, pivot_cfg(c), lists, stride, lists + 3 * stride, nullptr, nullptr,
                            0, cnt, st, true);
