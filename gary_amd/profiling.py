@@ -1,6 +1,7 @@
 """Summaries of rocprofv3 CSV output for the profiles/ folder.
 
     python -m gary_amd.profiling kernels <kernel_stats.csv> --cmd "<command>" > profiles/X.md
+    python -m gary_amd.profiling counters <counter_collection.csv>... --kernel K --cmd "<commands>"
     python -m gary_amd.profiling traffic <fetch_counter_collection.csv> <write_counter_collection.csv>
         --kernel tc_pivot_kernel<4 --copy-bytes N --cmd "<command>" > profiles/Y.md
 
@@ -76,6 +77,20 @@ def traffic(fetch_csv: str, write_csv: str, kernel: str, copy_bytes: int) -> tup
     return "\n".join(lines), t
 
 
+def counters_table(csvs: list[str], kernel: str) -> str:
+    """Average per dispatch of every counter found in the given counter_collection CSVs, for the
+    kernels whose name contains `kernel`."""
+    rows = []
+    for path in csvs:
+        agg = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if kernel in r["Kernel_Name"]:
+                agg[r["Counter_Name"]].append(float(r["Counter_Value"]))
+        for name, vals in sorted(agg.items()):
+            rows.append(f"| {name} | {len(vals)} | {sum(vals) / len(vals):.0f} |")
+    return "\n".join(["| counter | dispatches | average per dispatch |", "|---|---|---|", *rows])
+
+
 def read_traffic(md_path: str | Path) -> int | None:
     p = Path(md_path)
     if not p.exists():
@@ -91,6 +106,11 @@ def main(argv=None) -> int:
     k.add_argument("stats_csv")
     k.add_argument("--cmd", required=True)
     k.add_argument("--title", default="rocprofv3 kernel trace")
+    c = sub.add_parser("counters")
+    c.add_argument("csvs", nargs="+")
+    c.add_argument("--kernel", required=True)
+    c.add_argument("--cmd", required=True)
+    c.add_argument("--title", default="PMC counters per dispatch")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv")
     t.add_argument("write_csv")
@@ -101,6 +121,9 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     if a.what == "kernels":
         print(f"# {a.title}\n\ncommand:\n\n    {a.cmd}\n\n{kernel_table(a.stats_csv)}")
+    elif a.what == "counters":
+        print(f"# {a.title}\n\ncommands (one PMC pass each):\n\n    {a.cmd}\n\n"
+              f"kernel: `{a.kernel}`\n\n{counters_table(a.csvs, a.kernel)}")
     else:
         body, _ = traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes)
         print(f"# {a.title}\n\ncommands:\n\n    {a.cmd}\n\n{body}")

@@ -224,7 +224,8 @@ int gary_tc_count_sync(const gary_device_graph *g, int strategy, const gary_laun
 int gary_tc_count_legacy(const gary_device_graph *g, uint32_t *d_count, void *stream);
 
 /* ---- strategy selection: routed counting, heuristic, learned MLP selector ------------------ */
-/* Arcs (u,v) fall in bin 16*min(15, log2 d+(u)) + min(15, log2 d+(v)). A route table assigns each
+/* No counterpart in the reference, whose dispatcher intersect() names four size cases but always
+ * binary-searches (include/ops/Intersection.cuh:182-193). Arcs (u,v) fall in bin 16*min(15, log2 d+(u)) + min(15, log2 d+(v)). A route table assigns each
  * of the 256 bins one route; counts are exact for every table. */
 enum {
   GARY_ROUTE_BITMAP = 0,         /* streamed through the pivot's LDS set */
@@ -263,9 +264,9 @@ int gary_mlp_loss_grad(const float *h_params, const float *h_x, const float *h_t
 int gary_selector_train(const float *h_x, const float *h_t, const float *h_mask, int32_t n,
                         int32_t epochs, float lr, uint64_t seed, float *h_loss);
 /* Model parameters in/out; `trained` enables GARY_TC_AUTO_LEARNED (else it runs the heuristic
- * table); `allowed` is a bit mask of the routes the model may pick. */
-/* margin: the learned table leaves the all-bitmap routing only when another route subset is
- * predicted to save more than this fraction of its time (0 <= margin < 1). */
+ * table); `allowed` is a bit mask of the routes the model may pick; the learned table leaves the
+ * all-bitmap routing only when another route subset is predicted to save more than `margin`
+ * (a fraction, 0 <= margin < 1) of its time. */
 int gary_selector_set_params(const float *h_params, int trained, float cost_big, uint32_t allowed,
                              float margin);
 int gary_selector_get_params(float *h_params);
@@ -277,6 +278,9 @@ int gary_selector_set_fixed_costs(int32_t npoints, const float *h_nnz, const flo
 int gary_selector_learned_table(const gary_device_graph *g, uint8_t *h_routes, float *h_costs);
 
 /* ---- batched triangle counting (one persistent launch over many graphs, hipGraph) -------- */
+/* No counterpart in the reference; its batched shapes are the batched search test kernels
+ * (test/SearchTest.cu:69-136) and the block-scope queue CTAQueue (include/ops/DeviceQueue.cuh:33-182),
+ * whose role the persistent kernels' atomic work cursors take here. */
 typedef struct gary_batch gary_batch;
 /* The batch is the disjoint union of its graphs in one oriented CSR: graph g owns the vertex ids
  * [vbase[g], vbase[g+1]) with vbase the prefix sum of graph_vertices, and every arc stays inside
