@@ -24,6 +24,20 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Calls f(key, mask, leader) once per distinct key among the wave's lanes with key >= 0 (every
+// lane of the wave must call it): mask = the lanes holding that key, leader = the lowest of them.
+template <class F>
+__device__ __forceinline__ void for_each_key(int key, F &&f) {
+  uint64_t todo = __ballot(key >= 0);
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int k = __shfl(key, leader, kWave);
+    const uint64_t mask = __ballot(key == k);
+    f(k, mask, leader);
+    todo &= ~mask;
+  }
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, kWave);
@@ -57,6 +71,19 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan_u32(uint32_t v) {
     uint32_t t = (uint32_t)__shfl_up((int)v, o, kWave);
     if (l >= o) v += t;
   }
+  return v;
+}
+
+// Inclusive max-scan over the wave with DPP: row shifts by 1, 2, 4, 8 inside each 16-lane row,
+// then the last lane of rows 0 and 2 into rows 1 and 3 (row_bcast:15), then lane 31 into rows
+// 2 and 3 (row_bcast:31). Lanes without a source keep INT32_MIN (the identity).
+__device__ __forceinline__ int32_t wave_inclusive_max_i32(int32_t v) {
+  v = max(v, __builtin_amdgcn_update_dpp((int)INT32_MIN, v, 0x111, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp((int)INT32_MIN, v, 0x112, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp((int)INT32_MIN, v, 0x114, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp((int)INT32_MIN, v, 0x118, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp((int)INT32_MIN, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp((int)INT32_MIN, v, 0x143, 0xc, 0xf, false));
   return v;
 }
 

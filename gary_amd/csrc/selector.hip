@@ -161,12 +161,17 @@ __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int
   const int64_t nruns = (nnz + kWave - 1) / kWave;
   for (int64_t k = wave0 * hist_every; k < nruns; k += nwaves * hist_every) {
     const int64_t e = k * kWave + lane;
-    if (e >= nnz) continue;
-    const int32_t u = g.coo.rowIdx[e], v = g.coo.colIdx[e];
-    if ((uint32_t)u >= (uint32_t)m || (uint32_t)v >= (uint32_t)m) continue;
-    const int32_t a = g.csr.deg[u], b = g.csr.deg[v];
-    if (a < 2 || b < 1) continue;
-    atomicAdd(&s_arcs[ilog2c(a) * 16 + ilog2c(b)], (float)hist_every);
+    int bin = -1;
+    if (e < nnz) {
+      const int32_t u = g.coo.rowIdx[e], v = g.coo.colIdx[e];
+      if ((uint32_t)u < (uint32_t)m && (uint32_t)v < (uint32_t)m) {
+        const int32_t a = g.csr.deg[u], b = g.csr.deg[v];
+        if (a >= 2 && b >= 1) bin = ilog2c(a) * 16 + ilog2c(b);
+      }
+    }
+    for_each_key(bin, [&](int key, uint64_t mask, int leader) {  // one LDS atomic per bin
+      if (lane == leader) atomicAdd(&s_arcs[key], (float)(__popcll(mask) * hist_every));
+    });
   }
   const int64_t novl = (nnz + ovl_every - 1) / ovl_every;
   for (int64_t k = wave0; k < novl; k += nwaves) {

@@ -311,23 +311,25 @@ __device__ __forceinline__ uint32_t pivot_probe(const int32_t *t, const PivotSet
   }
 }
 
-// Per-wave LDS scratch of the flattened row streaming: 64 rows of metadata.
-struct RowBatch {
-  int32_t end[kWave];   // inclusive prefix of the rows' 16-byte chunk counts
-  int32_t dv[kWave];    // row lengths
-  int64_t pv[kWave];    // row starts (element index into colIdx)
+// Per-wave LDS scratch of the flattened row streaming: one slot per lane.
+struct RowSlots {
+  int32_t s[kWave];
 };
 
 // The rows r0, r0 + stride, ... (< d) of pivot list L, 64 rows per batch. A batch is flattened
-// into the 16-byte chunks that overlap its rows; lane l takes chunks q0 + U*l .. q0 + U*l + U-1,
-// so one binary search over the batch's chunk prefix (LDS) places the lane, its U loads are
-// issued together, and the wave's loads cover one contiguous stretch of colIdx. Elements of a
-// chunk outside the chunk's row are masked; the array's last partial chunk is read element-wise.
-template <int U>
+// into the 16-byte chunks that overlap its rows; each window of 64 chunks gives lane l chunk
+// w0 + l, so a wave's loads cover one contiguous stretch of colIdx, and K windows are loaded
+// before any is probed. The row owning chunk w0 + l is the last row starting at or before it:
+// each row with chunks writes its lane number into the slot of its first chunk in the window
+// (the row running into the window writes slot 0), and an inclusive max-scan over the slots
+// hands every lane its row; the row's start, first chunk and element window come from that
+// row's lane by shuffles. Elements of a chunk outside its row are masked; the array's last
+// partial chunk is read element-wise. Chunk indices are 32-bit (colIdx below 2^34 elements).
+template <int K>
 __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
                                                 const int32_t *__restrict__ L, int32_t d,
                                                 int32_t r0, int32_t stride, const int32_t *table,
-                                                const PivotSet &ps, RowBatch &rb, int lane,
+                                                const PivotSet &ps, RowSlots &rs, int lane,
                                                 const uint8_t *__restrict__ Lroute) {
   const uint32_t m = (uint32_t)g.csr.m;
   const int64_t nnz = g.csr.nnz;
@@ -346,60 +348,48 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
     }
     const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + 3) >> 2) - (pv >> 2)) : 0;
     const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
+    const int32_t start = incl - nch;  // the row's first chunk in the batch
     const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
-    rb.end[lane] = incl;
-    rb.dv[lane] = dv;
-    rb.pv[lane] = pv;
-    wave_sync();
-    for (int32_t q0 = 0; q0 < total; q0 += U * kWave) {
-      const int32_t qb = q0 + lane * U;
-      if (qb >= total) continue;
-      int32_t k = 0;  // first row with end > qb
+    const int32_t cb = (int32_t)(uint32_t)(pv >> 2);  // the row's first chunk in colIdx
+    const int32_t rlo = (int32_t)(pv & 3);             // its first element's slot in that chunk
+    const int32_t rhi = rlo + dv;                      // one past its last element's slot
+    for (int32_t q0 = 0; q0 < total; q0 += K * kWave) {
+      int4 data[K];
+      int32_t jlo[K], jhi[K];
 #pragma unroll
-      for (int32_t half = kWave / 2; half > 0; half >>= 1)
-        if (rb.end[k + half - 1] <= qb) k += half;
-      k += (rb.end[k] <= qb);
-      int32_t kend = rb.end[k];
-      int32_t first = k ? rb.end[k - 1] : 0;
-      int64_t lo = rb.pv[k];
-      int32_t len = rb.dv[k];
-      int4 data[U];
-      int32_t jlo[U], jhi[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int32_t q = qb + u;
-        jlo[u] = 4;
-        jhi[u] = 0;
-        if (q < total) {
-          while (q >= kend) {  // rows without chunks share their predecessor's end
-            ++k;
-            first = kend;
-            kend = rb.end[k];
-            lo = rb.pv[k];
-            len = rb.dv[k];
-          }
-          const int64_t e0 = 4 * ((lo >> 2) + (q - first));
-          jlo[u] = (int32_t)max<int64_t>(0, lo - e0);
-          jhi[u] = (int32_t)min<int64_t>(4, lo + len - e0);
+      for (int k = 0; k < K; ++k) {
+        const int32_t w0 = q0 + k * kWave;
+        rs.s[lane] = -1;
+        wave_sync();
+        if (nch > 0 && start < w0 + kWave && incl > w0) rs.s[max(0, start - w0)] = lane;
+        wave_sync();
+        const int32_t row = max(0, wave_inclusive_max_i32(rs.s[lane]));
+        const int32_t j = w0 + lane - __shfl(start, row, kWave);  // chunk index within the row
+        const uint32_t ch = (uint32_t)__shfl(cb, row, kWave) + (uint32_t)j;
+        jlo[k] = max(0, __shfl(rlo, row, kWave) - 4 * j);
+        jhi[k] = min(4, __shfl(rhi, row, kWave) - 4 * j);
+        data[k] = make_int4(0, 0, 0, 0);
+        if (w0 + lane < total) {
+          const int64_t e0 = 4 * (int64_t)ch;
           if (e0 + 3 < nnz) {
-            data[u] = *reinterpret_cast<const int4 *>(col + e0);
+            data[k] = *reinterpret_cast<const int4 *>(col + e0);
           } else {
-            data[u].x = e0 < nnz ? col[e0] : 0;
-            data[u].y = e0 + 1 < nnz ? col[e0 + 1] : 0;
-            data[u].z = e0 + 2 < nnz ? col[e0 + 2] : 0;
-            data[u].w = 0;
+            data[k].x = e0 < nnz ? col[e0] : 0;
+            data[k].y = e0 + 1 < nnz ? col[e0 + 1] : 0;
+            data[k].z = e0 + 2 < nnz ? col[e0 + 2] : 0;
           }
+        } else {
+          jhi[k] = 0;
         }
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        c += (jlo[u] <= 0 && 0 < jhi[u]) ? pivot_probe(table, ps, data[u].x) : 0u;
-        c += (jlo[u] <= 1 && 1 < jhi[u]) ? pivot_probe(table, ps, data[u].y) : 0u;
-        c += (jlo[u] <= 2 && 2 < jhi[u]) ? pivot_probe(table, ps, data[u].z) : 0u;
-        c += (jlo[u] <= 3 && 3 < jhi[u]) ? pivot_probe(table, ps, data[u].w) : 0u;
+      for (int k = 0; k < K; ++k) {
+        c += (jlo[k] <= 0 && 0 < jhi[k]) ? pivot_probe(table, ps, data[k].x) : 0u;
+        c += (jlo[k] <= 1 && 1 < jhi[k]) ? pivot_probe(table, ps, data[k].y) : 0u;
+        c += (jlo[k] <= 2 && 2 < jhi[k]) ? pivot_probe(table, ps, data[k].z) : 0u;
+        c += (jlo[k] <= 3 && 3 < jhi[k]) ? pivot_probe(table, ps, data[k].w) : 0u;
       }
     }
-    wave_sync();  // rb is rewritten by the next batch
   }
   return c;
 }
@@ -447,13 +437,12 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     const int32_t *__restrict__ split) {
   __shared__ int32_t table[TABLE_WORDS];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
-  __shared__ RowBatch s_rows[WAVES];
-  // No separate LDS word for the pivot index: at <4,4096> the kernel's LDS is then exactly
-  // 20 KiB, and eight workgroups fit a CU (160 KiB) instead of seven. The index is passed
-  // through table[0] between pivots, when the set is dead.
+  __shared__ RowSlots s_rows[WAVES];
+  // No separate LDS word for the pivot index (it is passed through table[0] between pivots,
+  // when the set is dead): every word kept out of LDS helps eight workgroups fit a CU.
   const int lane = lane_id();
   const int wib = threadIdx.x / kWave;
-  RowBatch &rb = s_rows[wib];
+  RowSlots &rb = s_rows[wib];
   const int32_t n_small = counts[0], n_big = counts[1];
   if (n_small == 0 && n_big == 0) return;
   const uint32_t m = (uint32_t)g.csr.m;
@@ -579,19 +568,6 @@ __device__ __forceinline__ int pivot_key(const int32_t *__restrict__ deg, int64_
     return (cst > cost_big ? kBuckets : 0) + (15 - lc);
   }
   return (d <= t_small ? 0 : kBuckets) + (15 - lg);
-}
-
-// Calls f(key, mask, leader) once per distinct key of the wave's active lanes (key >= 0).
-template <class F>
-__device__ __forceinline__ void for_each_key(int key, F &&f) {
-  uint64_t todo = __ballot(key >= 0);
-  while (todo) {
-    const int leader = __ffsll((unsigned long long)todo) - 1;
-    const int k = __shfl(key, leader, kWave);
-    const uint64_t mask = __ballot(key == k);
-    f(k, mask, leader);
-    todo &= ~mask;
-  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -783,6 +759,8 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
   if (m == 0 || g.csr.nnz == 0) return GARY_OK;
   GARY_CHECK_ARG(((uintptr_t)g.csr.colIdx & 15) == 0,
                  "pivot strategies read colIdx in 16-byte chunks: colIdx must be 16-byte aligned");
+  GARY_CHECK_ARG(g.csr.nnz < (1ll << 34),
+                 "pivot strategies index colIdx in 32-bit chunks: nnz must be below 2^34");
   // counters: [0] #small, [1] #big, [2] always 0, [3] #hub; cursors: [4..5] main, [6..7] hub;
   // then the binning histogram and scatter cursors
   fill_zero_kernel<<<1, 256, 0, st>>>((uint32_t *)counters, kPivotCounterWords);
