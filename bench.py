@@ -136,7 +136,7 @@ class Bench:
             if kms is not None:
                 kernel_ms[nm] = kms
             log(f"{nm}: {ms:.3f} ms/step" + (f", triangles={tri}" if nm == a.strategy else ""))
-        self.dispatch_ms = self.time_dispatch(ctx, tri)
+        self.dispatch_ms = None if a.only else self.time_dispatch(ctx, tri)
         pivot_path = a.strategy in ("learned", "heuristic", "bitmap")
         k_ms = kernel_ms.get(a.strategy, results[a.strategy])
         k_bytes = B_alg_main if pivot_path else B_alg
