@@ -142,8 +142,8 @@ __global__ void __launch_bounds__(256)
 // sums[bin] = {arcs seen, overlap samples, Σ |N+(u) ∩ N+(v)| / min(d+(u), d+(v))}. Arcs are
 // sampled in runs of 64 consecutive ones (one wave, coalesced COO reads): every `hist_every`-th
 // run is counted (scaled back by hist_every). Every arc whose index is a multiple of `ovl_every`
-// also measures its overlap, one wave per sampled arc: the lanes stride over the shorter list and
-// binary-search the longer one, so no thread walks a whole list alone.
+// also measures its overlap, one wave per sampled arc: up to 64 evenly spaced keys of the shorter
+// list, one per lane, are binary-searched in the longer one (the overlap is the hit fraction).
 struct BinSums {
   float arcs, samples, ovl;
 };
@@ -184,8 +184,12 @@ __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int
     const int32_t *A = g.csr.colIdx + g.csr.indPtr[u], *B = g.csr.colIdx + g.csr.indPtr[v];
     int32_t na = a, nb = b;
     if (na > nb) { const int32_t *t = A; A = B; B = t; int32_t tn = na; na = nb; nb = tn; }
+    // up to 64 keys of the shorter list, evenly spaced, one per lane (a bounded chain of
+    // dependent loads per sample)
+    const int32_t ns = min(na, kWave);
     uint32_t hit = 0;
-    for (int32_t i = lane; i < na; i += kWave) {
+    if (lane < ns) {
+      const int32_t i = (int32_t)(((int64_t)lane * na) / ns);
       int32_t l = 0, r = nb - 1;
       const int32_t key = A[i];
       while (l <= r) {
@@ -199,7 +203,7 @@ __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int
     hit = wave_sum_u32(hit);
     if (lane == 0) {
       atomicAdd(&s_samp[bin], 1.f);
-      atomicAdd(&s_ovl[bin], (float)hit / (float)na);
+      atomicAdd(&s_ovl[bin], (float)hit / (float)ns);
     }
   }
   __syncthreads();

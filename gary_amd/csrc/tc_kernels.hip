@@ -770,7 +770,7 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                        0, st>>>((uint32_t *)gcount, (int64_t)ngraphs * 2);
     GARY_HIP(hipGetLastError());
   }
-  const int bgrid = (int)std::min<int64_t>(((int64_t)m + 4095) / 4096, 1024);
+  const int bgrid = (int)std::min<int64_t>(((int64_t)m + 1023) / 1024, 4096);
   GARY_CHECK_ARG(pc.waves == 2 || pc.waves == 4 || pc.waves == 8,
                  "pivot path: %d waves per workgroup (2, 4 or 8)", pc.waves);
   GARY_CHECK_ARG(pc.bin_small >= 2 && pc.bin_small <= kPivotWords / pc.waves / 2 &&
