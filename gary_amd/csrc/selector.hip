@@ -527,7 +527,12 @@ __global__ void fill_zero_kernel_sel(uint32_t *__restrict__ p, int32_t n) {
 int compute_bin_sums(const gary_device_graph &g, BinSums *sums, hipStream_t st) {
   GARY_HIP(hipMemsetAsync(sums, 0, sizeof(BinSums) * kRouteBins, st));
   if (g.csr.nnz == 0) return GARY_OK;
-  bin_features_kernel<<<1024, 256, 0, st>>>(g, 8, 2048, sums);
+  // a bounded sample whatever the graph size: at most ~32K runs of 64 arcs for the histogram
+  // and ~8K arcs for the overlap (every 8th run and every 2048th arc on smaller graphs)
+  const int64_t nruns = (g.csr.nnz + kWave - 1) / kWave;
+  const int32_t hist_every = (int32_t)std::max<int64_t>(8, (nruns + 32767) / 32768);
+  const int32_t ovl_every = (int32_t)std::max<int64_t>(2048, (g.csr.nnz + 8191) / 8192);
+  bin_features_kernel<<<1024, 256, 0, st>>>(g, hist_every, ovl_every, sums);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
 }

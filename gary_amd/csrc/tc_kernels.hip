@@ -548,9 +548,10 @@ constexpr int kCurOff = 8 + kKeys;
 constexpr int kSplitOff = kCurOff + kKeys;
 static_assert(kSplitOff + 4 <= kPivotCounterWords, "pivot counter words");
 
-// ordered == false (per-graph counting of batched unions) puts every pivot in the lightest
-// bucket: the lists then keep vertex order, so consecutive pivots of a wave mostly belong to one
-// graph and its per-graph partial count is flushed rarely.
+// ordered == false (per-graph counting of batched unions) puts every small pivot in the
+// lightest bucket: the small list then keeps vertex order, so consecutive pivots of a wave mostly
+// belong to one graph and its per-graph partial count is flushed rarely (big and hub pivots
+// flush once each anyway and stay heaviest first).
 __device__ __forceinline__ int pivot_key(const int32_t *__restrict__ deg, int64_t v, int64_t end,
                                          int32_t t_small, int32_t t_big, uint32_t rowmask,
                                          const float *__restrict__ pcost, float cost_big,
@@ -559,15 +560,15 @@ __device__ __forceinline__ int pivot_key(const int32_t *__restrict__ deg, int64_
   const int32_t d = deg[v];
   if (d < 2) return -1;
   if (!((rowmask >> min(15, 31 - __clz(d))) & 1u)) return -1;
-  const int lg = ordered ? min(15, 31 - __clz(d)) : 0;
+  const int lg = min(15, 31 - __clz(d));
   if (d > t_big) return 2 * kBuckets + (15 - lg);
   if (pcost && cost_big > 0.f) {
     const float cst = pcost[v];
     if (!(cst > 0.f)) return -1;
-    const int lc = ordered ? min(15, max(0, (int)log2f(cst))) : 0;
-    return (cst > cost_big ? kBuckets : 0) + (15 - lc);
+    const int lc = min(15, max(0, (int)log2f(cst)));
+    return cst > cost_big ? kBuckets + (15 - lc) : (ordered ? 15 - lc : 15);
   }
-  return (d <= t_small ? 0 : kBuckets) + (15 - lg);
+  return d <= t_small ? (ordered ? 15 - lg : 15) : kBuckets + (15 - lg);
 }
 
 __global__ void __launch_bounds__(256)
