@@ -349,7 +349,7 @@ __device__ __forceinline__ int32_t lower_bound(const int32_t *__restrict__ N, in
 __global__ void __launch_bounds__(kHomBlock)
     c4_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
               uint32_t *__restrict__ counters, int32_t *__restrict__ touched_all,
-              int32_t *__restrict__ cursor, unsigned long long *c4_acc) {
+              int32_t *__restrict__ cursor, unsigned long long *c4_acc, int32_t heavy_lo) {
   __shared__ int32_t s_item, s_nt;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t m = g.csr.m;
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(kHomBlock)
     if (u >= m) break;
     const int32_t nl = lo_len[u];
     const int32_t *Nu = col + g.csr.indPtr[u];
-    if (nl >= 2) {
+    if (nl >= 2 && nl < heavy_lo) {  // vertices with heavy_lo or more lower neighbours: c4_window_kernel
       for (int32_t k = wib; k < nl; k += kHomWaves) {
         const int32_t v = Nu[k];
         const int32_t *Nv = col + g.csr.indPtr[v];
@@ -404,6 +404,126 @@ __global__ void __launch_bounds__(kHomBlock)
   if (lane == 0) add_u128(c4_acc, s);
 }
 
+// ---- 4-cycles at heavy vertices: endpoint windows in LDS -----------------------------------------
+//
+// A vertex u with many lower neighbours has most of the wedges and endpoints spread over all of
+// [0, u), so its dense counters would live in HBM. Here the endpoint range is cut into windows of
+// kC4Win ids whose counters fit LDS; a work item is (u, kC4Group consecutive windows). The
+// endpoints of N(v) in a window form one run of N(v) (sorted) that starts at a per-neighbour
+// cursor: its first four elements are read together, and only a run longer than that is ended
+// by a binary search. The runs of 64 neighbours v are flattened over a wave (as in the triangle
+// kernel's row streaming) so that every lane counts one endpoint per step.
+constexpr int kC4HeavyLo = 256; // lower neighbours from which a vertex takes the window path
+constexpr int kC4Win = 16384;   // 64 KiB of u32 counters
+constexpr int kC4Group = 16;    // windows per work item
+constexpr int kC4Block = 512;
+constexpr int kC4Waves = kC4Block / kWave;
+
+// heavy[0 .. *nheavy) = the vertices with at least heavy_lo lower neighbours (any order);
+// *max_lo = their largest lower-neighbour count.
+__global__ void heavy_list_kernel(const int32_t *__restrict__ lo_len, int32_t m, int32_t heavy_lo,
+                                  int32_t *__restrict__ heavy, int32_t *__restrict__ nheavy,
+                                  int32_t *__restrict__ max_lo) {
+  const int lane = lane_id();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < m;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = base + lane;
+    const bool h = u < m && lo_len[u] >= heavy_lo;
+    const uint64_t mask = __ballot(h);
+    if (!mask) continue;
+    int32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(nheavy, __popcll(mask));
+    pos = __shfl(pos, 0, kWave);
+    if (h) {
+      heavy[pos + lanes_below(mask)] = (int32_t)u;
+      atomicMax(max_lo, lo_len[u]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kC4Block)
+    c4_window_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
+                     const int32_t *__restrict__ heavy, const int32_t *__restrict__ nheavy_p,
+                     int32_t jmax, int64_t *__restrict__ cursors, int64_t cursor_stride,
+                     int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
+  __shared__ uint32_t cnt[kC4Win];
+  __shared__ int32_t slots[kC4Waves][kWave];
+  __shared__ int32_t s_item;
+  const int lane = lane_id(), wib = threadIdx.x / kWave;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  const int64_t nheavy = *nheavy_p;
+  const int64_t nitems = nheavy * jmax;
+  int64_t *cur = cursors + (int64_t)blockIdx.x * cursor_stride;
+  uint64_t acc = 0;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(item_cursor, 1);
+    __syncthreads();
+    const int64_t it = s_item;
+    __syncthreads();  // everyone holds the item before s_item is written again
+    if (it >= nitems) break;
+    const int32_t u = heavy[it % nheavy];
+    const int32_t j = (int32_t)(it / nheavy);
+    const int32_t K = (u + kC4Win - 1) / kC4Win;
+    const int32_t k0 = j * kC4Group, k1 = min(K, k0 + kC4Group);
+    if (k0 >= K) continue;
+    const int32_t nl = lo_len[u];
+    const int32_t *Nu = col + g.csr.indPtr[u];
+    for (int32_t i = threadIdx.x; i < nl; i += kC4Block) {  // cursors at the first window
+      const int32_t v = Nu[i];
+      const int64_t pv = g.csr.indPtr[v];
+      const int32_t dv = g.csr.deg[v];
+      cur[2 * i] = pv + lower_bound(col + pv, dv, k0 * kC4Win);
+      cur[2 * i + 1] = pv + dv;
+    }
+    __syncthreads();
+    for (int32_t k = k0; k < k1; ++k) {
+      const int32_t wlo = k * kC4Win, whi = min(wlo + kC4Win, u);
+      for (int32_t x = threadIdx.x; x < kC4Win; x += kC4Block) cnt[x] = 0u;
+      __syncthreads();
+      for (int32_t i0 = wib * kWave; i0 < nl; i0 += kC4Block) {
+        const int32_t i = i0 + lane;
+        int64_t p = 0;
+        int32_t len = 0;
+        if (i < nl) {  // endpoints w in [wlo, whi): a prefix of N(v) from the cursor
+          p = cur[2 * i];
+          const int64_t vend = cur[2 * i + 1];
+          const int32_t e0 = p < vend ? col[p] : INT32_MAX;
+          const int32_t e1 = p + 1 < vend ? col[p + 1] : INT32_MAX;
+          const int32_t e2 = p + 2 < vend ? col[p + 2] : INT32_MAX;
+          const int32_t e3 = p + 3 < vend ? col[p + 3] : INT32_MAX;
+          len = (e0 < whi) + (e1 < whi) + (e2 < whi) + (e3 < whi);
+          if (len == 4 && p + 4 < vend) len = 4 + lower_bound(col + p + 4, (int32_t)(vend - p - 4), whi);
+          cur[2 * i] = p + len;
+        }
+        const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)len);
+        const int32_t start = incl - len;
+        const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+        for (int32_t w0 = 0; w0 < total; w0 += kWave) {
+          slots[wib][lane] = -1;
+          wave_sync();
+          if (len > 0 && start < w0 + kWave && incl > w0) slots[wib][max(0, start - w0)] = lane;
+          wave_sync();
+          const int32_t row = max(0, wave_inclusive_max_i32(slots[wib][lane]));
+          const int64_t rp = __shfl(p, row, kWave);
+          const int32_t rs = __shfl(start, row, kWave);
+          if (w0 + lane < total) {
+            const int32_t w = col[rp + (w0 + lane - rs)];
+            atomicAdd(&cnt[w - wlo], 1u);
+          }
+        }
+      }
+      __syncthreads();
+      for (int32_t x = threadIdx.x; x < whi - wlo; x += kC4Block) {
+        const uint64_t c = cnt[x];
+        acc += c * (c - (c > 0)) / 2;
+      }
+      __syncthreads();
+    }
+  }
+  const uint64_t sacc = wave_sum_u64(acc);
+  if (lane == 0) add_u128(c4_acc, sacc);
+}
+
 }  // namespace
 }  // namespace gary
 
@@ -430,6 +550,7 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   // many: 2.1 s, against 2.06 s).
   const int grid_local = std::max(1, ncu * 4);
   const int grid_c4 = std::max(1, ncu);
+  const int grid_win = std::max(1, ncu * 2);  // 66 KiB of LDS each: two per CU
   const int64_t scratch_words = (int64_t)kMaxHomDeg * (kMaxHomDeg / 64) * 2;
   // workspace layout (bytes, 256-aligned pieces)
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -443,10 +564,11 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   const size_t b_rows = need_local ? al(sizeof(uint32_t) * (size_t)scratch_words * grid_local) : 0;
   const size_t b_cnt = need_c4 ? al(sizeof(uint32_t) * (size_t)m * grid_c4) : 0;
   const size_t b_touched = need_c4 ? al(sizeof(int32_t) * (size_t)m * grid_c4) : 0;
+  const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * (size_t)m) : 0;
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)3,
                          b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_cnt +
-                             b_touched, &ws));
+                             b_touched + b_heavy, &ws));
   char *p = (char *)ws;
   int64_t *up_start = (int64_t *)p; p += b_up_start;
   int32_t *up_len = (int32_t *)p; p += b_up_len;
@@ -455,8 +577,11 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   char *misc = p; p += b_misc;
   uint32_t *rows = (uint32_t *)p; p += b_rows;
   uint32_t *cnts = (uint32_t *)p; p += b_cnt;
-  int32_t *touched = (int32_t *)p;
-  int32_t *bad = (int32_t *)misc;               // [0] bad, [1] unsupported, [2..3] cursors
+  int32_t *touched = (int32_t *)p; p += b_touched;
+  int32_t *heavy = (int32_t *)p;
+  // [0] bad, [1] unsupported, [2..3] cursors, [4] #heavy, [5] max lower degree of a heavy
+  // vertex, [6] window-item cursor
+  int32_t *bad = (int32_t *)misc;
   HomTotals *tot = (HomTotals *)(misc + 64);    // tri, k4
   unsigned long long *sums = (unsigned long long *)(misc + 128);  // t2 lo/hi, c2 lo/hi, c4 lo/hi
   GARY_HIP(hipMemsetAsync(misc, 0, b_misc, st));
@@ -473,8 +598,27 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   hom_sums_kernel<<<1024, 256, 0, st>>>(*g, up_start, up_len, tcount, sums, sums + 2);
   GARY_HIP(hipGetLastError());
   if (need_c4) {
-    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, cnts, touched, bad + 3, sums + 4);
+    heavy_list_kernel<<<std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
+        lo_len, m, kC4HeavyLo, heavy, bad + 4, bad + 5);
     GARY_HIP(hipGetLastError());
+    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, cnts, touched, bad + 3, sums + 4,
+                                              kC4HeavyLo);
+    GARY_HIP(hipGetLastError());
+    int32_t hv[2];  // #heavy, max lower degree: sizes the per-workgroup cursor slices
+    GARY_HIP(hipMemcpyAsync(hv, bad + 4, sizeof hv, hipMemcpyDeviceToHost, st));
+    GARY_HIP(hipStreamSynchronize(st));
+    if (hv[0] > 0) {
+      const int64_t stride = std::max<int32_t>(hv[1], 1);
+      void *cws = nullptr;
+      GARY_TRY(workspace_get(dev, (void *)4, 2 * sizeof(int64_t) * (size_t)stride * grid_win, &cws));
+      const int32_t windows = (m + kC4Win - 1) / kC4Win;
+      const int32_t jmax = (windows + kC4Group - 1) / kC4Group;
+      GARY_CHECK_ARG((int64_t)hv[0] * jmax < INT32_MAX, "gary_hom4_count: too many window items");
+      c4_window_kernel<<<grid_win, kC4Block, 0, st>>>(*g, lo_len, heavy, bad + 4, jmax,
+                                                      (int64_t *)cws, 2 * stride, bad + 6,
+                                                      sums + 4);
+      GARY_HIP(hipGetLastError());
+    }
   }
   int32_t flags[2];
   unsigned long long asym = 0;
