@@ -414,10 +414,12 @@ __global__ void __launch_bounds__(kHomBlock)
 // by a binary search. The runs of 64 neighbours v are flattened over a wave (as in the triangle
 // kernel's row streaming) so that every lane counts one endpoint per step.
 constexpr int kC4HeavyLo = 256; // lower neighbours from which a vertex takes the window path
-constexpr int kC4Win = 16384;   // 64 KiB of u32 counters
-constexpr int kC4Group = 16;    // windows per work item
-constexpr int kC4Block = 512;
+constexpr int kC4Win = 32768;   // 128 KiB of u32 counters: one workgroup per CU
+constexpr int kC4Group = 8;     // windows per work item
+constexpr int kC4Block = 1024;
 constexpr int kC4Waves = kC4Block / kWave;
+constexpr int kC4Touch = 2048;  // endpoints listed per window; a window with more is swept whole
+constexpr int kC4Steps = 4;     // 64-endpoint steps whose loads are issued together
 
 // heavy[0 .. *nheavy) = the vertices with at least heavy_lo lower neighbours (any order);
 // *max_lo = their largest lower-neighbour count.
@@ -447,14 +449,17 @@ __global__ void __launch_bounds__(kC4Block)
                      int32_t jmax, int64_t *__restrict__ cursors, int64_t cursor_stride,
                      int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
   __shared__ uint32_t cnt[kC4Win];
+  __shared__ int32_t touch[kC4Touch];  // window offsets whose counter left zero
   __shared__ int32_t slots[kC4Waves][kWave];
-  __shared__ int32_t s_item;
+  __shared__ int32_t s_item, s_ntouch;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int64_t nheavy = *nheavy_p;
   const int64_t nitems = nheavy * jmax;
   int64_t *cur = cursors + (int64_t)blockIdx.x * cursor_stride;
   uint64_t acc = 0;
+  for (int32_t x = threadIdx.x; x < kC4Win; x += kC4Block) cnt[x] = 0u;  // kept zero between windows
+  if (threadIdx.x == 0) s_ntouch = 0;
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(item_cursor, 1);
     __syncthreads();
@@ -478,8 +483,6 @@ __global__ void __launch_bounds__(kC4Block)
     __syncthreads();
     for (int32_t k = k0; k < k1; ++k) {
       const int32_t wlo = k * kC4Win, whi = min(wlo + kC4Win, u);
-      for (int32_t x = threadIdx.x; x < kC4Win; x += kC4Block) cnt[x] = 0u;
-      __syncthreads();
       for (int32_t i0 = wib * kWave; i0 < nl; i0 += kC4Block) {
         const int32_t i = i0 + lane;
         int64_t p = 0;
@@ -498,25 +501,52 @@ __global__ void __launch_bounds__(kC4Block)
         const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)len);
         const int32_t start = incl - len;
         const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
-        for (int32_t w0 = 0; w0 < total; w0 += kWave) {
-          slots[wib][lane] = -1;
-          wave_sync();
-          if (len > 0 && start < w0 + kWave && incl > w0) slots[wib][max(0, start - w0)] = lane;
-          wave_sync();
-          const int32_t row = max(0, wave_inclusive_max_i32(slots[wib][lane]));
-          const int64_t rp = __shfl(p, row, kWave);
-          const int32_t rs = __shfl(start, row, kWave);
-          if (w0 + lane < total) {
-            const int32_t w = col[rp + (w0 + lane - rs)];
-            atomicAdd(&cnt[w - wlo], 1u);
+        // kC4Steps steps of 64 endpoints per trip: their owners first, then all loads, then
+        // all counter updates, so the loads are in flight together
+        for (int32_t w0 = 0; w0 < total; w0 += kC4Steps * kWave) {
+          int64_t idx[kC4Steps];
+#pragma unroll
+          for (int r = 0; r < kC4Steps; ++r) {
+            const int32_t ws = w0 + r * kWave;
+            slots[wib][lane] = -1;
+            wave_sync();
+            if (len > 0 && start < ws + kWave && incl > ws) slots[wib][max(0, start - ws)] = lane;
+            wave_sync();
+            const int32_t row = max(0, wave_inclusive_max_i32(slots[wib][lane]));
+            const int64_t rp = __shfl(p, row, kWave);
+            const int32_t rs = __shfl(start, row, kWave);
+            idx[r] = ws + lane < total ? rp + (ws + lane - rs) : -1;
+            wave_sync();  // the slots are rewritten by the next step
+          }
+          int32_t x[kC4Steps];
+#pragma unroll
+          for (int r = 0; r < kC4Steps; ++r) x[r] = idx[r] >= 0 ? col[idx[r]] - wlo : -1;
+#pragma unroll
+          for (int r = 0; r < kC4Steps; ++r) {
+            if (x[r] >= 0 && atomicAdd(&cnt[x[r]], 1u) == 0u) {
+              const int32_t t = atomicAdd(&s_ntouch, 1);
+              if (t < kC4Touch) touch[t] = x[r];
+            }
           }
         }
       }
       __syncthreads();
-      for (int32_t x = threadIdx.x; x < whi - wlo; x += kC4Block) {
-        const uint64_t c = cnt[x];
-        acc += c * (c - (c > 0)) / 2;
+      const int32_t nt = s_ntouch;
+      if (nt <= kC4Touch) {  // read and clear the listed counters only
+        for (int32_t t = threadIdx.x; t < nt; t += kC4Block) {
+          const uint64_t c = cnt[touch[t]];
+          cnt[touch[t]] = 0u;
+          acc += c * (c - 1) / 2;
+        }
+      } else {
+        for (int32_t x = threadIdx.x; x < whi - wlo; x += kC4Block) {
+          const uint64_t c = cnt[x];
+          cnt[x] = 0u;
+          acc += c * (c - (c > 0)) / 2;
+        }
       }
+      __syncthreads();
+      if (threadIdx.x == 0) s_ntouch = 0;
       __syncthreads();
     }
   }
@@ -550,7 +580,7 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   // many: 2.1 s, against 2.06 s).
   const int grid_local = std::max(1, ncu * 4);
   const int grid_c4 = std::max(1, ncu);
-  const int grid_win = std::max(1, ncu * 2);  // 66 KiB of LDS each: two per CU
+  const int grid_win = std::max(1, ncu);  // 140 KiB of LDS each: one per CU
   const int64_t scratch_words = (int64_t)kMaxHomDeg * (kMaxHomDeg / 64) * 2;
   // workspace layout (bytes, 256-aligned pieces)
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
