@@ -408,14 +408,17 @@ __global__ void __launch_bounds__(kHomBlock)
 //
 // A vertex u with many lower neighbours has most of the wedges and endpoints spread over all of
 // [0, u), so its dense counters would live in HBM. Here the endpoint range is cut into windows of
-// kC4Win ids whose counters fit LDS; a work item is (u, kC4Group consecutive windows). The
+// kC4Win ids whose counters fit LDS (16-bit counters, two per word: a count never exceeds the
+// lower degree of u; a vertex with 65536 or more lower neighbours counts each window in two
+// halves with 32-bit counters); a work item is (u, kC4Group consecutive windows). The
 // endpoints of N(v) in a window form one run of N(v) (sorted) that starts at a per-neighbour
 // cursor: its first four elements are read together, and only a run longer than that is ended
 // by a binary search. The runs of 64 neighbours v are flattened over a wave (as in the triangle
 // kernel's row streaming) so that every lane counts one endpoint per step.
 constexpr int kC4HeavyLo = 256; // lower neighbours from which a vertex takes the window path
-constexpr int kC4Win = 32768;   // 128 KiB of u32 counters: one workgroup per CU
-constexpr int kC4Group = 8;     // windows per work item
+constexpr int kC4Words = 32768; // 128 KiB of LDS counters: one workgroup per CU
+constexpr int kC4Win = 65536;   // ids per window: 16-bit counters, two per word
+constexpr int kC4Group = 4;     // windows per work item
 constexpr int kC4Block = 1024;
 constexpr int kC4Waves = kC4Block / kWave;
 constexpr int kC4Touch = 2048;  // endpoints listed per window; a window with more is swept whole
@@ -448,7 +451,7 @@ __global__ void __launch_bounds__(kC4Block)
                      const int32_t *__restrict__ heavy, const int32_t *__restrict__ nheavy_p,
                      int32_t jmax, int64_t *__restrict__ cursors, int64_t cursor_stride,
                      int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
-  __shared__ uint32_t cnt[kC4Win];
+  __shared__ uint32_t cnt[kC4Words];
   __shared__ int32_t touch[kC4Touch];  // window offsets whose counter left zero
   __shared__ int32_t slots[kC4Waves][kWave];
   __shared__ int32_t s_item, s_ntouch;
@@ -458,7 +461,7 @@ __global__ void __launch_bounds__(kC4Block)
   const int64_t nitems = nheavy * jmax;
   int64_t *cur = cursors + (int64_t)blockIdx.x * cursor_stride;
   uint64_t acc = 0;
-  for (int32_t x = threadIdx.x; x < kC4Win; x += kC4Block) cnt[x] = 0u;  // kept zero between windows
+  for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
   if (threadIdx.x == 0) s_ntouch = 0;
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(item_cursor, 1);
@@ -481,8 +484,12 @@ __global__ void __launch_bounds__(kC4Block)
       cur[2 * i + 1] = pv + dv;
     }
     __syncthreads();
-    for (int32_t k = k0; k < k1; ++k) {
-      const int32_t wlo = k * kC4Win, whi = min(wlo + kC4Win, u);
+    const bool packed = nl < 65536;  // counts fit 16 bits
+    const int32_t span = packed ? kC4Win : kC4Words;
+    for (int32_t k = k0 * (kC4Win / span); k < k1 * (kC4Win / span); ++k) {
+      const int32_t wlo = k * span;
+      if (wlo >= u) break;
+      const int32_t whi = min(wlo + span, u);
       for (int32_t i0 = wib * kWave; i0 < nl; i0 += kC4Block) {
         const int32_t i = i0 + lane;
         int64_t p = 0;
@@ -523,7 +530,15 @@ __global__ void __launch_bounds__(kC4Block)
           for (int r = 0; r < kC4Steps; ++r) x[r] = idx[r] >= 0 ? col[idx[r]] - wlo : -1;
 #pragma unroll
           for (int r = 0; r < kC4Steps; ++r) {
-            if (x[r] >= 0 && atomicAdd(&cnt[x[r]], 1u) == 0u) {
+            if (x[r] < 0) continue;
+            bool first;
+            if (packed) {
+              const uint32_t sh = (x[r] & 1) * 16;
+              first = ((atomicAdd(&cnt[x[r] >> 1], 1u << sh) >> sh) & 0xffffu) == 0u;
+            } else {
+              first = atomicAdd(&cnt[x[r]], 1u) == 0u;
+            }
+            if (first) {
               const int32_t t = atomicAdd(&s_ntouch, 1);
               if (t < kC4Touch) touch[t] = x[r];
             }
@@ -534,15 +549,24 @@ __global__ void __launch_bounds__(kC4Block)
       const int32_t nt = s_ntouch;
       if (nt <= kC4Touch) {  // read and clear the listed counters only
         for (int32_t t = threadIdx.x; t < nt; t += kC4Block) {
-          const uint64_t c = cnt[touch[t]];
-          cnt[touch[t]] = 0u;
+          const int32_t x = touch[t];
+          uint64_t c;
+          if (packed) {  // the word's other half may be cleared by another thread meanwhile
+            const uint32_t sh = (x & 1) * 16;
+            c = (atomicAnd(&cnt[x >> 1], ~(0xffffu << sh)) >> sh) & 0xffffu;
+          } else {
+            c = cnt[x];
+            cnt[x] = 0u;
+          }
           acc += c * (c - 1) / 2;
         }
       } else {
-        for (int32_t x = threadIdx.x; x < whi - wlo; x += kC4Block) {
-          const uint64_t c = cnt[x];
+        const int32_t words = packed ? (whi - wlo + 1) / 2 : whi - wlo;
+        for (int32_t x = threadIdx.x; x < words; x += kC4Block) {
+          const uint32_t v = cnt[x];
           cnt[x] = 0u;
-          acc += c * (c - (c > 0)) / 2;
+          const uint64_t c0 = packed ? (v & 0xffffu) : v, c1 = packed ? (v >> 16) : 0u;
+          acc += c0 * (c0 - (c0 > 0)) / 2 + c1 * (c1 - (c1 > 0)) / 2;
         }
       }
       __syncthreads();
