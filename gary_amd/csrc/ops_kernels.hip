@@ -275,6 +275,18 @@ int grid_for(int64_t items, int cap = 4096) {
 }  // namespace
 }  // namespace gary
 
+namespace gary {
+namespace {
+// Zeroes n bytes: a kernel rather than a memset node, so callers may capture the operators in a
+// graph.
+__global__ void fill_bytes_kernel(uint8_t *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+}  // namespace
+}  // namespace gary
+
 extern "C" {
 
 int gary_search(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64_t nkeys,
@@ -285,7 +297,9 @@ int gary_search(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64
   GARY_CHECK_ARG(d_keys && d_results, "gary_search: null pointer");
   hipStream_t st = (hipStream_t)stream;
   if (len == 0) {
-    GARY_HIP(hipMemsetAsync(d_results, 0, (size_t)nkeys, st));
+    fill_bytes_kernel<<<(int)std::min<int64_t>((nkeys + 255) / 256, 4096), 256, 0, st>>>(
+        d_results, nkeys);
+    GARY_HIP(hipGetLastError());
     return GARY_OK;
   }
   GARY_CHECK_ARG(d_list, "gary_search: null list");
@@ -388,7 +402,8 @@ int gary_intersect_emit(const int32_t *d_a, int64_t na, const int32_t *d_b, int6
     GARY_TRY(check_search_args(0, 0, GARY_SEARCH_2PHASE, cache_depth));
   hipStream_t st = (hipStream_t)stream;
   if (na == 0 || nb == 0) {
-    GARY_HIP(hipMemsetAsync(d_count, 0, sizeof(uint64_t), st));
+    fill_bytes_kernel<<<1, 64, 0, st>>>((uint8_t *)d_count, (int64_t)sizeof(uint64_t));
+    GARY_HIP(hipGetLastError());
     return GARY_OK;
   }
   GARY_CHECK_ARG(d_a && d_b && d_out, "gary_intersect_emit: null pointer");

@@ -147,6 +147,21 @@ __global__ void __launch_bounds__(256)
 struct BinSums {
   float arcs, samples, ovl;
 };
+// Workgroups add their partial sums into one of kSumCopies copies of the 256 bins (by workgroup
+// index), so fewer of them contend for one address; readers add the copies.
+constexpr int kSumCopies = 8;
+
+__device__ __forceinline__ BinSums read_sums(const BinSums *__restrict__ sums, int b) {
+  BinSums t{0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < kSumCopies; ++c) {
+    const BinSums x = sums[c * kRouteBins + b];
+    t.arcs += x.arcs;
+    t.samples += x.samples;
+    t.ovl += x.ovl;
+  }
+  return t;
+}
 
 __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int32_t ovl_every,
                                     BinSums *__restrict__ sums) {
@@ -186,12 +201,21 @@ __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int
     if (na > nb) { const int32_t *t = A; A = B; B = t; int32_t tn = na; na = nb; nb = tn; }
     // up to 64 keys of the shorter list, evenly spaced, one per lane (a bounded chain of
     // dependent loads per sample)
+    // The search starts from 64 splitters of the longer list held one per lane
+    // (B[k*nb/64]): a search over lanes by shuffles, then a binary search of at most
+    // ceil(nb/64) elements in memory.
     const int32_t ns = min(na, kWave);
+    const int32_t key = lane < ns ? A[(int32_t)(((int64_t)lane * na) / ns)] : INT32_MAX;
+    const int32_t sp = B[(int32_t)(((int64_t)lane * nb) / kWave)];
+    int32_t ks = 0;  // last splitter <= key (0 also when key < B[0])
+#pragma unroll
+    for (int32_t step = kWave / 2; step > 0; step >>= 1)
+      if (__shfl(sp, ks + step, kWave) <= key) ks += step;
     uint32_t hit = 0;
     if (lane < ns) {
-      const int32_t i = (int32_t)(((int64_t)lane * na) / ns);
-      int32_t l = 0, r = nb - 1;
-      const int32_t key = A[i];
+      int32_t l = (int32_t)(((int64_t)ks * nb) / kWave);
+      int32_t r = (int32_t)(((int64_t)(ks + 1) * nb) / kWave) - 1;
+      if (ks == kWave - 1) r = nb - 1;
       while (l <= r) {
         const int32_t mid = (l + r) >> 1;
         const int32_t x = B[mid];
@@ -207,11 +231,12 @@ __global__ void bin_features_kernel(gary_device_graph g, int32_t hist_every, int
     }
   }
   __syncthreads();
+  BinSums *mine = sums + (blockIdx.x % kSumCopies) * kRouteBins;
   for (int i = threadIdx.x; i < kRouteBins; i += blockDim.x) {
-    if (s_arcs[i] > 0.f) atomicAdd(&sums[i].arcs, s_arcs[i]);
+    if (s_arcs[i] > 0.f) atomicAdd(&mine[i].arcs, s_arcs[i]);
     if (s_samp[i] > 0.f) {
-      atomicAdd(&sums[i].samples, s_samp[i]);
-      atomicAdd(&sums[i].ovl, s_ovl[i]);
+      atomicAdd(&mine[i].samples, s_samp[i]);
+      atomicAdd(&mine[i].ovl, s_ovl[i]);
     }
   }
 }
@@ -231,7 +256,7 @@ __global__ void bin_feature_rows_kernel(const BinSums *__restrict__ sums, float 
                                         float *__restrict__ X, float *__restrict__ arcs) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= kRouteBins) return;
-  BinSums s = sums[b];
+  BinSums s = read_sums(sums, b);
   bin_feature_vector(b, s, log_nnz, X + (int64_t)b * kIn);
   arcs[b] = s.arcs;
 }
@@ -383,7 +408,7 @@ __global__ void __launch_bounds__(kRouteBins)
   __syncthreads();
   const int b = threadIdx.x;
   float x[kIn], h1[kHid], h2[kHid], y[kOut], c[kOut];
-  const BinSums s = sums[b];
+  const BinSums s = read_sums(sums, b);
   bin_feature_vector(b, s, log_nnz, x);
   mlp_forward(sP, x, h1, h2, y);
   for (int o = 0; o < kOut; ++o) c[o] = exp2f(fminf(y[o], 30.f));
@@ -487,7 +512,7 @@ int state_for(int dev, DeviceState **out) {
     heuristic_table_host(h);
     GARY_HIP(hipMalloc(&s.heuristic, sizeof(RouteTable)));
     GARY_HIP(hipMalloc(&s.scratch, sizeof(RouteTable)));
-    GARY_HIP(hipMalloc(&s.sums, sizeof(BinSums) * kRouteBins));
+    GARY_HIP(hipMalloc(&s.sums, sizeof(BinSums) * kRouteBins * kSumCopies));
     GARY_HIP(hipMemcpy(s.heuristic, &h, sizeof h, hipMemcpyHostToDevice));
   }
   *out = &s;
@@ -525,7 +550,10 @@ __global__ void fill_zero_kernel_sel(uint32_t *__restrict__ p, int32_t n) {
 }
 
 int compute_bin_sums(const gary_device_graph &g, BinSums *sums, hipStream_t st) {
-  GARY_HIP(hipMemsetAsync(sums, 0, sizeof(BinSums) * kRouteBins, st));
+  // a fill kernel, not a memset node, so a caller may capture the learned strategy in a graph
+  fill_zero_kernel_sel<<<(kRouteBins * kSumCopies * 3 + 255) / 256, 256, 0, st>>>(
+      (uint32_t *)sums, kRouteBins * kSumCopies * 3);
+  GARY_HIP(hipGetLastError());
   if (g.csr.nnz == 0) return GARY_OK;
   // a bounded sample whatever the graph size: at most ~32K runs of 64 arcs for the histogram
   // and ~8K arcs for the overlap (every 8th run and every 2048th arc on smaller graphs)

@@ -541,11 +541,14 @@ constexpr int kBuckets = 16;
 constexpr int kKeys = 3 * kBuckets;
 // counter words: [0] #small, [1] #big, [2] always 0, [3] #hub, [4..5] main cursors,
 // [6..7] hub cursors, [8, 8+kKeys) histogram, [8+kKeys, 8+2*kKeys) scatter cursors
+// The histogram and the scatter cursors have kBinCopies copies: workgroup b of both kernels
+// (the same vertex range) uses copy b % kBinCopies, so fewer workgroups contend for one word.
+constexpr int kBinCopies = 8;
 constexpr int kHistOff = 8;
-constexpr int kCurOff = 8 + kKeys;
+constexpr int kCurOff = kHistOff + kBinCopies * kKeys;
 // [kSplitOff] heavy/light split of the small list; [+2] the same for the hub launch (whose
 // small list is empty: always 0)
-constexpr int kSplitOff = kCurOff + kKeys;
+constexpr int kSplitOff = kCurOff + kBinCopies * kKeys;
 static_assert(kSplitOff + 4 <= kPivotCounterWords, "pivot counter words");
 
 // ordered == false (per-graph counting of batched unions) puts every small pivot in the
@@ -591,7 +594,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kKeys; i += blockDim.x)
-    if (s_hist[i]) atomicAdd(&counters[kHistOff + i], s_hist[i]);
+    if (s_hist[i]) atomicAdd(&counters[kHistOff + (blockIdx.x % kBinCopies) * kKeys + i], s_hist[i]);
 }
 
 // Same vertex ranges as bin_hist_kernel; bucket starts come from the finished histogram.
@@ -601,14 +604,26 @@ __global__ void __launch_bounds__(256)
                        int32_t *__restrict__ counters, const RouteTable *__restrict__ table,
                        const float *__restrict__ pcost, float cost_big, bool ordered) {
   const uint32_t rowmask = table ? table->rowmask : 0xffffffffu;
-  __shared__ int32_t s_cnt[kKeys], s_base[kKeys], s_start[kKeys];
+  __shared__ int32_t s_cnt[kKeys], s_base[kKeys], s_start[kKeys], s_tot[kKeys], s_pre[kKeys];
   const int lane = lane_id();
+  const int copy = blockIdx.x % kBinCopies;
+  if (threadIdx.x < kKeys) {  // key totals, and the part counted by lower copies
+    int32_t tot = 0, pre = 0;
+    for (int c = 0; c < kBinCopies; ++c) {
+      const int32_t h = counters[kHistOff + c * kKeys + threadIdx.x];
+      pre += c < copy ? h : 0;
+      tot += h;
+    }
+    s_tot[threadIdx.x] = tot;
+    s_pre[threadIdx.x] = pre;
+  }
+  __syncthreads();
   if (threadIdx.x < 3) {  // bucket starts of list threadIdx.x (heaviest bucket first)
     const int b = threadIdx.x;
     int32_t run = 0;
     for (int k = 0; k < kBuckets; ++k) {
       s_start[b * kBuckets + k] = run;
-      run += counters[kHistOff + b * kBuckets + k];
+      run += s_tot[b * kBuckets + k];
     }
     if (blockIdx.x == 0) {
       counters[b == 2 ? 3 : b] = run;
@@ -628,7 +643,8 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kKeys; i += blockDim.x) {
-    s_base[i] = s_cnt[i] ? s_start[i] + atomicAdd(&counters[kCurOff + i], s_cnt[i]) : 0;
+    s_base[i] = s_cnt[i]
+        ? s_start[i] + s_pre[i] + atomicAdd(&counters[kCurOff + copy * kKeys + i], s_cnt[i]) : 0;
     s_cnt[i] = 0;
   }
   __syncthreads();

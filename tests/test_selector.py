@@ -107,6 +107,26 @@ def test_bin_features_track_exact_histogram():
     assert np.all(np.abs(arcs[big] - exact[big]) <= 0.25 * exact[big])
     assert np.all((X[:, 3] >= 0) & (X[:, 3] <= 1))
     assert np.all(arcs[exact == 0] == 0)  # a sample only lands in a bin that has arcs
+    # the overlap feature against its definition: every ovl_every-th arc (u, v), up to 64
+    # evenly spaced keys of the shorter of N+(u), N+(v), the fraction found in the longer
+    ovl_every = max(2048, -(-g.nnz // 8192))
+    row = np.repeat(np.arange(g.m), g.deg)
+    sums = np.zeros(256)
+    cnt = np.zeros(256)
+    for e in range(0, g.nnz, ovl_every):
+        u, v = int(row[e]), int(g.colIdx[e])
+        a, b = int(g.deg[u]), int(g.deg[v])
+        if a < 2 or b < 1:
+            continue
+        A, B = g.N(u), g.N(v)
+        if len(A) > len(B):
+            A, B = B, A
+        ns = min(len(A), 64)
+        keys = A[(np.arange(ns) * len(A)) // ns]
+        sums[selector.bin_of(a, b)] += np.isin(keys, B).sum() / ns
+        cnt[selector.bin_of(a, b)] += 1
+    want = np.where(cnt > 0, sums / np.maximum(cnt, 1), 0)
+    np.testing.assert_allclose(X[:, 3], want, atol=1e-5)
 
 
 @pytest.mark.gpu
