@@ -63,10 +63,13 @@ def build_lib(jobs: int = 8) -> Path:
 
 CPP_TEST_SRC = ROOT / "tests" / "cpp" / "tri_query_test.cpp"
 CPP_TEST_BIN = ROOT / "tests" / "cpp" / "tri_query_test"
+CAPTURE_TEST_SRC = ROOT / "tests" / "cpp" / "graph_capture_test.cpp"
+CAPTURE_TEST_BIN = ROOT / "tests" / "cpp" / "graph_capture_test"
 
 
 def build_cpp_test() -> Path:
-    """The façade test: plain g++ host code linked against libgary.so and the oracle."""
+    """The façade test: plain g++ host code linked against libgary.so and the oracle; and the
+    graph-capture test: g++ host code against libgary.so and the HIP runtime API."""
     cmd = ["g++", "-std=c++17", "-O2", f"-I{ROOT / 'include'}", str(CPP_TEST_SRC),
            f"-L{PKG}", "-lgary", f"-L{ROOT / 'oracle'}", "-loracle",
            "-Wl,-rpath,$ORIGIN/../../gary_amd", "-Wl,-rpath,$ORIGIN/../../oracle",
@@ -74,6 +77,13 @@ def build_cpp_test() -> Path:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"C++ façade test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    cmd = ["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}",
+           "-I/opt/rocm/include", str(CAPTURE_TEST_SRC), f"-L{PKG}", "-lgary",
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../gary_amd",
+           "-Wl,-rpath,/opt/rocm/lib", "-o", str(CAPTURE_TEST_BIN)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"graph-capture test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return CPP_TEST_BIN
 
 

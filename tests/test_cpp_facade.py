@@ -37,3 +37,16 @@ def test_facade_tri_query(scale):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL PASSED" in r.stdout
+
+
+CAPTURE = ROOT / "tests" / "cpp" / "graph_capture_test"
+
+
+@pytest.mark.gpu
+def test_graph_capture_replays():
+    """gary_tc_count (learned, heuristic, bitmap) captured in a HIP graph by a C++ caller:
+    every replay adds the eager count, with eager counts between replays."""
+    r = subprocess.run([str(CAPTURE), "14"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout
