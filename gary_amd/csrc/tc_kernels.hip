@@ -9,12 +9,13 @@
 //    (:118-135, Search.cuh:93-138) or a merge-path merge (the reference's intersectMerge is a
 //    stub, :172-180).
 //
-// 2. Pivot-centric ("bitmap"): a wave (or a workgroup for hubs) owns a pivot vertex u, builds an
-//    LDS membership structure of N+(u) — a bitmap over [min N+(u), max N+(u)] when that window
-//    fits, else an open-addressing hash — and streams every N+(v), v in N+(u), through it. The
-//    (v, w) pairs are flattened across the 64 lanes by a wave prefix scan over deg(v), so lanes
-//    stay busy however short the lists are and every N+(v) is read with consecutive addresses.
-//    Each N+(u) is read once per pivot rather than once per edge.
+// 2. Pivot-centric ("bitmap"): a wave (small pivots) or a workgroup (larger ones; an 8-wave
+//    kernel with a larger set for the largest) owns a pivot vertex u, builds an LDS membership
+//    set of N+(u) — a two-level rank bitmap, or an open-addressing hash when that does not fit —
+//    and streams every N+(v), v in N+(u), through it. The rows of 64 v are flattened into
+//    16-byte chunks read by consecutive lanes, so lanes stay busy however short the lists are
+//    and a wave's loads are contiguous. Each N+(u) is read once per pivot rather than once per
+//    arc. Pivots are binned on the device by out-degree, heaviest first.
 //
 // Counts are per-lane 32-bit partials, reduced per wave to 64 bits and added with one atomic per
 // wave.
@@ -425,7 +426,7 @@ constexpr int kSmallBatchHeavy = 4;
 constexpr int kSmallBatchLight = 16;
 constexpr int kLightLog = 4;
 static_assert(kLightLog >= 1 && kLightLog <= 15, "light-pivot threshold");
-constexpr int kUnroll = 2;
+constexpr int kStreamWindows = 2;  // 64-chunk windows loaded before any is probed
 
 template <int WAVES, int TABLE_WORDS>
 __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8)))
@@ -461,7 +462,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       PivotSet s;
       if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
         pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
-        c += stream_rows<kUnroll>(g, L, d, wib, WAVES, table, s, rb, lane,
+        c += stream_rows<kStreamWindows>(g, L, d, wib, WAVES, table, s, rb, lane,
                                   route ? route + g.csr.indPtr[u] : nullptr);
       } else {
         c += pivot_fallback(g, L, d, wib, WAVES, lane, route ? route + g.csr.indPtr[u] : nullptr);
@@ -509,7 +510,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
         continue;
       }
       pivot_build<false>(wt, s, L, d, lane, kWave);
-      c += stream_rows<kUnroll>(g, L, d, 0, 1, wt, s, rb, lane,
+      c += stream_rows<kStreamWindows>(g, L, d, 0, 1, wt, s, rb, lane,
                                 route ? route + g.csr.indPtr[u] : nullptr);
       wave_sync();  // the set is rebuilt for the next pivot
     }
