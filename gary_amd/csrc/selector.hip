@@ -14,8 +14,9 @@
 // The MLP (6 -> 32 -> 32 -> 3, ReLU) maps bin features to log2(ns per arc) for each strategy. It
 // is trained with the kernels below (forward, per-sample backward, ordered gradient reduction,
 // Adam) on per-bin timings that the library measures itself; inference is one fused kernel that
-// writes the route table and the predicted costs, which also drive the pivot split (workgroup vs
-// wave) by predicted cost instead of by degree.
+// picks the route subset with the least predicted total (leaving the all-bitmap routing only for
+// a gain above a margin) and writes the route table and the predicted costs, which can also
+// drive the pivot split (workgroup vs wave) by predicted cost instead of by degree.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
