@@ -33,7 +33,8 @@ namespace {
 constexpr int kHomWaves = 4;
 constexpr int kHomBlock = kHomWaves * 64;
 constexpr int kSetWords = 4096;    // LDS words for the position-returning set of N+(u)
-constexpr int kRowLdsWords = 4096; // LDS words for the local bitset graph
+constexpr int kRowLdsWords = 4095; // LDS words for the local bitset graph (the kernel's LDS is
+                                   // then exactly 40 KiB: four workgroups per CU)
 constexpr int kMaxHomDeg = 2048;   // largest out-degree handled (LDS column counters)
 
 __device__ __forceinline__ uint32_t hslot(int32_t w, int32_t log_t) {
@@ -268,14 +269,41 @@ __global__ void __launch_bounds__(kHomBlock)
       const uint32_t add = rc + (uint32_t)colcnt[i];
       if (add) atomicAdd(&tcount[ubase + i], add);
     }
-    // K4: for every local arc (i, j): popc(row_i & row_j)
+    // K4: for every local arc (i, j): popc(row_i & row_j). A wave takes row i; lane l holds its
+    // 32-bit word l (2W <= 64 words), and the row's set bits are enumerated 64 at a time: arc a
+    // belongs to the last word whose exclusive bit prefix is <= a (a search over lanes by
+    // shuffles), and is the (a - prefix)-th set bit of that word. Row j only has bits above j,
+    // so the AND starts at the word holding bit j + 1.
     for (int32_t i = wib; i < d; i += kHomWaves) {
       const uint32_t *ri = rows + (int64_t)i * W * 2;
-      for (int32_t j = lane; j < d; j += kWave) {
-        if (!((ri[(j >> 5)] >> (j & 31)) & 1u)) continue;
+      const uint32_t word = lane < 2 * W ? ri[lane] : 0u;
+      const int32_t nb = __popc(word);
+      const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nb);
+      const int32_t excl = incl - nb;
+      const int32_t arcs = __builtin_amdgcn_readlane(incl, kWave - 1);
+      for (int32_t a0 = 0; a0 < arcs; a0 += kWave) {
+        const int32_t a = a0 + lane;
+        int32_t o = 0;  // last word with excl <= a
+#pragma unroll
+        for (int32_t step = kWave / 2; step > 0; step >>= 1)
+          if (__shfl(excl, o + step, kWave) <= a) o += step;
+        uint32_t wv = (uint32_t)__shfl((int32_t)word, o, kWave);
+        int32_t k = a - __shfl(excl, o, kWave);
+        if (a >= arcs) continue;
+        int32_t pos = 0;  // position of the k-th set bit of wv
+#pragma unroll
+        for (int32_t width = 16; width > 0; width >>= 1) {
+          const int32_t c = __popc(wv & ((1u << width) - 1u));
+          if (k >= c) {
+            k -= c;
+            wv >>= width;
+            pos += width;
+          }
+        }
+        const int32_t j = o * 32 + pos;
         const uint32_t *rj = rows + (int64_t)j * W * 2;
         uint32_t c = 0;
-        for (int32_t w = 0; w < 2 * W; ++w) c += (uint32_t)__popc(ri[w] & rj[w]);
+        for (int32_t w = (j + 1) >> 5; w < 2 * W; ++w) c += (uint32_t)__popc(ri[w] & rj[w]);
         k4 += c;
       }
     }
