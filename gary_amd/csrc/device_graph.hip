@@ -72,6 +72,40 @@ __global__ void derive_rows_kernel(const int64_t *__restrict__ indPtr, int32_t m
 
 }  // namespace
 
+namespace gary {
+
+int ctx_device(gary_ctx *ctx, int *device) {
+  GARY_CHECK_ARG(ctx && device, "null ctx");
+  *device = ctx->device;
+  return GARY_OK;
+}
+
+// Takes ownership of device arrays built on the ctx's device (indPtr[m+1], colIdx[nnz]) and
+// derives deg, as gary_ctx_load_csr does after its copies.
+int ctx_adopt(gary_ctx *ctx, int32_t m, int64_t nnz, int64_t *indPtr, int32_t *colIdx) {
+  GARY_TRY(use_device(ctx->device));
+  release(ctx);
+  ctx->indPtr = indPtr;
+  ctx->colIdx = colIdx;
+  GARY_HIP(hipMalloc(&ctx->deg, sizeof(int32_t) * (size_t)(m > 0 ? m : 1)));
+  int32_t *bad = nullptr;
+  GARY_HIP(hipMalloc(&bad, sizeof(int32_t)));
+  GARY_HIP(hipMemset(bad, 0, sizeof(int32_t)));
+  if (m > 0) {
+    int grid = (int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096);
+    derive_deg_kernel<<<grid, 256>>>(ctx->indPtr, m, ctx->deg, bad);
+    GARY_HIP(hipGetLastError());
+  }
+  GARY_HIP(hipDeviceSynchronize());
+  GARY_HIP(hipFree(bad));
+  ctx->m = m;
+  ctx->nnz = nnz;
+  ctx->loaded = true;
+  return GARY_OK;
+}
+
+}  // namespace gary
+
 extern "C" {
 
 int gary_ctx_create(int device, gary_ctx **out) {

@@ -15,8 +15,14 @@
 #include <vector>
 
 #include "common.h"
+#include "prng.h"
 
 namespace {
+
+using gary::bounded32;
+using gary::mix64;
+using gary::next64;
+using gary::stream_state;
 
 // ---- parallel helpers -----------------------------------------------------------------------
 
@@ -43,22 +49,6 @@ void parallel_chunks(int64_t n, int workers, const std::function<void(int, int64
   }
   for (auto &t : ts) t.join();
 }
-
-// ---- PRNG: splitmix64 (Steele, Lea, Flood 2014) ------------------------------------------------
-
-inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-inline uint64_t next64(uint64_t &s) {
-  s += 0x9E3779B97F4A7C15ull;
-  return mix64(s);
-}
-// Independent stream for item i of a generator seeded with `seed`.
-inline uint64_t stream_state(uint64_t seed, uint64_t i) { return mix64(seed ^ mix64(i + 1)); }
-// Lemire's multiply-shift map of a 32-bit draw onto [0, n).
-inline uint32_t bounded32(uint32_t x, uint32_t n) { return (uint32_t)(((uint64_t)x * n) >> 32); }
 
 // ---- LSD radix sort of 64-bit keys (only the low `bits` bits are significant) ----------------
 
@@ -145,23 +135,11 @@ int gary_gen_rmat(int32_t scale, int64_t ntuples, double a, double b, double c, 
   GARY_CHECK_ARG(scale >= 1 && scale <= 30 && ntuples >= 0, "gary_gen_rmat: scale=%d", scale);
   GARY_CHECK_ARG(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0, "gary_gen_rmat: bad a,b,c");
   GARY_CHECK_ARG(ntuples == 0 || (src && dst), "gary_gen_rmat: null output");
-  // Quadrant thresholds on 32-bit draws: [0,tA) -> (0,0), [tA,tAB) -> (0,1),
-  // [tAB,tABC) -> (1,0), rest -> (1,1). No noise, no vertex permutation.
-  const uint64_t tA = (uint64_t)(a * 4294967296.0);
-  const uint64_t tAB = (uint64_t)((a + b) * 4294967296.0);
-  const uint64_t tABC = (uint64_t)((a + b + c) * 4294967296.0);
+  const gary::RmatThresholds t = gary::rmat_thresholds(a, b, c);
   parallel_chunks(ntuples, num_workers(), [&](int, int64_t beg, int64_t end) {
     for (int64_t i = beg; i < end; ++i) {
-      uint64_t s = stream_state(seed, (uint64_t)i);
-      uint32_t u = 0, v = 0;
-      uint64_t x = 0;
-      for (int lvl = 0; lvl < scale; ++lvl) {
-        if ((lvl & 1) == 0) x = next64(s);
-        uint64_t r = (lvl & 1) ? (x & 0xffffffffu) : (x >> 32);
-        uint32_t ub = r >= tAB, vb = (r >= tA && r < tAB) || r >= tABC;
-        u = (u << 1) | ub;
-        v = (v << 1) | vb;
-      }
+      uint32_t u, v;
+      gary::rmat_tuple(seed, (uint64_t)i, scale, t.tA, t.tAB, t.tABC, &u, &v);
       src[i] = (int32_t)u;
       dst[i] = (int32_t)v;
     }

@@ -23,6 +23,11 @@ _bind("gary_ctx_populate", C.c_int, [_P])
 _bind("gary_ctx_device_graph", C.c_int, [_P, C.POINTER(DeviceGraph)])
 _bind("gary_ctx_copy_to_host", C.c_int, [_P, _P, _P, _P])
 _bind("gary_ctx_clear", C.c_int, [_P])
+_bind("gary_ctx_build_from_edges", C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, C.c_int, _P])
+_bind("gary_gen_rmat_device", C.c_int, [C.c_int32, C.c_int64, C.c_double, C.c_double,
+                                        C.c_double, C.c_uint64, _P, _P, _P])
+_bind("gary_csr_from_edges_device", C.c_int, [C.c_int32, C.c_int64, _P, _P, C.c_int, _P, _P,
+                                              C.c_int64, _P, _P])
 
 
 class DeviceBuffer:
@@ -82,6 +87,16 @@ class DeviceGraphCtx:
                                     ptr(graph.colIdx)))
         self.hostCSR = graph
 
+    def buildFromEdges(self, n: int, src, dst, ntuples: int | None = None,
+                       mode: int = 0) -> None:
+        """Build the CSR on the device from tuple arrays (host numpy arrays are uploaded first;
+        DeviceBuffers are used in place): symmetrize, drop loops, dedup, orient (`mode` as in
+        graph.csr_from_edges). Replaces the reference's host GraphCOO -> GraphCSR build
+        (src/Graph.cpp:19-91) followed by loadFromHost."""
+        src_b, dst_b, nt = _tuple_buffers(src, dst, ntuples, self.device)
+        check(lib.gary_ctx_build_from_edges(self._h, n, nt, src_b.ptr, dst_b.ptr, mode, None))
+        self.hostCSR = None
+
     def populateDevice(self) -> None:
         """Derive the COO rowIdx on the device (src/DeviceGraph.cu:36-47)."""
         check(lib.gary_ctx_populate(self._h))
@@ -131,3 +146,37 @@ class DeviceGraphCtx:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def _tuple_buffers(src, dst, ntuples, device):
+    if isinstance(src, DeviceBuffer):
+        if ntuples is None:
+            raise ValueError("ntuples is required with device buffers")
+        return src, dst, int(ntuples)
+    src_b = DeviceBuffer.from_array(np.ascontiguousarray(src, np.int32), device)
+    dst_b = DeviceBuffer.from_array(np.ascontiguousarray(dst, np.int32), device)
+    return src_b, dst_b, int(np.asarray(src).shape[0])
+
+
+def gen_rmat_device(scale: int, ntuples: int, a=0.57, b=0.19, c=0.19, seed: int = 1,
+                    device: int = 0) -> tuple[DeviceBuffer, DeviceBuffer]:
+    """R-MAT tuples generated on the device, equal to graph.gen_rmat's bit for bit."""
+    src = DeviceBuffer(max(4 * ntuples, 4), device)
+    dst = DeviceBuffer(max(4 * ntuples, 4), device)
+    check(lib.gary_gen_rmat_device(scale, ntuples, a, b, c, seed & (2**64 - 1), src.ptr,
+                                   dst.ptr, None))
+    check(lib.gary_device_synchronize(device))
+    return src, dst
+
+
+def csr_from_edges_device(n: int, src, dst, mode: int = 0, ntuples: int | None = None,
+                          device: int = 0) -> GraphCSR:
+    """graph.csr_from_edges computed on the device (gary_csr_from_edges_device), copied back."""
+    src_b, dst_b, nt = _tuple_buffers(src, dst, ntuples, device)
+    cap = 2 * nt if mode in (1, 3) else nt
+    ip = DeviceBuffer(8 * (n + 1), device)
+    ci = DeviceBuffer(max(4 * cap, 4), device)
+    nnz = C.c_int64(0)
+    check(lib.gary_csr_from_edges_device(n, nt, src_b.ptr, dst_b.ptr, mode, ip.ptr, ci.ptr, cap,
+                                         C.byref(nnz), None))
+    return GraphCSR(ip.to_array(np.int64, n + 1), ci.to_array(np.int32, nnz.value))

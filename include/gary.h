@@ -102,6 +102,22 @@ int gary_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const in
                         int mode, int64_t *indPtr, int32_t *colIdx, int64_t cap,
                         int64_t *nnz_out);
 
+/* ---- device graph construction (the same builds on the GPU; the reference has host code only:
+ *      src/Graph.cpp:19-91, common/ParRadixSort.h:218-306, include/Graph.h:242-248) -------- */
+
+/* gary_gen_rmat on the device: the same tuples bit for bit, written to device arrays src/dst
+ * (ntuples each) on `stream`. Asynchronous. */
+int gary_gen_rmat_device(int32_t scale, int64_t ntuples, double a, double b, double c,
+                         uint64_t seed, int32_t *src, int32_t *dst, void *stream);
+
+/* gary_csr_from_edges with device arrays: src/dst (ntuples each) in, indPtr[n+1] and colIdx
+ * (capacity cap) out, all device pointers; identical output for every mode. Keys are limited to
+ * 2^31 - 1 (ntuples, or 2*ntuples for the symmetric modes). Returns when the CSR is complete
+ * (it reads the deduplicated edge count back to the host for *nnz_out). */
+int gary_csr_from_edges_device(int32_t n, int64_t ntuples, const int32_t *src,
+                               const int32_t *dst, int mode, int64_t *indPtr, int32_t *colIdx,
+                               int64_t cap, int64_t *nnz_out, void *stream);
+
 /* ---- device graph context (DeviceGraphCtx) ------------------------------------------------- */
 typedef struct gary_ctx gary_ctx;
 
@@ -110,6 +126,11 @@ int gary_ctx_destroy(gary_ctx *ctx);
 /* loadFromHost(shared_ptr<GraphCSR>): H2D of indPtr and colIdx; deg is derived on the device. */
 int gary_ctx_load_csr(gary_ctx *ctx, int32_t m, int64_t nnz, const int64_t *indPtr,
                       const int32_t *colIdx);
+/* Builds the ctx's CSR on its device from device tuple arrays (gary_csr_from_edges_device), so
+ * nothing crosses PCIe; replaces loadFromHost(GraphCOO -> GraphCSR) of the reference's host
+ * path (src/Graph.cpp:19-91, src/DeviceGraph.cu:18-28). */
+int gary_ctx_build_from_edges(gary_ctx *ctx, int32_t n, int64_t ntuples, const int32_t *src,
+                              const int32_t *dst, int mode, void *stream);
 /* populateDevice(): derive the COO rowIdx on the device. */
 int gary_ctx_populate(gary_ctx *ctx);
 /* getDeviceCSR() + getDeviceCOO() in one view; valid while the ctx lives and is not reloaded. */
