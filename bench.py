@@ -16,6 +16,9 @@ Further legs (--legs, default all):
          per-graph uint64 counts;
   hom4   K4 / diamond / 4-cycle homomorphism counts on R-MAT scale 22 (wall clock per call:
          each call returns its counts to the host).
+  build  graph construction on the device (R-MAT tuples, symmetrize, dedup, orient, CSR) for
+         R-MAT 20 degree-oriented and R-MAT 22 ranked symmetric, against the host build of the
+         same tuples (equal arrays required).
 Launch knobs come from the autotuner's file (gary_amd/tuned.json) where it has an entry for the
 workload.
 
@@ -253,6 +256,52 @@ class Bench:
         leg["edges_per_s"] = (g.nnz // 2) / (leg["all_ms"] * 1e-3)
         return leg
 
+    def build(self) -> dict:
+        """Device graph construction against the host build of the same tuples (wall clock,
+        median of 5 after a warm-up; the device build returns after reading its edge count)."""
+        from gary_amd.device import gen_rmat_device
+        np, graph, gq = self.np, self.graph, self.gq
+
+        def med(f, k=5):
+            f()
+            ts = []
+            for _ in range(k):
+                t0 = time.perf_counter()
+                f()
+                gq.synchronize()
+                ts.append(time.perf_counter() - t0)
+            return sorted(ts)[k // 2] * 1e3
+
+        leg = {}
+        for name, scale, seed, mode in (("rmat20_degree", 20, graph.SEEDS["rmat20"],
+                                         graph.ORIENT_DEGREE),
+                                        ("rmat22_ranked", 22, graph.SEEDS["rmat22"],
+                                         graph.SYMMETRIC_RANKED)):
+            n, nt = 1 << scale, 16 << scale
+            host = {}
+
+            def host_build():
+                s, d = graph.gen_rmat(scale, nt, seed=seed)
+                host["g"] = graph.csr_from_edges(n, s, d, mode)
+
+            host_ms = med(host_build, 3)
+            gen_ms = med(lambda: gen_rmat_device(scale, nt, seed=seed))
+            ds, dd = gen_rmat_device(scale, nt, seed=seed)
+            ctx = self.DeviceGraphCtx()
+            build_ms = med(lambda: ctx.buildFromEdges(n, ds, dd, nt, mode))
+            g = ctx.getHostCSR()
+            if not (np.array_equal(g.indPtr, host["g"].indPtr)
+                    and np.array_equal(g.colIdx, host["g"].colIdx)):
+                raise SystemExit(f"device build of {name} differs from the host build")
+            leg[name] = {"tuples": nt, "stored_edges": g.nnz, "device_gen_ms": gen_ms,
+                         "device_build_ms": build_ms, "host_gen_build_ms": host_ms,
+                         "speedup": host_ms / (gen_ms + build_ms)}
+            log(f"build {name}: device gen {gen_ms:.2f} ms + build {build_ms:.2f} ms, "
+                f"host {host_ms:.1f} ms")
+            ctx.close()
+            del ds, dd, host
+        return leg
+
     def cpu(self) -> dict:
         import oracle
         a, g = self.a, self.g
@@ -290,7 +339,7 @@ def main(argv=None) -> int:
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--strategy", default="learned")
-    ap.add_argument("--legs", default="headline,c3,batch,hom4")
+    ap.add_argument("--legs", default="headline,c3,batch,hom4,build")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--only", action="store_true",
@@ -328,6 +377,8 @@ def main(argv=None) -> int:
         out["batched"] = b.batch()
     if "hom4" in legs:
         out["hom4"] = b.hom4()
+    if "build" in legs:
+        out["build"] = b.build()
     if a.calibrate_copy:
         b.calibrate_copy(a.calibrate_copy)
     out["cpu_baseline"] = b.cpu() if "headline" in legs and not a.no_cpu else None

@@ -8,14 +8,17 @@
 // on the host (same modes, same sorted lists, same indPtr), in a few milliseconds instead of
 // seconds.
 //
-// Pipeline (every step a pass over HBM; no host round trip except reading the edge count):
-//   1. canonical keys lo<<vb | hi of non-loop tuples (loops become a sentinel that sorts last);
-//   2. stable LSD radix sort of the keys on their 2*vb significant bits;
-//   3. ordered compaction of the distinct keys (dedup);
-//   4. degrees: run lengths of lo within a wave plus atomics for hi;
-//   5. arc keys src<<vb | dst for the requested mode (one per edge oriented by (degree, id) or by
-//      id, two per edge symmetric; ranked ids come from a sort of (degree, id) keys);
-//   6. radix sort of the arc keys; colIdx = low bits, indPtr[v] = first arc with src >= v.
+// Pipeline (every step a pass over HBM; the host reads back only edge counts):
+//   1. both arcs u<<vb|v and v<<vb|u of every non-loop tuple (loops become a sentinel that sorts
+//      last);
+//   2. stable LSD radix sort of the arcs on their 2*vb significant bits;
+//   3. ordered compaction of the first arc of each run (dedup): the symmetric arc list in
+//      (src, dst) order, whose row lengths are the degrees;
+//   4. the mode's arcs: all of them (symmetric), an order-preserving compaction of one arc per
+//      edge (oriented by (degree, id) or by id; it stays sorted), or relabeled by the rank of
+//      (degree, id) and sorted again (ranked);
+//   5. colIdx = dst of each arc; indPtr = exclusive scan of the row lengths, which come from the
+//      first and last arc of every row.
 //
 // Radix sort (reduce-then-scan, 8-bit digits): per tile of 4096 keys, every wave ranks its keys
 // by digit with ballots (the lanes holding a digit are found with 8 ballots, the lowest of them
@@ -24,11 +27,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <vector>
 
 #include "common.h"
 #include "device_util.h"
 #include "prng.h"
+#include "workspace.h"
 
 namespace {
 
@@ -241,10 +244,10 @@ __global__ void __launch_bounds__(kBlock) scan_down_kernel(uint32_t *__restrict_
   block_scan_tile(a + base, std::min<uint32_t>(kTile, len - base), part[blockIdx.x], lds);
 }
 
-// ---- keys, dedup, degrees, arcs, CSR -------------------------------------------------------------
+// ---- keys, dedup, orientation, CSR --------------------------------------------------------------
 
-// key = lo << vb | hi of each tuple; loops -> sentinel (all 2*vb bits set, never a valid edge);
-// ids outside [0, n) raise *bad.
+// Both arcs u<<vb|v and v<<vb|u of each tuple at 2i, 2i+1; loops -> sentinel (all 2*vb bits set,
+// never a valid arc); ids outside [0, n) raise *bad.
 __global__ void __launch_bounds__(kBlock) canon_kernel(const int32_t *__restrict__ src,
                                                        const int32_t *__restrict__ dst,
                                                        uint64_t nt, int32_t n, int vb,
@@ -254,26 +257,47 @@ __global__ void __launch_bounds__(kBlock) canon_kernel(const int32_t *__restrict
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < nt;
        i += (uint64_t)gridDim.x * kBlock) {
     const int32_t u = src[i], v = dst[i];
-    uint64_t k = sentinel;
+    uint64_t a = sentinel, b = sentinel;
     if (u < 0 || v < 0 || u >= n || v >= n) {
       *bad = 1;
     } else if (u != v) {
-      const uint32_t lo = (uint32_t)min(u, v), hi = (uint32_t)max(u, v);
-      k = ((uint64_t)lo << vb) | hi;
+      a = ((uint64_t)(uint32_t)u << vb) | (uint32_t)v;
+      b = ((uint64_t)(uint32_t)v << vb) | (uint32_t)u;
     }
-    keys[i] = k;
+    keys[2 * i] = a;
+    keys[2 * i + 1] = b;
   }
 }
 
-// Keys of a sorted array that start a run of equal keys and are not the sentinel.
-__device__ __forceinline__ bool is_first(const uint64_t *keys, uint32_t i, uint64_t k,
-                                         uint64_t sentinel) {
-  return k != sentinel && (i == 0 || keys[i - 1] != k);
-}
+// Keep rules of the ordered compactions. Each sees the sorted array, the index and the key.
+struct KeepFirst {  // first key of each run of equal keys, sentinel dropped (dedup)
+  uint64_t sentinel;
+  __device__ bool operator()(const uint64_t *keys, uint32_t i, uint64_t k) const {
+    return k != sentinel && (i == 0 || keys[i - 1] != k);
+  }
+};
+struct KeepDegreeOrder {  // arc s -> d with (deg s, s) < (deg d, d)
+  int vb;
+  uint64_t mask;
+  const uint32_t *deg;
+  __device__ bool operator()(const uint64_t *, uint32_t, uint64_t k) const {
+    const uint32_t s = (uint32_t)(k >> vb), d = (uint32_t)(k & mask);
+    const uint32_t ds = deg[s], dd = deg[d];
+    return ds < dd || (ds == dd && s < d);
+  }
+};
+struct KeepIdOrder {  // arc s -> d with s > d (the reference's GraphCOO::orientation)
+  int vb;
+  uint64_t mask;
+  __device__ bool operator()(const uint64_t *, uint32_t, uint64_t k) const {
+    return (uint32_t)(k >> vb) > (uint32_t)(k & mask);
+  }
+};
 
-// cnt[t] = distinct non-sentinel keys of tile t.
-__global__ void __launch_bounds__(kBlock) uniq_count_kernel(const uint64_t *__restrict__ keys,
-                                                            uint32_t n, uint64_t sentinel,
+// cnt[t] = kept keys of tile t.
+template <class Keep>
+__global__ void __launch_bounds__(kBlock) keep_count_kernel(const uint64_t *__restrict__ keys,
+                                                            uint32_t n, Keep keep,
                                                             uint32_t *__restrict__ cnt) {
   __shared__ uint32_t ws[kWaves];
   const int w = threadIdx.x / kWave, lane = lane_id();
@@ -282,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) uniq_count_kernel(const uint64_t *__re
 #pragma unroll 4
   for (int k = 0; k < kIpt; ++k) {
     const uint32_t i = base + (uint32_t)k * kWave + (uint32_t)lane;
-    if (i < n) c += is_first(keys, i, keys[i], sentinel);
+    if (i < n) c += keep(keys, i, keys[i]);
   }
   c = gary::wave_sum_u32(c);
   if (lane == 0) ws[w] = c;
@@ -294,27 +318,28 @@ __global__ void __launch_bounds__(kBlock) uniq_count_kernel(const uint64_t *__re
   }
 }
 
-// Ordered compaction of the distinct keys: tile t writes from off[t].
-__global__ void __launch_bounds__(kBlock) uniq_write_kernel(const uint64_t *__restrict__ keys,
-                                                            uint32_t n, uint64_t sentinel,
+// Ordered compaction of the kept keys: tile t writes from off[t].
+template <class Keep>
+__global__ void __launch_bounds__(kBlock) keep_write_kernel(const uint64_t *__restrict__ keys,
+                                                            uint32_t n, Keep keep,
                                                             const uint32_t *__restrict__ off,
                                                             uint64_t *__restrict__ out) {
   __shared__ uint32_t ws[kWaves];
   const int w = threadIdx.x / kWave, lane = lane_id();
   const uint32_t base = blockIdx.x * (uint32_t)kTile + (uint32_t)w * kWaveSpan;
   uint64_t key[kIpt];
-  uint64_t keep[kIpt];
+  uint64_t kept[kIpt];
   uint32_t c = 0;
 #pragma unroll
   for (int k = 0; k < kIpt; ++k) {
     const uint32_t i = base + (uint32_t)k * kWave + (uint32_t)lane;
-    key[k] = i < n ? keys[i] : sentinel;
+    key[k] = i < n ? keys[i] : 0ull;
   }
 #pragma unroll
   for (int k = 0; k < kIpt; ++k) {
     const uint32_t i = base + (uint32_t)k * kWave + (uint32_t)lane;
-    keep[k] = __ballot(i < n && is_first(keys, i, key[k], sentinel));
-    c += (uint32_t)__popcll(keep[k]);
+    kept[k] = __ballot(i < n && keep(keys, i, key[k]));
+    c += (uint32_t)__popcll(kept[k]);
   }
   if (lane == 0) ws[w] = c;
   __syncthreads();
@@ -322,74 +347,49 @@ __global__ void __launch_bounds__(kBlock) uniq_write_kernel(const uint64_t *__re
   for (int q = 0; q < w; ++q) pos += ws[q];
 #pragma unroll
   for (int k = 0; k < kIpt; ++k) {
-    if ((keep[k] >> lane) & 1ull) out[pos + (uint32_t)lanes_below(keep[k])] = key[k];
-    pos += (uint32_t)__popcll(keep[k]);
+    if ((kept[k] >> lane) & 1ull) out[pos + (uint32_t)lanes_below(kept[k])] = key[k];
+    pos += (uint32_t)__popcll(kept[k]);
   }
 }
 
-// deg[v] += number of distinct edges at v. Keys are sorted by lo, so the lanes of a wave that
-// share lo are consecutive: the first lane of each run adds the run length.
-__global__ void __launch_bounds__(kBlock) degree_kernel(const uint64_t *__restrict__ keys,
-                                                        uint32_t m, int vb,
-                                                        int32_t *__restrict__ deg) {
+// Rows of arcs sorted by (src, dst): start[s] = first arc of row s, end[s] = one past its last
+// (both zero for rows without arcs); colIdx[i] = dst of arc i when colIdx is given.
+__global__ void __launch_bounds__(kBlock) row_bounds_kernel(const uint64_t *__restrict__ arcs,
+                                                            uint32_t narcs, int vb,
+                                                            uint32_t *__restrict__ start,
+                                                            uint32_t *__restrict__ end,
+                                                            int32_t *__restrict__ colIdx) {
   const uint64_t mask = (vb == 0) ? 0 : ((1ull << vb) - 1);
-  const uint32_t stride = gridDim.x * (uint32_t)kBlock;
-  for (uint32_t b = blockIdx.x * (uint32_t)kBlock + (threadIdx.x & ~(kWave - 1)); b < m;
-       b += stride) {
-    const uint32_t i = b + (uint32_t)lane_id();
-    const bool valid = i < m;
-    const uint64_t k = valid ? keys[i] : 0ull;
-    const int32_t lo = valid ? (int32_t)(k >> vb) : -1;
-    const int32_t prev = __shfl_up(lo, 1, kWave);
-    const uint64_t heads = __ballot(valid && (lane_id() == 0 || prev != lo));
-    const int nvalid = (int)__popcll(__ballot(valid));  // valid lanes are a prefix of the wave
-    if (valid) {
-      atomicAdd(&deg[(uint32_t)(k & mask)], 1);
-      if ((heads >> lane_id()) & 1ull) {
-        const uint64_t later = heads & ~((2ull << lane_id()) - 1);
-        const int next = later ? (__ffsll((unsigned long long)later) - 1) : nvalid;
-        atomicAdd(&deg[lo], next - lane_id());
-      }
-    }
-  }
-}
-
-enum ArcMode { kArcDegree = 0, kArcSym = 1, kArcId = 2, kArcRanked = 3 };
-
-// Arc keys src << vb | dst of the distinct edges for the mode (two per edge when symmetric).
-__global__ void __launch_bounds__(kBlock) arcs_kernel(const uint64_t *__restrict__ ukeys,
-                                                      uint32_t m, int vb, int mode,
-                                                      const int32_t *__restrict__ deg,
-                                                      const int32_t *__restrict__ newid,
-                                                      uint64_t *__restrict__ arcs) {
-  const uint64_t mask = (vb == 0) ? 0 : ((1ull << vb) - 1);
-  for (uint32_t i = blockIdx.x * (uint32_t)kBlock + threadIdx.x; i < m;
+  for (uint32_t i = blockIdx.x * (uint32_t)kBlock + threadIdx.x; i < narcs;
        i += gridDim.x * (uint32_t)kBlock) {
-    const uint64_t k = ukeys[i];
-    uint32_t lo = (uint32_t)(k >> vb), hi = (uint32_t)(k & mask);
-    if (mode == kArcDegree) {
-      const int32_t dl = deg[lo], dh = deg[hi];
-      const bool lo_first = dl < dh || (dl == dh && lo < hi);
-      arcs[i] = lo_first ? ((uint64_t)lo << vb | hi) : ((uint64_t)hi << vb | lo);
-    } else if (mode == kArcId) {  // src > dst
-      arcs[i] = (uint64_t)hi << vb | lo;
-    } else {
-      if (mode == kArcRanked) {
-        lo = (uint32_t)newid[lo];
-        hi = (uint32_t)newid[hi];
-      }
-      arcs[2 * (size_t)i] = (uint64_t)lo << vb | hi;
-      arcs[2 * (size_t)i + 1] = (uint64_t)hi << vb | lo;
-    }
+    const uint64_t k = arcs[i];
+    const uint32_t s = (uint32_t)(k >> vb);
+    if (i == 0 || (uint32_t)(arcs[i - 1] >> vb) != s) start[s] = i;
+    if (i + 1 == narcs || (uint32_t)(arcs[i + 1] >> vb) != s) end[s] = i + 1;
+    if (colIdx) colIdx[i] = (int32_t)(k & mask);
   }
+}
+
+// len[v] = end[v] - start[v] in place of end, for v < n, and len[n] = 0 (the scan's total slot).
+__global__ void __launch_bounds__(kBlock) row_len_kernel(const uint32_t *__restrict__ start,
+                                                         uint32_t *__restrict__ end, int32_t n) {
+  for (int32_t v = blockIdx.x * kBlock + threadIdx.x; v <= n; v += gridDim.x * kBlock)
+    end[v] = v < n ? end[v] - start[v] : 0u;
+}
+
+// indPtr[v] = the exclusive row-length prefix, widened to 64 bits.
+__global__ void __launch_bounds__(kBlock) widen_kernel(const uint32_t *__restrict__ pre,
+                                                       int32_t n, int64_t *__restrict__ indPtr) {
+  for (int32_t v = blockIdx.x * kBlock + threadIdx.x; v <= n; v += gridDim.x * kBlock)
+    indPtr[v] = (int64_t)pre[v];
 }
 
 // Ranking keys (deg v) << vb | v of every vertex.
-__global__ void __launch_bounds__(kBlock) rank_keys_kernel(const int32_t *__restrict__ deg,
+__global__ void __launch_bounds__(kBlock) rank_keys_kernel(const uint32_t *__restrict__ deg,
                                                            int32_t n, int vb,
                                                            uint64_t *__restrict__ keys) {
   for (int32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock)
-    keys[v] = (uint64_t)(uint32_t)deg[v] << vb | (uint32_t)v;
+    keys[v] = (uint64_t)deg[v] << vb | (uint32_t)v;
 }
 
 // newid[v] = position of (deg v, v) in ascending order.
@@ -401,18 +401,17 @@ __global__ void __launch_bounds__(kBlock) rank_ids_kernel(const uint64_t *__rest
     newid[(uint32_t)(sorted[i] & mask)] = i;
 }
 
-// colIdx[i] = dst of arc i; indPtr[v] = first arc with src >= v (indPtr[n] = narcs).
-__global__ void __launch_bounds__(kBlock) csr_kernel(const uint64_t *__restrict__ arcs,
-                                                     uint32_t narcs, int32_t n, int vb,
-                                                     int64_t *__restrict__ indPtr,
-                                                     int32_t *__restrict__ colIdx) {
+// Arc s -> d becomes newid[s] -> newid[d].
+__global__ void __launch_bounds__(kBlock) relabel_kernel(const uint64_t *__restrict__ arcs,
+                                                         uint32_t narcs, int vb,
+                                                         const int32_t *__restrict__ newid,
+                                                         uint64_t *__restrict__ out) {
   const uint64_t mask = (vb == 0) ? 0 : ((1ull << vb) - 1);
-  for (uint32_t i = blockIdx.x * (uint32_t)kBlock + threadIdx.x; i <= narcs;
+  for (uint32_t i = blockIdx.x * (uint32_t)kBlock + threadIdx.x; i < narcs;
        i += gridDim.x * (uint32_t)kBlock) {
-    const int32_t lo_v = i == 0 ? 0 : (int32_t)(arcs[i - 1] >> vb) + 1;
-    const int32_t hi_v = i == narcs ? n : (int32_t)(arcs[i] >> vb);
-    for (int32_t v = lo_v; v <= hi_v; ++v) indPtr[v] = (int64_t)i;
-    if (i < narcs) colIdx[i] = (int32_t)(arcs[i] & mask);
+    const uint64_t k = arcs[i];
+    out[i] = (uint64_t)(uint32_t)newid[(uint32_t)(k >> vb)] << vb |
+             (uint32_t)newid[(uint32_t)(k & mask)];
   }
 }
 
@@ -428,19 +427,14 @@ int bits_for(int64_t n) {
   return b;
 }
 
-// Device buffers released at scope exit.
-struct DevBufs {
-  std::vector<void *> ptrs;
-  ~DevBufs() {
-    for (void *p : ptrs) (void)hipFree(p);
-  }
+// Byte offsets of 256-byte aligned sub-buffers of one scratch allocation.
+struct Carve {
+  size_t bytes = 0;
   template <class T>
-  int alloc(T **p, size_t count) {
-    void *q = nullptr;
-    GARY_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
-    ptrs.push_back(q);
-    *p = (T *)q;
-    return GARY_OK;
+  size_t take(size_t count) {
+    const size_t at = bytes;
+    bytes += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    return at;
   }
 };
 
@@ -483,113 +477,151 @@ int radix_sort(uint64_t *keys, uint64_t *tmp, uint32_t n, int bits, uint32_t *bh
 size_t sort_bh_count(uint64_t n) { return (size_t)kBins * tiles_of(n); }
 size_t sort_part_count(uint64_t n) { return tiles_of(sort_bh_count(n)) + 1; }
 
+// Ordered compaction of keys[0..n) by `keep` into out; *kept receives the count (read back to the
+// host, so this synchronizes the stream). cnt holds tiles_of(n) + 1 counters.
+template <class Keep>
+int compact(const uint64_t *keys, uint32_t n, Keep keep, uint64_t *out, uint32_t *cnt,
+            uint32_t *part, hipStream_t s, uint32_t *kept) {
+  *kept = 0;
+  if (n == 0) return GARY_OK;
+  const uint32_t tiles = tiles_of(n);
+  keep_count_kernel<Keep><<<tiles, kBlock, 0, s>>>(keys, n, keep, cnt);
+  GARY_HIP(hipGetLastError());
+  GARY_HIP(hipMemsetAsync(cnt + tiles, 0, sizeof(uint32_t), s));
+  GARY_TRY(scan_u32(cnt, tiles + 1, part, s));
+  keep_write_kernel<Keep><<<tiles, kBlock, 0, s>>>(keys, n, keep, cnt, out);
+  GARY_HIP(hipGetLastError());
+  GARY_HIP(hipMemcpyAsync(kept, cnt + tiles, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  GARY_HIP(hipStreamSynchronize(s));
+  return GARY_OK;
+}
+
+// Row lengths of arcs sorted by (src, dst) into len[0..n) (len[n] = 0); start and len hold n + 1
+// counters; colIdx (optional) receives the dst of every arc.
+int row_lengths(const uint64_t *arcs, uint32_t narcs, int32_t n, int vb, uint32_t *start,
+                uint32_t *len, int32_t *colIdx, hipStream_t s) {
+  GARY_HIP(hipMemsetAsync(start, 0, sizeof(uint32_t) * ((size_t)n + 1), s));
+  GARY_HIP(hipMemsetAsync(len, 0, sizeof(uint32_t) * ((size_t)n + 1), s));
+  if (narcs > 0) {
+    row_bounds_kernel<<<grid_for(narcs), kBlock, 0, s>>>(arcs, narcs, vb, start, len, colIdx);
+    GARY_HIP(hipGetLastError());
+  }
+  row_len_kernel<<<grid_for((uint64_t)n + 1), kBlock, 0, s>>>(start, len, n);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
+// The CSR of arcs sorted by (src, dst): colIdx = dst, indPtr = exclusive prefix of row lengths.
+int csr_from_sorted(const uint64_t *arcs, uint32_t narcs, int32_t n, int vb, uint32_t *start,
+                    uint32_t *len, uint32_t *part, int64_t *indPtr, int32_t *colIdx,
+                    hipStream_t s) {
+  GARY_TRY(row_lengths(arcs, narcs, n, vb, start, len, colIdx, s));
+  GARY_TRY(scan_u32(len, (uint32_t)n + 1, part, s));
+  widen_kernel<<<grid_for((uint64_t)n + 1), kBlock, 0, s>>>(len, n, indPtr);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
 // The build: the CSR of `mode` into indPtr[n+1] / colIdx (capacity cap), stored count in
-// *nnz_out. When colIdx is null, *colIdx_alloc receives a hipMalloc'ed array of exactly the
-// stored count (at least one element), owned by the caller.
+// *nnz_out. When colIdx_alloc is given, it receives a hipMalloc'ed colIdx of exactly the stored
+// count (at least one element), owned by the caller, and colIdx / cap are ignored.
+//
+// Both arcs of every tuple are sorted together, so the deduplicated list is the symmetric CSR in
+// (src, dst) order and the degrees are its row lengths. The oriented modes keep one arc per edge
+// by an order-preserving compaction, which stays sorted; only the ranked mode sorts again, after
+// relabeling.
 int build_csr(int32_t n, int64_t nt, const int32_t *src, const int32_t *dst, int mode,
               int64_t *indPtr, int32_t *colIdx, int64_t cap, int64_t *nnz_out,
               int32_t **colIdx_alloc, hipStream_t s) {
-  const bool sym = mode == GARY_CSR_SYMMETRIC || mode == GARY_CSR_SYMMETRIC_RANKED;
-  GARY_CHECK_ARG((sym ? 2 * nt : nt) < ((int64_t)1 << 31),
-                 "device CSR build: %lld tuples (limit 2^31 - 1 keys)", (long long)nt);
+  GARY_CHECK_ARG(2 * nt < ((int64_t)1 << 31),
+                 "device CSR build: %lld tuples (limit 2^30 - 1)", (long long)nt);
   const int vb = bits_for(n);
-  GARY_CHECK_ARG(2 * vb <= 62, "device CSR build: n=%d too large", n);
+  const uint64_t mask = (vb == 0) ? 0 : ((1ull << vb) - 1);
   const uint64_t sentinel = (vb == 0) ? 0 : ((1ull << (2 * vb)) - 1);
-  DevBufs bufs;
-  // 1-3: canonical keys, sort, dedup
-  int64_t m = 0;
-  uint64_t *ukeys = nullptr;
-  const uint64_t keycap = std::max<uint64_t>((uint64_t)nt, sym ? 2 * (uint64_t)nt : 0);
-  uint64_t *ka = nullptr, *kb = nullptr;
-  uint32_t *bh = nullptr, *part = nullptr;
-  int32_t *flag = nullptr;
-  GARY_TRY(bufs.alloc(&ka, std::max<uint64_t>(keycap, (uint64_t)n)));
-  GARY_TRY(bufs.alloc(&kb, std::max<uint64_t>(keycap, (uint64_t)n)));
-  const uint64_t sortmax = std::max<uint64_t>(keycap, (uint64_t)n);
-  GARY_TRY(bufs.alloc(&bh, std::max(sort_bh_count(sortmax), (size_t)tiles_of(sortmax) + 1)));
-  GARY_TRY(bufs.alloc(&part, sort_part_count(sortmax)));
-  GARY_TRY(bufs.alloc(&flag, 2));
-  GARY_HIP(hipMemsetAsync(flag, 0, 2 * sizeof(int32_t), s));
+  const uint32_t nkeys = (uint32_t)(2 * nt);
+  const uint64_t cap_keys = std::max<uint64_t>(nkeys, (uint64_t)n);
+  // Scratch: the library's workspace of (device, stream), carved into the sort's two key buffers
+  // and counters, so repeated builds allocate nothing.
+  int device = 0;
+  GARY_TRY(gary::current_device(&device));
+  Carve plan;
+  const size_t o_ka = plan.take<uint64_t>(cap_keys), o_kb = plan.take<uint64_t>(cap_keys);
+  const size_t o_bh = plan.take<uint32_t>(std::max(sort_bh_count(cap_keys),
+                                                   (size_t)tiles_of(cap_keys) + 1));
+  const size_t o_part = plan.take<uint32_t>(
+      std::max(sort_part_count(cap_keys), (size_t)tiles_of((uint64_t)n + 1) + 1));
+  const size_t o_start = plan.take<uint32_t>((size_t)n + 1), o_len = plan.take<uint32_t>((size_t)n + 1);
+  const size_t o_deg = plan.take<uint32_t>((size_t)n + 1), o_newid = plan.take<int32_t>((size_t)n);
+  const size_t o_rk = plan.take<uint64_t>((size_t)n), o_flag = plan.take<int32_t>(1);
+  char *ws = nullptr;
+  GARY_TRY(gary::workspace_get(device, (void *)s, plan.bytes, (void **)&ws));
+  uint64_t *ka = (uint64_t *)(ws + o_ka), *kb = (uint64_t *)(ws + o_kb);
+  uint32_t *bh = (uint32_t *)(ws + o_bh), *part = (uint32_t *)(ws + o_part);
+  uint32_t *start = (uint32_t *)(ws + o_start), *len = (uint32_t *)(ws + o_len);
+  int32_t *flag = (int32_t *)(ws + o_flag);
+  GARY_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
+  // 1-3: both arcs of every tuple, sorted, deduplicated -> the symmetric arc list `sym`
+  uint64_t *sym = ka, *other = kb;
+  uint32_t m2 = 0;
   if (nt > 0) {
     canon_kernel<<<grid_for((uint64_t)nt), kBlock, 0, s>>>(src, dst, (uint64_t)nt, n, vb,
                                                             sentinel, ka, flag);
     GARY_HIP(hipGetLastError());
     uint64_t *sorted = nullptr;
-    GARY_TRY(radix_sort(ka, kb, (uint32_t)nt, 2 * vb, bh, part, s, &sorted));
-    uint64_t *other = sorted == ka ? kb : ka;
-    const uint32_t tiles = tiles_of((uint64_t)nt);
-    uint32_t *cnt = bh;  // reuse: tiles + 1 counters
-    uniq_count_kernel<<<tiles, kBlock, 0, s>>>(sorted, (uint32_t)nt, sentinel, cnt);
-    GARY_HIP(hipGetLastError());
-    GARY_HIP(hipMemsetAsync(cnt + tiles, 0, sizeof(uint32_t), s));
-    GARY_TRY(scan_u32(cnt, tiles + 1, part, s));
-    uniq_write_kernel<<<tiles, kBlock, 0, s>>>(sorted, (uint32_t)nt, sentinel, cnt, other);
-    GARY_HIP(hipGetLastError());
-    uint32_t hm = 0;
+    GARY_TRY(radix_sort(ka, kb, nkeys, 2 * vb, bh, part, s, &sorted));
+    sym = sorted == ka ? kb : ka;
+    GARY_TRY(compact(sorted, nkeys, KeepFirst{sentinel}, sym, bh, part, s, &m2));
+    other = sorted;
     int32_t hbad = 0;
-    GARY_HIP(hipMemcpyAsync(&hm, cnt + tiles, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     GARY_HIP(hipMemcpyAsync(&hbad, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     GARY_HIP(hipStreamSynchronize(s));
     GARY_CHECK_ARG(!hbad, "device CSR build: vertex id out of [0, n)");
-    m = hm;
-    ukeys = other;
   }
-  const int64_t need = sym ? 2 * m : m;
-  if (!colIdx_alloc && (need > cap || (need > 0 && !colIdx)))
-    return gary::set_error(GARY_ERR_INVALID, "device CSR build: colIdx capacity %lld < %lld",
-                           (long long)cap, (long long)need);
+  const bool symmetric = mode == GARY_CSR_SYMMETRIC || mode == GARY_CSR_SYMMETRIC_RANKED;
+  const int64_t need = symmetric ? (int64_t)m2 : (int64_t)m2 / 2;
   if (colIdx_alloc) {
     void *q = nullptr;
     GARY_HIP(hipMalloc(&q, sizeof(int32_t) * (size_t)std::max<int64_t>(need, 1)));
     *colIdx_alloc = (int32_t *)q;
     colIdx = (int32_t *)q;
+  } else if (need > cap || (need > 0 && !colIdx)) {
+    return gary::set_error(GARY_ERR_INVALID, "device CSR build: colIdx capacity %lld < %lld",
+                           (long long)cap, (long long)need);
   }
-  // 4: degrees (the ranked and degree modes need them)
-  int32_t *deg = nullptr, *newid = nullptr;
-  if (mode == GARY_CSR_ORIENT_DEGREE || mode == GARY_CSR_SYMMETRIC_RANKED) {
-    GARY_TRY(bufs.alloc(&deg, (size_t)n));
-    GARY_HIP(hipMemsetAsync(deg, 0, sizeof(int32_t) * (size_t)std::max(n, 1), s));
-    if (m > 0) {
-      degree_kernel<<<grid_for((uint64_t)m), kBlock, 0, s>>>(ukeys, (uint32_t)m, vb, deg);
-      GARY_HIP(hipGetLastError());
-    }
-  }
-  uint64_t *arcbuf = ukeys == ka ? kb : ka;  // free half of the key buffers
-  if (mode == GARY_CSR_SYMMETRIC_RANKED && n > 0) {
-    GARY_TRY(bufs.alloc(&newid, (size_t)n));
-    rank_keys_kernel<<<grid_for((uint64_t)n), kBlock, 0, s>>>(deg, n, vb, arcbuf);
-    GARY_HIP(hipGetLastError());
-    // ukeys is still live, so the sort's ping-pong partner is a buffer of its own; degrees are
-    // at most n - 1 < 2^vb, so the keys have 2*vb significant bits.
-    uint64_t *rk = nullptr;
-    GARY_TRY(bufs.alloc(&rk, (size_t)n));
-    uint64_t *sorted = nullptr;
-    GARY_TRY(radix_sort(arcbuf, rk, (uint32_t)n, 2 * vb, bh, part, s, &sorted));
-    rank_ids_kernel<<<grid_for((uint64_t)n), kBlock, 0, s>>>(sorted, n, vb, newid);
-    GARY_HIP(hipGetLastError());
-  }
-  // 5-6: arcs, sort, CSR. Symmetric modes write 2m arcs: ukeys must not alias arcbuf's range,
-  // and the sort's ping-pong partner is a fresh buffer when the arc count exceeds nt.
-  const int am = mode == GARY_CSR_ORIENT_DEGREE ? kArcDegree
-               : mode == GARY_CSR_ORIENT_ID     ? kArcId
-               : mode == GARY_CSR_SYMMETRIC     ? kArcSym
-                                                : kArcRanked;
-  if (m > 0) {
-    arcs_kernel<<<grid_for((uint64_t)m), kBlock, 0, s>>>(ukeys, (uint32_t)m, vb, am, deg, newid,
-                                                         arcbuf);
-    GARY_HIP(hipGetLastError());
-  }
-  uint64_t *sorted = arcbuf;
-  if (need > 1) {
-    // ukeys is dead once the arcs are written: its buffer is the ping-pong partner
-    GARY_TRY(radix_sort(arcbuf, ukeys, (uint32_t)need, 2 * vb, bh, part, s, &sorted));
-  }
-  if (n > 0 || need > 0) {
-    csr_kernel<<<grid_for((uint64_t)need + 1), kBlock, 0, s>>>(sorted, (uint32_t)need, n, vb,
-                                                              indPtr, colIdx);
-    GARY_HIP(hipGetLastError());
+  if (mode == GARY_CSR_SYMMETRIC) {
+    GARY_TRY(csr_from_sorted(sym, m2, n, vb, start, len, part, indPtr, colIdx, s));
+  } else if (mode == GARY_CSR_ORIENT_ID) {
+    uint32_t kept = 0;
+    GARY_TRY(compact(sym, m2, KeepIdOrder{vb, mask}, other, bh, part, s, &kept));
+    GARY_TRY(csr_from_sorted(other, kept, n, vb, start, len, part, indPtr, colIdx, s));
   } else {
-    GARY_HIP(hipMemsetAsync(indPtr, 0, sizeof(int64_t), s));
+    // degrees = row lengths of the symmetric list
+    uint32_t *deg = (uint32_t *)(ws + o_deg);
+    GARY_TRY(row_lengths(sym, m2, n, vb, start, deg, nullptr, s));
+    if (mode == GARY_CSR_ORIENT_DEGREE) {
+      uint32_t kept = 0;
+      GARY_TRY(compact(sym, m2, KeepDegreeOrder{vb, mask, deg}, other, bh, part, s, &kept));
+      GARY_TRY(csr_from_sorted(other, kept, n, vb, start, len, part, indPtr, colIdx, s));
+    } else {  // ranked: new id = rank of (deg v, v); relabel and sort again
+      int32_t *newid = (int32_t *)(ws + o_newid);
+      uint64_t *rk = (uint64_t *)(ws + o_rk);
+      uint64_t *arcs = sym;
+      if (n > 0) {
+        // degrees are at most n - 1 < 2^vb, so the ranking keys have 2*vb significant bits
+        rank_keys_kernel<<<grid_for((uint64_t)n), kBlock, 0, s>>>(deg, n, vb, other);
+        GARY_HIP(hipGetLastError());
+        uint64_t *sorted = nullptr;
+        GARY_TRY(radix_sort(other, rk, (uint32_t)n, 2 * vb, bh, part, s, &sorted));
+        rank_ids_kernel<<<grid_for((uint64_t)n), kBlock, 0, s>>>(sorted, n, vb, newid);
+        GARY_HIP(hipGetLastError());
+      }
+      if (m2 > 0) {
+        relabel_kernel<<<grid_for(m2), kBlock, 0, s>>>(sym, m2, vb, newid, other);
+        GARY_HIP(hipGetLastError());
+        GARY_TRY(radix_sort(other, sym, m2, 2 * vb, bh, part, s, &arcs));
+      }
+      GARY_TRY(csr_from_sorted(arcs, m2, n, vb, start, len, part, indPtr, colIdx, s));
+    }
   }
   GARY_HIP(hipStreamSynchronize(s));
   *nnz_out = need;
