@@ -277,6 +277,8 @@ class Bench:
                                          graph.ORIENT_DEGREE),
                                         ("rmat22_ranked", 22, graph.SEEDS["rmat22"],
                                          graph.SYMMETRIC_RANKED)):
+            if name not in self.a.build_graphs.split(","):
+                continue
             n, nt = 1 << scale, 16 << scale
             host = {}
 
@@ -340,6 +342,8 @@ def main(argv=None) -> int:
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--strategy", default="learned")
     ap.add_argument("--legs", default="headline,c3,batch,hom4,build")
+    ap.add_argument("--build-graphs", default="rmat20_degree,rmat22_ranked",
+                    help="graphs of the build leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--only", action="store_true",

@@ -61,7 +61,8 @@ class GraphCSR {
     if (src.size() != dst.size()) throw gary::Error(GARY_ERR_INVALID, "src/dst size mismatch");
     const int64_t nt = (int64_t)src.size();
     std::vector<eidT> ptr((size_t)n + 1);
-    std::vector<vidT> idx((size_t)(mode == GARY_CSR_SYMMETRIC ? 2 * nt : nt) + 1);
+    const bool sym = mode == GARY_CSR_SYMMETRIC || mode == GARY_CSR_SYMMETRIC_RANKED;
+    std::vector<vidT> idx((size_t)(sym ? 2 * nt : nt) + 1);
     int64_t nnz = 0;
     gary::check(gary_csr_from_edges(n, nt, src.data(), dst.data(), mode, ptr.data(), idx.data(),
                                     (int64_t)idx.size() - 1, &nnz));
@@ -93,6 +94,13 @@ class DeviceGraphCtx {
     gary::check(gary_ctx_load_csr(h_, graph->numV(), graph->numE(), graph->getPtr().data(),
                                   graph->getIdx().data()));
     host_ = std::move(graph);
+  }
+  // Build the CSR on the device from device tuple arrays (symmetrize, drop loops, dedup, orient
+  // by `mode`), replacing the host GraphCOO -> GraphCSR build followed by loadFromHost.
+  void buildFromEdges(vidT n, const vidT *d_src, const vidT *d_dst, int64_t ntuples,
+                      int mode = GARY_CSR_ORIENT_DEGREE, void *stream = nullptr) {
+    gary::check(gary_ctx_build_from_edges(h_, n, ntuples, d_src, d_dst, mode, stream));
+    host_.reset();
   }
   // Create the COO rowIdx on the device.
   void populateDevice() { gary::check(gary_ctx_populate(h_)); }

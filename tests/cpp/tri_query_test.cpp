@@ -124,6 +124,19 @@ int main(int argc, char **argv) {
                          g.csr.colIdx + csr->getPtr()[b], csr->getDegree(b), counts.data(), m);
     expect("SetOp intersect_count", counts.download()[0], want, timer.stop());
   }
+  {  // the same CSR built on the device from the uploaded tuples
+    DeviceArray<vidT> ds(src.size()), dd(dst.size());
+    ds.upload(src.data());
+    dd.upload(dst.data());
+    DeviceGraphCtx built;
+    timer.start();
+    built.buildFromEdges(n, ds.data(), dd.data(), (int64_t)src.size());
+    const float ms = timer.stop();
+    auto h = built.getHostCSR();
+    const bool same = h->numE() == csr->numE() && h->getPtr() == csr->getPtr() &&
+                      h->getIdx() == csr->getIdx();
+    expect("DeviceGraphCtx::buildFromEdges", same ? 1 : 0, 1, ms);
+  }
   // the façade's exceptions: an out-of-range id is rejected before any device work
   try {
     std::vector<vidT> bad_s = {0}, bad_d = {n + 5};
