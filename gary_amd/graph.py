@@ -125,3 +125,71 @@ def oriented(n, s, d) -> GraphCSR:
 
 def symmetric(n, s, d) -> GraphCSR:
     return csr_from_edges(n, s, d, SYMMETRIC)
+
+
+# ---- graph files (the reference's saveToBinary / loadFromBinary / loadFromSNAP) -------------
+
+_lib.bind("gary_graph_file_info", C.c_int, [C.c_char_p, C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_int64)])
+_lib.bind("gary_graph_file_read_csr", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p])
+_lib.bind("gary_graph_file_read_coo", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p])
+_lib.bind("gary_graph_file_write_csr", C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p])
+_lib.bind("gary_graph_file_write_coo", C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.c_void_p,
+                                                 C.c_void_p])
+_lib.bind("gary_snap_read", C.c_int, [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64,
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32)])
+
+FILE_CSR, FILE_COO = 0, 1
+
+
+def _path(p) -> bytes:
+    return str(p).encode()
+
+
+def file_info(path) -> tuple[int, int, int]:
+    """(format, m, nnz) of a binary graph file (cereal binary archive, SerDesVersion V1)."""
+    fmt, m, nnz = C.c_int(0), C.c_int32(0), C.c_int64(0)
+    check(lib.gary_graph_file_info(_path(path), C.byref(fmt), C.byref(m), C.byref(nnz)))
+    return fmt.value, m.value, nnz.value
+
+
+def load_from_binary(path):
+    """loadFromBinary<GraphCSR | GraphCOO> (include/Graph.h:301-309): a GraphCSR for a CSR file,
+    (m, row, col) for a COO file."""
+    fmt, m, nnz = file_info(path)
+    if fmt == FILE_CSR:
+        indPtr = np.empty(m + 1, np.int64)
+        colIdx = np.empty(max(nnz, 1), np.int32)
+        deg = np.empty(max(m, 1), np.int32)
+        check(lib.gary_graph_file_read_csr(_path(path), ptr(indPtr), ptr(colIdx), ptr(deg)))
+        return GraphCSR(indPtr, colIdx[:nnz].copy(), deg[:m].copy())
+    row = np.empty(max(nnz, 1), np.int32)
+    col = np.empty(max(nnz, 1), np.int32)
+    check(lib.gary_graph_file_read_coo(_path(path), ptr(row), ptr(col)))
+    return m, row[:nnz].copy(), col[:nnz].copy()
+
+
+def save_to_binary(graph, path) -> None:
+    """saveToBinary (include/Graph.h:311-316) of a GraphCSR, or of a COO given as (m, row, col)."""
+    if isinstance(graph, GraphCSR):
+        check(lib.gary_graph_file_write_csr(_path(path), graph.m, graph.nnz, ptr(graph.indPtr),
+                                            ptr(graph.colIdx), ptr(graph.deg)))
+    else:
+        m, row, col = graph
+        row = np.ascontiguousarray(row, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        check(lib.gary_graph_file_write_coo(_path(path), int(m), int(row.shape[0]), ptr(row),
+                                            ptr(col)))
+
+
+def load_from_snap(path, remove_loops: bool = True) -> tuple[int, np.ndarray, np.ndarray]:
+    """loadFromSNAP (src/Graph.cpp:157-196): (number of vertices = max id + 1, src, dst)."""
+    nt, n = C.c_int64(0), C.c_int32(0)
+    check(lib.gary_snap_read(_path(path), int(remove_loops), None, None, 0, C.byref(nt),
+                             C.byref(n)))
+    src = np.empty(max(nt.value, 1), np.int32)
+    dst = np.empty(max(nt.value, 1), np.int32)
+    check(lib.gary_snap_read(_path(path), int(remove_loops), ptr(src), ptr(dst), nt.value,
+                             C.byref(nt), C.byref(n)))
+    return n.value, src[:nt.value].copy(), dst[:nt.value].copy()

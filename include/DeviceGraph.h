@@ -78,6 +78,41 @@ class GraphCSR {
   std::vector<vidT> deg_;
 };
 
+// loadFromBinary<GraphCSR> (include/Graph.h:301-309): a CSR file written by the reference's
+// saveToBinary (cereal binary archive, SerDesVersion V1) or by saveToBinary below.
+inline GraphCSR loadCSRFromBinary(const std::string &path) {
+  int format = 0;
+  vidT m = 0;
+  eidT nnz = 0;
+  gary::check(gary_graph_file_info(path.c_str(), &format, &m, &nnz));
+  std::vector<eidT> ptr((size_t)m + 1);
+  std::vector<vidT> idx((size_t)nnz);
+  gary::check(gary_graph_file_read_csr(path.c_str(), ptr.data(), idx.data(), nullptr));
+  return GraphCSR(m, nnz, std::move(ptr), std::move(idx));
+}
+
+// saveToBinary (include/Graph.h:311-316) of a CSR, readable by the reference's loadFromBinary.
+inline void saveToBinary(const GraphCSR &g, const std::string &path) {
+  gary::check(gary_graph_file_write_csr(path.c_str(), g.numV(), g.numE(), g.getPtr().data(),
+                                        g.getIdx().data(), g.getDegreeAll().data()));
+}
+
+// loadFromSNAP (src/Graph.cpp:157-196): tuples of a SNAP edge list, n = max id + 1.
+struct EdgeList {
+  vidT n = 0;
+  std::vector<vidT> src, dst;
+};
+inline EdgeList loadFromSNAP(const std::string &path, bool removeSelfLoops = true) {
+  EdgeList e;
+  int64_t nt = 0;
+  gary::check(gary_snap_read(path.c_str(), removeSelfLoops, nullptr, nullptr, 0, &nt, &e.n));
+  e.src.resize((size_t)nt);
+  e.dst.resize((size_t)nt);
+  gary::check(gary_snap_read(path.c_str(), removeSelfLoops, e.src.data(), e.dst.data(), nt, &nt,
+                             &e.n));
+  return e;
+}
+
 // Owner of one graph's device buffers.
 class DeviceGraphCtx {
  public:

@@ -102,6 +102,29 @@ int gary_csr_from_edges(int32_t n, int64_t ntuples, const int32_t *src, const in
                         int mode, int64_t *indPtr, int32_t *colIdx, int64_t cap,
                         int64_t *nnz_out);
 
+/* ---- graph files (the reference's saveToBinary / loadFromBinary and loadFromSNAP,
+ *      include/Graph.h:16-23,133-148,270-285,301-320, src/Graph.cpp:157-196) ------------------ */
+enum { GARY_GRAPH_FILE_CSR = 0, GARY_GRAPH_FILE_COO = 1 }; /* GraphFormat */
+
+/* Header of a binary graph file (cereal binary archive, SerDesVersion V1): its format, m and nnz.
+ * Files written by the reference's saveToBinary<GraphCSR / GraphCOO> read unchanged. */
+int gary_graph_file_info(const char *path, int *format, int32_t *m, int64_t *nnz);
+/* A GraphCSR file into indPtr[m+1], colIdx[nnz], deg[m] (a NULL array is skipped). */
+int gary_graph_file_read_csr(const char *path, int64_t *indPtr, int32_t *colIdx, int32_t *deg);
+/* A GraphCOO file into row[nnz], col[nnz]. */
+int gary_graph_file_read_coo(const char *path, int32_t *row, int32_t *col);
+/* Write a GraphCSR file the reference's loadFromBinary<GraphCSR> reads (deg NULL: derived). */
+int gary_graph_file_write_csr(const char *path, int32_t m, int64_t nnz, const int64_t *indPtr,
+                              const int32_t *colIdx, const int32_t *deg);
+/* Write a GraphCOO file the reference's loadFromBinary<GraphCOO> reads. */
+int gary_graph_file_write_coo(const char *path, int32_t m, int64_t nnz, const int32_t *row,
+                              const int32_t *col);
+/* SNAP edge list: one "src dst" pair per line, '#' comments and blank lines skipped, loops
+ * dropped when remove_loops. Call with src = dst = NULL for *ntuples and *n (max id + 1, at
+ * least 1), then with arrays of cap >= *ntuples entries. */
+int gary_snap_read(const char *path, int remove_loops, int32_t *src, int32_t *dst, int64_t cap,
+                   int64_t *ntuples, int32_t *n);
+
 /* ---- device graph construction (the same builds on the GPU; the reference has host code only:
  *      src/Graph.cpp:19-91, common/ParRadixSort.h:218-306, include/Graph.h:242-248) -------- */
 

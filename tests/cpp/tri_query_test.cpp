@@ -137,6 +137,15 @@ int main(int argc, char **argv) {
                       h->getIdx() == csr->getIdx();
     expect("DeviceGraphCtx::buildFromEdges", same ? 1 : 0, 1, ms);
   }
+  {  // saveToBinary / loadFromBinary<GraphCSR> round trip of the reference's file format
+    const std::string path = "/tmp/gary_tri_query_test_csr.bin";
+    saveToBinary(*csr, path);
+    GraphCSR back = loadCSRFromBinary(path);
+    std::remove(path.c_str());
+    const bool same = back.numV() == csr->numV() && back.getPtr() == csr->getPtr() &&
+                      back.getIdx() == csr->getIdx() && back.getDegreeAll() == csr->getDegreeAll();
+    expect("saveToBinary/loadFromBinary", same ? 1 : 0, 1, 0.0);
+  }
   // the façade's exceptions: an out-of-range id is rejected before any device work
   try {
     std::vector<vidT> bad_s = {0}, bad_d = {n + 5};
