@@ -356,6 +356,155 @@ __global__ void hom_sums_kernel(gary_device_graph g, const int64_t *__restrict__
   }
 }
 
+// ---- k-cliques ----------------------------------------------------------------------------------
+//
+// #K_k = Σ over pivots u of the (k-1)-cliques of the local DAG G_u on N+(u) (rows as in
+// hom_local_kernel: bit j of row i set iff the j-th vertex of N+(u) is an out-neighbour of the
+// i-th; only bits above i). A wave takes top-level rows i = wave, wave + 4, ... and walks the
+// candidate sets depth first: S_1 = row_i, S_{t+1} = S_t & row_x for each x in S_t; lane l holds
+// 32-bit word l of the current set (2W <= 64 words since d <= 2048), so an intersection is one
+// AND per lane and the set bits of S_t are taken in order with a ballot and a readlane. The last
+// level counts the arcs inside a set lane-parallel, as hom_local_kernel does for K4: the set is
+// staged in LDS, its set bits are spread over the lanes, and lane a adds popc(S & row_y) for its
+// bit y.
+
+// Arcs of the local DAG inside the set whose 32-bit word `lane` is S (per-lane partial).
+__device__ uint64_t kc_arcs_in(const uint32_t *rows, int32_t w2, uint32_t S, uint32_t *stage) {
+  const int lane = lane_id();
+  stage[lane] = S;
+  wave_sync();
+  const int32_t nb = __popc(S);
+  const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nb);
+  const int32_t excl = incl - nb;
+  const int32_t bits = __builtin_amdgcn_readlane(incl, kWave - 1);
+  uint64_t acc = 0;
+  for (int32_t a0 = 0; a0 < bits; a0 += kWave) {
+    const int32_t a = a0 + lane;
+    int32_t o = 0;  // last word with excl <= a
+#pragma unroll
+    for (int32_t step = kWave / 2; step > 0; step >>= 1)
+      if (__shfl(excl, o + step, kWave) <= a) o += step;
+    uint32_t wv = (uint32_t)__shfl((int32_t)S, o, kWave);
+    int32_t k = a - __shfl(excl, o, kWave);
+    if (a >= bits) continue;
+    int32_t pos = 0;  // position of the k-th set bit of wv
+#pragma unroll
+    for (int32_t width = 16; width > 0; width >>= 1) {
+      const int32_t c = __popc(wv & ((1u << width) - 1u));
+      if (k >= c) {
+        k -= c;
+        wv >>= width;
+        pos += width;
+      }
+    }
+    const int32_t y = o * 32 + pos;
+    const uint32_t *ry = rows + (int64_t)y * w2;
+    uint32_t c = 0;
+    for (int32_t w = (y + 1) >> 5; w < w2; ++w) c += (uint32_t)__popc(stage[w] & ry[w]);
+    acc += c;
+  }
+  wave_sync();
+  return acc;
+}
+
+// Depth-first levels above the last: for each x in S, recurse on S & row_x.
+template <int D>
+__device__ uint64_t kc_dfs(const uint32_t *rows, int32_t w2, uint32_t S, uint32_t *stage) {
+  const int lane = lane_id();
+  uint32_t rem = S;
+  uint64_t acc = 0;
+  while (true) {
+    const uint64_t nz = __ballot(rem != 0u);
+    if (!nz) break;
+    const int l = __ffsll((unsigned long long)nz) - 1;
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readlane((int)rem, l);
+    const int32_t x = l * 32 + (__ffs(wv) - 1);
+    if (lane == l) rem &= rem - 1u;
+    const uint32_t sx = lane < w2 ? (S & rows[(int64_t)x * w2 + lane]) : 0u;
+    if constexpr (D == 1) {
+      acc += kc_arcs_in(rows, w2, sx, stage);
+    } else {
+      acc += kc_dfs<D - 1>(rows, w2, sx, stage);
+    }
+  }
+  return acc;
+}
+
+struct U128Acc {
+  unsigned long long lo, hi;
+};
+
+// One workgroup per pivot (atomic cursor over all vertices), the local graph built as in
+// hom_local_kernel. K = clique size, 3 <= K <= 10.
+template <int K>
+__global__ void __launch_bounds__(kHomBlock)
+    kclique_kernel(gary_device_graph g, const int64_t *__restrict__ up_start,
+                   const int32_t *__restrict__ up_len, int32_t *__restrict__ cursor,
+                   uint32_t *__restrict__ row_scratch, int64_t scratch_words, U128Acc *total,
+                   int32_t *__restrict__ unsupported) {
+  __shared__ int32_t set[kSetWords];
+  __shared__ uint32_t rows_lds[kRowLdsWords];
+  __shared__ uint32_t stage[kHomWaves][kWave];
+  __shared__ int32_t s_item;
+  const int lane = lane_id(), wib = threadIdx.x / kWave;
+  const int32_t m = g.csr.m;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  uint64_t acc = 0;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
+    __syncthreads();
+    const int32_t u = s_item;
+    if (u >= m) break;
+    const int32_t d = up_len[u];
+    if (d < K - 1) {  // fewer than K - 1 out-neighbours: no K-clique has u as its lowest vertex
+      __syncthreads();
+      continue;
+    }
+    const int32_t *L = col + up_start[u];
+    PosSet ps;
+    const int32_t W = (d + 63) >> 6;
+    const int32_t w2 = 2 * W;  // 32-bit words per local row
+    const int64_t row_words = (int64_t)d * w2;
+    if (d > kMaxHomDeg || !posset_plan(L[0], L[d - 1], d, kSetWords, ps) ||
+        (row_words > kRowLdsWords && row_words > scratch_words)) {
+      if (threadIdx.x == 0) *unsupported = d;
+      __syncthreads();
+      continue;
+    }
+    uint32_t *rows = row_words <= kRowLdsWords ? rows_lds
+                                               : row_scratch + (int64_t)blockIdx.x * scratch_words;
+    for (int64_t i = threadIdx.x; i < row_words; i += kHomBlock) rows[i] = 0;
+    posset_build(set, ps, L, d);  // begins and ends with a workgroup barrier
+    for (int32_t i = wib; i < d; i += kHomWaves) {
+      const int32_t v = L[i];
+      if ((uint32_t)v >= (uint32_t)m) continue;
+      const int64_t vb = up_start[v];
+      const int32_t dv = up_len[v];
+      for (int32_t p = lane; p < dv; p += kWave) {
+        const int32_t j = posset_find(set, ps, col[vb + p]);
+        if (j >= 0) atomicOr(&rows[(int64_t)i * w2 + (j >> 5)], 1u << (j & 31));
+      }
+    }
+    __syncthreads();
+    for (int32_t i = wib; i < d; i += kHomWaves) {
+      const uint32_t S = lane < w2 ? rows[(int64_t)i * w2 + lane] : 0u;
+      if constexpr (K == 3) {
+        acc += (uint32_t)__popc(S);
+      } else if constexpr (K == 4) {
+        acc += kc_arcs_in(rows, w2, S, stage[wib]);
+      } else {
+        acc += kc_dfs<K - 4>(rows, w2, S, stage[wib]);
+      }
+    }
+    __syncthreads();
+  }
+  const uint64_t s = wave_sum_u64(acc);
+  if (lane == 0 && s) {
+    const unsigned long long old = atomicAdd(&total->lo, (unsigned long long)s);
+    if (old + s < old) atomicAdd(&total->hi, 1ull);
+  }
+}
+
 // ---- 4-cycles ------------------------------------------------------------------------------------
 
 // First index of N(v) with value >= key.
@@ -741,6 +890,83 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
     out->hom_c4 = (uint64_t)hc4;
     out->c4 = (uint64_t)c4;
   }
+  return GARY_OK;
+  GARY_API_END
+}
+
+int gary_kclique_count(const gary_device_graph *g, int k, uint64_t *count, void *stream) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(g && count, "gary_kclique_count: null pointer");
+  GARY_CHECK_ARG(k >= 1 && k <= 10, "gary_kclique_count: k=%d outside [1, 10]", k);
+  GARY_CHECK_ARG(g->csr.m >= 0 && g->csr.nnz >= 0, "gary_kclique_count: bad sizes");
+  *count = 0;
+  const int32_t m = g->csr.m;
+  if (k == 1) {
+    *count = (uint64_t)m;
+    return GARY_OK;
+  }
+  if (m == 0 || g->csr.nnz == 0) return GARY_OK;
+  GARY_CHECK_ARG(g->csr.indPtr && g->csr.colIdx && g->csr.deg, "gary_kclique_count: null arrays");
+  hipStream_t st = (hipStream_t)stream;
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  int ncu = 0;
+  GARY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = std::max(1, ncu * 4);
+  const int64_t scratch_words = (int64_t)kMaxHomDeg * (kMaxHomDeg / 64) * 2;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_up_start = al(sizeof(int64_t) * (size_t)m);
+  const size_t b_len = al(sizeof(int32_t) * (size_t)m);
+  const size_t b_misc = 256;  // [0] bad, [1] unsupported, [2] cursor; +64: total; +192: asym
+  const size_t b_rows = k >= 3 ? al(sizeof(uint32_t) * (size_t)scratch_words * grid) : 0;
+  void *ws = nullptr;
+  GARY_TRY(workspace_get(dev, (void *)3, b_up_start + 2 * b_len + b_misc + b_rows, &ws));
+  char *p = (char *)ws;
+  int64_t *up_start = (int64_t *)p; p += b_up_start;
+  int32_t *up_len = (int32_t *)p; p += b_len;
+  int32_t *lo_len = (int32_t *)p; p += b_len;
+  char *misc = p; p += b_misc;
+  uint32_t *rows = (uint32_t *)p;
+  int32_t *bad = (int32_t *)misc;
+  U128Acc *total = (U128Acc *)(misc + 64);
+  GARY_HIP(hipMemsetAsync(misc, 0, b_misc, st));
+  hom_split_kernel<<<std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
+      *g, up_start, up_len, lo_len, bad, (unsigned long long *)(misc + 192));
+  GARY_HIP(hipGetLastError());
+  if (k >= 3) {
+    switch (k) {
+#define GARY_KC(K)                                                                               \
+  case K:                                                                                        \
+    kclique_kernel<K><<<grid, kHomBlock, 0, st>>>(*g, up_start, up_len, bad + 2, rows,           \
+                                                   scratch_words, total, bad + 1);               \
+    break;
+      GARY_KC(3) GARY_KC(4) GARY_KC(5) GARY_KC(6) GARY_KC(7) GARY_KC(8) GARY_KC(9) GARY_KC(10)
+#undef GARY_KC
+    }
+    GARY_HIP(hipGetLastError());
+  }
+  int32_t flags[2];
+  unsigned long long asym = 0;
+  U128Acc tot;
+  GARY_HIP(hipMemcpyAsync(flags, bad, sizeof flags, hipMemcpyDeviceToHost, st));
+  GARY_HIP(hipMemcpyAsync(&asym, misc + 192, sizeof asym, hipMemcpyDeviceToHost, st));
+  GARY_HIP(hipMemcpyAsync(&tot, total, sizeof tot, hipMemcpyDeviceToHost, st));
+  GARY_HIP(hipStreamSynchronize(st));
+  if (flags[0] || asym)
+    return set_error(GARY_ERR_INVALID,
+                     "gary_kclique_count: the graph must be symmetric, loop-free and ranked "
+                     "(ids in non-decreasing degree order: GARY_CSR_SYMMETRIC_RANKED)");
+  if (k == 2) {
+    *count = (uint64_t)g->csr.nnz / 2;
+    return GARY_OK;
+  }
+  if (flags[1])
+    return set_error(GARY_ERR_UNSUPPORTED,
+                     "gary_kclique_count: a pivot with out-degree %d exceeds the local-graph limit %d",
+                     flags[1], kMaxHomDeg);
+  if (tot.hi) return set_error(GARY_ERR_OVERFLOW, "gary_kclique_count: the count exceeds 64 bits");
+  *count = tot.lo;
   return GARY_OK;
   GARY_API_END
 }

@@ -97,3 +97,15 @@ def hom4Count(g, patterns: int = HOM_ALL, stream: int | None = None) -> dict:
 def rectCountHom(g) -> int:
     """hom(C4) (the reference declares graph_query::rectCountHom without defining it)."""
     return hom4Count(g, HOM_C4)["hom_c4"]
+
+
+_bind("gary_kclique_count", C.c_int, [C.POINTER(DeviceGraph), C.c_int, C.POINTER(C.c_uint64), _P])
+
+
+def kcliqueCount(g, k: int, stream: int | None = None) -> int:
+    """Number of k-cliques (1 <= k <= 10) of a graph loaded as a ranked symmetric CSR
+    (graph.SYMMETRIC_RANKED, the input of hom4Count). No reference counterpart."""
+    dg = _graph(g)
+    out = C.c_uint64(0)
+    check(lib.gary_kclique_count(C.byref(dg), int(k), C.byref(out), stream))
+    return int(out.value)

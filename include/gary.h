@@ -361,6 +361,13 @@ typedef struct gary_hom4_counts {
 int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *out,
                     void *stream);
 
+/* Number of k-cliques (1 <= k <= 10) of the simple graph given as a GARY_CSR_SYMMETRIC_RANKED CSR:
+ * for each vertex u, the (k-1)-cliques of the DAG induced on its higher-ranked neighbours, counted
+ * on the device. k = 3 is the triangle count, k = 4 the K4 count of gary_hom4_count. No reference
+ * counterpart (the reference counts triangles only). Fails with GARY_ERR_UNSUPPORTED when a
+ * vertex has more than 2,048 higher-ranked neighbours. */
+int gary_kclique_count(const gary_device_graph *g, int k, uint64_t *count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

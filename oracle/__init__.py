@@ -41,6 +41,7 @@ def _load() -> C.CDLL:
         "or_omp_threads": (I32, [I32]),
         "or_hom4": (C.c_int, [I32, P, P, I32, P]),
         "or_hom4_ranked": (C.c_int, [I32, P, P, I32, P]),
+        "or_kclique_count": (C.c_int, [I32, P, P, I32, I32, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -157,3 +158,16 @@ def hom4_ranked(indPtr, colIdx, threads: int = 0) -> dict:
     if rc != 0:
         raise OverflowError("hom4 count overflows 64 bits")
     return {k: int(getattr(r, k)) for k, _ in Hom4._fields_}
+
+
+def kclique_count(indPtr, colIdx, k: int, threads: int = 0) -> int:
+    """Number of k-cliques from an acyclically oriented CSR (e.g. csr_from_edges mode 0)."""
+    indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
+    out = C.c_uint64(0)
+    rc = lib.or_kclique_count(indPtr.size - 1, indPtr.ctypes.data, colIdx.ctypes.data, int(k),
+                              threads, C.byref(out))
+    if rc == -4:
+        raise OverflowError("k-clique count overflows 64 bits")
+    if rc != 0:
+        raise ValueError(f"kclique_count: bad k={k}")
+    return int(out.value)

@@ -450,3 +450,70 @@ int or_hom4_ranked(int32_t n, const int64_t *indPtr, const int32_t *colIdx, int3
   out->c4 = (uint64_t)c4;
   return 0;
 }
+
+/* ---- k-cliques --------------------------------------------------------------------------- */
+
+static int64_t kc_merge(const int32_t *a, int64_t na, const int32_t *b, int64_t nb, int32_t *out) {
+  int64_t i = 0, j = 0, n = 0;
+  while (i < na && j < nb) {
+    if (a[i] < b[j]) ++i;
+    else if (a[i] > b[j]) ++j;
+    else {
+      out[n++] = a[i];
+      ++i;
+      ++j;
+    }
+  }
+  return n;
+}
+
+/* f(S, j) with S = bufs[level][0..ns); bufs[level + 1] receives the next candidate set. */
+static unsigned __int128 kc_rec(const int64_t *indPtr, const int32_t *colIdx, int32_t **bufs,
+                                int32_t level, int64_t ns, int32_t j) {
+  if (j == 1) return (unsigned __int128)ns;
+  const int32_t *S = bufs[level];
+  unsigned __int128 acc = 0;
+  for (int64_t x = 0; x < ns; ++x) {
+    const int32_t v = S[x];
+    const int64_t nn = kc_merge(S, ns, colIdx + indPtr[v], indPtr[v + 1] - indPtr[v],
+                                bufs[level + 1]);
+    if (nn >= j - 1) acc += kc_rec(indPtr, colIdx, bufs, level + 1, nn, j - 1);
+  }
+  return acc;
+}
+
+int or_kclique_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t k,
+                     int32_t threads, uint64_t *count) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  *count = 0;
+  if (k < 1) return -1;
+  if (k == 1) {
+    *count = (uint64_t)m;
+    return 0;
+  }
+  int64_t maxd = 0;
+  for (int32_t v = 0; v < m; ++v)
+    if (indPtr[v + 1] - indPtr[v] > maxd) maxd = indPtr[v + 1] - indPtr[v];
+  unsigned __int128 total = 0;
+#pragma omp parallel reduction(+ : total)
+  {
+    int32_t **bufs = (int32_t **)malloc(sizeof(int32_t *) * (size_t)k);
+    for (int32_t l = 0; l < k; ++l) bufs[l] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(maxd + 1));
+#pragma omp for schedule(dynamic, 64)
+    for (int32_t u = 0; u < m; ++u) {
+      const int64_t d = indPtr[u + 1] - indPtr[u];
+      if (d < k - 1) continue;
+      memcpy(bufs[0], colIdx + indPtr[u], sizeof(int32_t) * (size_t)d);
+      total += kc_rec(indPtr, colIdx, bufs, 0, d, k - 1);
+    }
+    for (int32_t l = 0; l < k; ++l) free(bufs[l]);
+    free(bufs);
+  }
+  if (total > (unsigned __int128)UINT64_MAX) return -4;
+  *count = (uint64_t)total;
+  return 0;
+}

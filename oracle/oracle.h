@@ -97,6 +97,16 @@ int or_hom4(int32_t n, const int64_t *indPtr, const int32_t *colIdx, int32_t thr
 int or_hom4_ranked(int32_t n, const int64_t *indPtr, const int32_t *colIdx, int32_t threads,
                    or_hom4_result *out);
 
+/* Number of k-cliques (k >= 1) of a simple graph given by an acyclic orientation (e.g. mode 0 of
+ * or_csr_from_edges; rows sorted ascending). No reference counterpart (the reference counts
+ * triangles only). Recursion over sorted-list intersections:
+ *   #K_k = sum over u of f(N+(u), k-1),  f(S, 1) = |S|,  f(S, j) = sum over v in S of
+ *   f(S ∩ N+(v), j-1);
+ * every clique is counted once, along its unique topological order. 128-bit accumulation;
+ * returns 0, or -4 when the count overflows 64 bits. */
+int or_kclique_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t k,
+                     int32_t threads, uint64_t *count);
+
 #ifdef __cplusplus
 }
 #endif

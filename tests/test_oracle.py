@@ -212,3 +212,59 @@ def test_oracle_plants_fail():
     ci = colIdx.copy()
     ci[2] = 4  # replace arc 0->3 by 0->4 (vertex 4 has no arcs): two triangles disappear
     assert oracle.tri_count(indPtr, ci) == t - 2
+
+
+# ---- k-cliques (no reference counterpart; pinned by networkx, closed forms, brute force) -------
+
+def _kc_fixture():
+    import json
+    return json.loads((GOLDEN / "kcliques.json").read_text())
+
+
+def test_kclique_oracle_matches_networkx_fixture():
+    for rec in _kc_fixture():
+        n = rec["n"]
+        ip, ci = oracle.csr_from_edges(n, np.array(rec["src"], np.int32),
+                                       np.array(rec["dst"], np.int32), 0)
+        got = [oracle.kclique_count(ip, ci, k) for k in rec["k"]]
+        assert got == rec["count"], rec["name"]
+
+
+def test_kclique_oracle_closed_forms():
+    from math import comb
+
+    from gary_amd import graph
+    # complete t-partite graph with parts of size s: #K_j = C(t, j) * s^j
+    for t, s in ((5, 3), (7, 2), (4, 4)):
+        part = np.repeat(np.arange(t), s)
+        n = t * s
+        src, dst = [], []
+        for a in range(n):
+            for b in range(a + 1, n):
+                if part[a] != part[b]:
+                    src.append(a)
+                    dst.append(b)
+        ip, ci = oracle.csr_from_edges(n, np.array(src, np.int32), np.array(dst, np.int32), 0)
+        for j in range(1, t + 2):
+            assert oracle.kclique_count(ip, ci, j) == comb(t, j) * s ** j, (t, s, j)
+    # id orientation gives the same counts as degree orientation
+    n, s, d = graph.rmat(9, 8, seed=4)
+    a = oracle.csr_from_edges(n, s, d, 0)
+    b = oracle.csr_from_edges(n, s, d, 2)
+    for k in range(3, 7):
+        assert oracle.kclique_count(*a, k) == oracle.kclique_count(*b, k)
+
+
+def test_kclique_oracle_brute_force():
+    import itertools
+    rng = np.random.default_rng(5)
+    for _ in range(6):
+        n = 9
+        adj = np.triu(rng.random((n, n)) < 0.6, 1)
+        src, dst = np.nonzero(adj)
+        A = adj | adj.T
+        ip, ci = oracle.csr_from_edges(n, src.astype(np.int32), dst.astype(np.int32), 0)
+        for k in range(3, 8):
+            want = sum(all(A[a, b] for a, b in itertools.combinations(q, 2))
+                       for q in itertools.combinations(range(n), k))
+            assert oracle.kclique_count(ip, ci, k) == want
