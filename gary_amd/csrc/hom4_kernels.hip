@@ -841,7 +841,7 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
     if (hv[0] > 0) {
       const int64_t stride = std::max<int32_t>(hv[1], 1);
       void *cws = nullptr;
-      GARY_TRY(workspace_get(dev, (void *)4, 2 * sizeof(int64_t) * (size_t)stride * grid_win, &cws));
+      GARY_TRY(workspace_get(dev, (void *)8, 2 * sizeof(int64_t) * (size_t)stride * grid_win, &cws));
       const int32_t windows = (m + kC4Win - 1) / kC4Win;
       const int32_t jmax = (windows + kC4Group - 1) / kC4Group;
       GARY_CHECK_ARG((int64_t)hv[0] * jmax < INT32_MAX, "gary_hom4_count: too many window items");

@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from . import _lib
 from ._lib import DeviceGraph, LaunchCfg, check, lib
 from .device import DeviceBuffer, DeviceGraphCtx
@@ -109,3 +111,27 @@ def kcliqueCount(g, k: int, stream: int | None = None) -> int:
     out = C.c_uint64(0)
     check(lib.gary_kclique_count(C.byref(dg), int(k), C.byref(out), stream))
     return int(out.value)
+
+
+_bind("gary_tri_list", C.c_int, [C.POINTER(DeviceGraph), _P, C.c_uint64, C.POINTER(C.c_uint64),
+                                 _P])
+
+
+def triangleListAsync(g, out: DeviceBuffer, cap: int, stream: int | None = None) -> int:
+    """Every triangle of a degree-oriented graph as int32 triples (u, v, w) in the device buffer
+    `out` (3 * cap int32); returns the number of triangles (GaryError with GARY_ERR_OVERFLOW when
+    it exceeds cap). No reference counterpart: the reference counts triangles only."""
+    dg = _graph(g)
+    total = C.c_uint64(0)
+    check(lib.gary_tri_list(C.byref(dg), out.ptr, int(cap), C.byref(total), stream))
+    return int(total.value)
+
+
+def triangleList(g, cap: int | None = None) -> np.ndarray:
+    """triangleListAsync into a fresh buffer, copied back as an (n, 3) int32 array (order set
+    by scheduling)."""
+    if cap is None:
+        cap = triangleCountHom(g, BITMAP)
+    out = DeviceBuffer(max(12 * cap, 4))
+    n = triangleListAsync(g, out, cap)
+    return out.to_array(np.int32, 3 * n).reshape(n, 3)

@@ -368,6 +368,16 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
  * vertex has more than 2,048 higher-ranked neighbours. */
 int gary_kclique_count(const gary_device_graph *g, int k, uint64_t *count, void *stream);
 
+/* Triangle listing on a degree-oriented CSR (GARY_CSR_ORIENT_DEGREE or ORIENT_ID; needs no COO):
+ * every triangle once, as the int32 triple (u, v, w) with v in N+(u) and w in N+(u) ∩ N+(v),
+ * written to the device array out[3*cap]. The order of the triples depends on scheduling.
+ * *total receives the number of triangles; when it exceeds cap only cap triples are written and
+ * the call returns GARY_ERR_OVERFLOW. The intersection-with-output of
+ * include/ops/Intersection.cuh:18-50,76-112 applied to the triangle query of
+ * include/kernels/GraphQueryHom.cuh:15-43 (which the reference only counts). Synchronous. */
+int gary_tri_list(const gary_device_graph *g, int32_t *out, uint64_t cap, uint64_t *total,
+                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

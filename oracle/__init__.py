@@ -42,6 +42,9 @@ def _load() -> C.CDLL:
         "or_hom4": (C.c_int, [I32, P, P, I32, P]),
         "or_hom4_ranked": (C.c_int, [I32, P, P, I32, P]),
         "or_kclique_count": (C.c_int, [I32, P, P, I32, I32, P]),
+        "or_tri_list": (I64, [I32, P, P, P, I64]),
+        "or_tri_list_checksum": (U64, [I32, P, P, I32]),
+        "or_triple_hash": (U64, [I32, I32, I32]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -171,3 +174,32 @@ def kclique_count(indPtr, colIdx, k: int, threads: int = 0) -> int:
     if rc != 0:
         raise ValueError(f"kclique_count: bad k={k}")
     return int(out.value)
+
+
+def tri_list(indPtr, colIdx) -> np.ndarray:
+    """All triangles as an (n, 3) int32 array of (u, v, w), v in N+(u), w in N+(u) ∩ N+(v)."""
+    indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
+    m = indPtr.size - 1
+    n = int(lib.or_tri_list(m, indPtr.ctypes.data, colIdx.ctypes.data, None, 0))
+    out = np.empty((max(n, 1), 3), np.int32)
+    lib.or_tri_list(m, indPtr.ctypes.data, colIdx.ctypes.data, out.ctypes.data, n)
+    return out[:n]
+
+
+def tri_list_checksum(indPtr, colIdx, threads: int = 0) -> int:
+    """Wrapping uint64 sum of triple_hash over the listing (order-independent)."""
+    indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
+    return int(lib.or_tri_list_checksum(indPtr.size - 1, indPtr.ctypes.data, colIdx.ctypes.data,
+                                        threads))
+
+
+def triple_hash_np(t: np.ndarray) -> np.ndarray:
+    """or_triple_hash over the rows of an (n, 3) int32 array, vectorised (uint64, wrapping)."""
+    u, v, w = (t[:, i].astype(np.uint32).astype(np.uint64) for i in range(3))
+    with np.errstate(over="ignore"):
+        x = (u * np.uint64(0x9E3779B97F4A7C15)) ^ (v * np.uint64(0xC2B2AE3D27D4EB4F)) ^ \
+            (w * np.uint64(0x165667B19E3779F9))
+        x ^= x >> np.uint64(29)
+        x *= np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(32)
+    return x

@@ -517,3 +517,66 @@ int or_kclique_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, in
   *count = (uint64_t)total;
   return 0;
 }
+
+/* ---- triangle listing ------------------------------------------------------------------- */
+
+uint64_t or_triple_hash(int32_t u, int32_t v, int32_t w) {
+  uint64_t x = (uint64_t)(uint32_t)u * 0x9E3779B97F4A7C15ull ^
+               (uint64_t)(uint32_t)v * 0xC2B2AE3D27D4EB4Full ^
+               (uint64_t)(uint32_t)w * 0x165667B19E3779F9ull;
+  x ^= x >> 29;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 32;
+  return x;
+}
+
+int64_t or_tri_list(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t *out,
+                    int64_t cap) {
+  int64_t n = 0;
+  for (int32_t u = 0; u < m; ++u)
+    for (int64_t a = indPtr[u]; a < indPtr[u + 1]; ++a) {
+      const int32_t v = colIdx[a];
+      int64_t i = indPtr[u], j = indPtr[v];
+      while (i < indPtr[u + 1] && j < indPtr[v + 1]) {
+        if (colIdx[i] < colIdx[j]) ++i;
+        else if (colIdx[i] > colIdx[j]) ++j;
+        else {
+          if (n < cap) {
+            out[3 * n] = u;
+            out[3 * n + 1] = v;
+            out[3 * n + 2] = colIdx[i];
+          }
+          ++n;
+          ++i;
+          ++j;
+        }
+      }
+    }
+  return n;
+}
+
+uint64_t or_tri_list_checksum(int32_t m, const int64_t *indPtr, const int32_t *colIdx,
+                              int32_t threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  uint64_t sum = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : sum)
+  for (int32_t u = 0; u < m; ++u)
+    for (int64_t a = indPtr[u]; a < indPtr[u + 1]; ++a) {
+      const int32_t v = colIdx[a];
+      int64_t i = indPtr[u], j = indPtr[v];
+      while (i < indPtr[u + 1] && j < indPtr[v + 1]) {
+        if (colIdx[i] < colIdx[j]) ++i;
+        else if (colIdx[i] > colIdx[j]) ++j;
+        else {
+          sum += or_triple_hash(u, v, colIdx[i]);
+          ++i;
+          ++j;
+        }
+      }
+    }
+  return sum;
+}

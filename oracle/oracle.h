@@ -107,6 +107,22 @@ int or_hom4_ranked(int32_t n, const int64_t *indPtr, const int32_t *colIdx, int3
 int or_kclique_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t k,
                      int32_t threads, uint64_t *count);
 
+/* Triangle listing (the emit variant of the triangle query, include/ops/Intersection.cuh:18-50
+ * applied to test/TriQueryTest.cu:17-37's loops): triples (u, v, w), v in N+(u), w in
+ * N+(u) ∩ N+(v), in order of u, then v's position, then w; writes the first cap triples to out
+ * and returns the number of triangles. */
+int64_t or_tri_list(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t *out,
+                    int64_t cap);
+
+/* Order-independent checksum of the listing: the wrapping uint64 sum over triangles of
+ * or_triple_hash(u, v, w) (threads <= 0: OpenMP default). */
+uint64_t or_tri_list_checksum(int32_t m, const int64_t *indPtr, const int32_t *colIdx,
+                              int32_t threads);
+
+/* x = u*0x9E3779B97F4A7C15 ^ v*0xC2B2AE3D27D4EB4F ^ w*0x165667B19E3779F9; x ^= x >> 29;
+ * x *= 0xBF58476D1CE4E5B9; x ^= x >> 32 (all uint64, wrapping). */
+uint64_t or_triple_hash(int32_t u, int32_t v, int32_t w);
+
 #ifdef __cplusplus
 }
 #endif
