@@ -16,6 +16,8 @@ Further legs (--legs, default all):
          per-graph uint64 counts;
   hom4   K4 / diamond / 4-cycle homomorphism counts on R-MAT scale 22 (wall clock per call:
          each call returns its counts to the host).
+  patterns  triangle listing (every triangle written as an int32 triple) on R-MAT 20, and
+         k-clique counts (k = 4, 5) on R-MAT 20 (ranked symmetric CSR built on the device);
   build  graph construction on the device (R-MAT tuples, symmetrize, dedup, orient, CSR) for
          R-MAT 20 degree-oriented and R-MAT 22 ranked symmetric, against the host build of the
          same tuples (equal arrays required).
@@ -304,6 +306,50 @@ class Bench:
             del ds, dd, host
         return leg
 
+    def patterns(self) -> dict:
+        """Triangle listing and k-cliques on R-MAT 20 (wall clock per call, median of 5 after a
+        warm-up; each call returns its count to the host)."""
+        from gary_amd.device import DeviceBuffer, gen_rmat_device
+        gq, graph = self.gq, self.graph
+
+        def med(f, k=5):
+            f()
+            ts = []
+            for _ in range(k):
+                t0 = time.perf_counter()
+                r = f()
+                ts.append(time.perf_counter() - t0)
+            return sorted(ts)[k // 2] * 1e3, r
+
+        n, nt = 1 << 20, 16 << 20
+        ds, dd = gen_rmat_device(20, nt, seed=graph.SEEDS["rmat20"])
+        ctx = self.DeviceGraphCtx()
+        ctx.buildFromEdges(n, ds, dd, nt, graph.ORIENT_DEGREE)
+        tri = gq.triangleCountHom(ctx, gq.BITMAP)
+        out = DeviceBuffer(12 * tri)
+        list_ms, listed = med(lambda: gq.triangleListAsync(ctx, out, tri))
+        if listed != tri:
+            raise SystemExit(f"listing found {listed} triangles, count {tri}")
+        g = ctx.getHostCSR()
+        b_alg, _ = self.pivot_bytes(g)
+        leg = {"workload": "rmat20-ef16 (a=.57,b=.19,c=.19), degree-oriented int32 CSR; every "
+                           "triangle written as an int32 triple",
+               "triangles": tri, "list_ms": list_ms, "triangles_per_s": tri / (list_ms * 1e-3),
+               "output_bytes": 12 * tri,
+               "roofline": {"algorithmic_bytes": b_alg + 12 * tri,
+                            "achieved": (b_alg + 12 * tri) / (list_ms * 1e-3) / 1e9,
+                            "frac": (b_alg + 12 * tri) / (list_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+        log(f"patterns: listed {tri} triangles in {list_ms:.2f} ms")
+        del out
+        rctx = self.DeviceGraphCtx()
+        rctx.buildFromEdges(n, ds, dd, nt, graph.SYMMETRIC_RANKED)
+        for k in (4, 5):
+            ms, c = med(lambda: gq.kcliqueCount(rctx, k), 3)
+            leg[f"k{k}_ms"] = ms
+            leg[f"k{k}"] = c
+            log(f"patterns: {c} {k}-cliques in {ms:.1f} ms")
+        return leg
+
     def cpu(self) -> dict:
         import oracle
         a, g = self.a, self.g
@@ -341,7 +387,7 @@ def main(argv=None) -> int:
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--strategy", default="learned")
-    ap.add_argument("--legs", default="headline,c3,batch,hom4,build")
+    ap.add_argument("--legs", default="headline,c3,batch,hom4,build,patterns")
     ap.add_argument("--build-graphs", default="rmat20_degree,rmat22_ranked",
                     help="graphs of the build leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -383,6 +429,8 @@ def main(argv=None) -> int:
         out["hom4"] = b.hom4()
     if "build" in legs:
         out["build"] = b.build()
+    if "patterns" in legs:
+        out["patterns"] = b.patterns()
     if a.calibrate_copy:
         b.calibrate_copy(a.calibrate_copy)
     out["cpu_baseline"] = b.cpu() if "headline" in legs and not a.no_cpu else None

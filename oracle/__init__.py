@@ -176,13 +176,15 @@ def kclique_count(indPtr, colIdx, k: int, threads: int = 0) -> int:
     return int(out.value)
 
 
-def tri_list(indPtr, colIdx) -> np.ndarray:
-    """All triangles as an (n, 3) int32 array of (u, v, w), v in N+(u), w in N+(u) ∩ N+(v)."""
+def tri_list(indPtr, colIdx, limit: int | None = None) -> np.ndarray:
+    """The triangles as an (n, 3) int32 array of (u, v, w), v in N+(u), w in N+(u) ∩ N+(v), in
+    the sequential order; only the first `limit` when given."""
     indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
     m = indPtr.size - 1
-    n = int(lib.or_tri_list(m, indPtr.ctypes.data, colIdx.ctypes.data, None, 0))
+    n = int(limit) if limit is not None else int(lib.or_tri_list(m, indPtr.ctypes.data,
+                                                                   colIdx.ctypes.data, None, 0))
     out = np.empty((max(n, 1), 3), np.int32)
-    lib.or_tri_list(m, indPtr.ctypes.data, colIdx.ctypes.data, out.ctypes.data, n)
+    n = int(lib.or_tri_list(m, indPtr.ctypes.data, colIdx.ctypes.data, out.ctypes.data, n))
     return out[:n]
 
 

@@ -541,7 +541,8 @@ int64_t or_tri_list(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int
         if (colIdx[i] < colIdx[j]) ++i;
         else if (colIdx[i] > colIdx[j]) ++j;
         else {
-          if (n < cap) {
+          if (out) {
+            if (n == cap) return n;
             out[3 * n] = u;
             out[3 * n + 1] = v;
             out[3 * n + 2] = colIdx[i];

@@ -109,8 +109,8 @@ int or_kclique_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, in
 
 /* Triangle listing (the emit variant of the triangle query, include/ops/Intersection.cuh:18-50
  * applied to test/TriQueryTest.cu:17-37's loops): triples (u, v, w), v in N+(u), w in
- * N+(u) ∩ N+(v), in order of u, then v's position, then w; writes the first cap triples to out
- * and returns the number of triangles. */
+ * N+(u) ∩ N+(v), in order of u, then v's position, then w. With out NULL returns the number of
+ * triangles; otherwise writes the first min(cap, count) triples and returns how many it wrote. */
 int64_t or_tri_list(int32_t m, const int64_t *indPtr, const int32_t *colIdx, int32_t *out,
                     int64_t cap);
 
