@@ -35,7 +35,7 @@ FIELDS = ("block", "grid", "group", "cache_depth", "partition", "bin_small", "bi
 PIVOT_SPACE = {"block": [128, 256, 512], "grid": [1024, 2048, 4096, 8192],
                "bin_small": [16, 32, 64, 128, 256], "bin_hub": [256, 512, 1024, 2048]}
 EDGE_SPACE = {"block": [128, 256], "group": [8, 16, 32, 64], "cache_depth": [16, 32, 64, 128],
-              "partition": [gq.INTERLEAVED, gq.CHUNKED], "grid": [0, 3072, 6144, 12288]}
+              "partition": [gq.INTERLEAVED, gq.CHUNKED, gq.DYNAMIC], "grid": [0, 3072, 6144, 12288]}
 PIVOT_STRATEGIES = ("bitmap", "heuristic", "learned")
 
 

@@ -106,11 +106,15 @@ __device__ __forceinline__ uint32_t isect_group(const int32_t *__restrict__ A, i
 // Edge-centric kernel: one group of G lanes per COO edge.
 // ======================================================================================
 
+// DYNAMIC partition: edges claimed kDynChunk at a time per group from a device-wide cursor.
+constexpr int kDynChunk = 16;
+
 template <int G, int METHOD, int PART>
 __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32_t n_cache,
                                                       unsigned long long *count,
                                                       const uint8_t *__restrict__ route,
-                                                      uint8_t want) {
+                                                      uint8_t want,
+                                                      unsigned long long *dyn_cursor = nullptr) {
   extern __shared__ int32_t smem[];
   const int glane = lane_id() & (G - 1);
   const int gpb = blockDim.x / G;
@@ -121,6 +125,27 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
   const int64_t E = g.coo.nnz;
   const uint32_t m = (uint32_t)g.csr.m;
   int64_t e, e_end, step;
+  if constexpr (PART == GARY_PARTITION_DYNAMIC) {
+    uint32_t c = 0;
+    const int leader = lane_id() & ~(G - 1);
+    while (true) {
+      unsigned long long base = 0;
+      if (glane == 0) base = atomicAdd(dyn_cursor, (unsigned long long)kDynChunk);
+      base = __shfl(base, leader, kWave);
+      if ((int64_t)base >= E) break;
+      const int64_t end = min((int64_t)base + kDynChunk, E);
+      for (e = (int64_t)base; e < end; ++e) {
+        if (route && route[e] != want) continue;
+        const int32_t src = g.coo.rowIdx[e], dst = g.coo.colIdx[e];
+        if ((uint32_t)src >= m || (uint32_t)dst >= m) continue;
+        const int32_t *A = g.csr.colIdx + g.csr.indPtr[src];
+        const int32_t *B = g.csr.colIdx + g.csr.indPtr[dst];
+        c += isect_group<G, METHOD>(A, g.csr.deg[src], B, g.csr.deg[dst], cache, n_cache, glane);
+      }
+    }
+    wave_atomic_add(count, c);
+    return;
+  }
   if constexpr (PART == GARY_PARTITION_CHUNKED) {
     e = gid * E / ngroups;
     e_end = (gid + 1) * E / ngroups;
@@ -709,7 +734,8 @@ int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
   GARY_CHECK_ARG(c.cache_depth >= 1 && c.cache_depth <= 4096 &&
                      (c.cache_depth & (c.cache_depth - 1)) == 0,
                  "launch cfg: cache_depth=%d must be a power of two <= 4096", c.cache_depth);
-  GARY_CHECK_ARG(c.partition == GARY_PARTITION_INTERLEAVED || c.partition == GARY_PARTITION_CHUNKED,
+  GARY_CHECK_ARG(c.partition == GARY_PARTITION_INTERLEAVED || c.partition == GARY_PARTITION_CHUNKED ||
+                     c.partition == GARY_PARTITION_DYNAMIC,
                  "launch cfg: bad partition %d", c.partition);
   GARY_CHECK_ARG(c.grid >= 0, "launch cfg: grid=%d", c.grid);
   GARY_CHECK_ARG(c.bin_small >= 2 && c.bin_hub >= c.bin_small && c.bin_hub <= kPivotWords / 2,
@@ -737,8 +763,22 @@ int launch_edge_g(const gary_device_graph &g, const TcCfg &c, unsigned long long
   const int grid = c.grid ? c.grid : 4096;
   const size_t lds = METHOD == kIsectCached
                          ? (size_t)(c.block / G) * c.cache_depth * sizeof(int32_t) : 0;
+  unsigned long long *cursor = nullptr;
+  if (c.partition == GARY_PARTITION_DYNAMIC) {
+    // the cursor lives in the stream's workspace and is cleared by a kernel (capture-safe)
+    int dev = 0;
+    GARY_TRY(current_device(&dev));
+    void *ws = nullptr;
+    GARY_TRY(workspace_get(dev, (void *)st, 256, &ws));
+    cursor = (unsigned long long *)ws;
+    fill_zero_kernel<<<1, 64, 0, st>>>((uint32_t *)cursor, 2);
+    GARY_HIP(hipGetLastError());
+  }
   GARY_TRY(timing_begin(st));
-  if (c.partition == GARY_PARTITION_CHUNKED)
+  if (c.partition == GARY_PARTITION_DYNAMIC)
+    tc_edge_kernel<G, METHOD, GARY_PARTITION_DYNAMIC><<<grid, c.block, lds, st>>>(
+        g, c.cache_depth, cnt, nullptr, 0, cursor);
+  else if (c.partition == GARY_PARTITION_CHUNKED)
     tc_edge_kernel<G, METHOD, GARY_PARTITION_CHUNKED><<<grid, c.block, lds, st>>>(g, c.cache_depth, cnt,
                                                                                 nullptr, 0);
   else

@@ -25,6 +25,7 @@ STRATEGIES = {"learned": AUTO_LEARNED, "heuristic": HEURISTIC, "merge": MERGE,
 
 INTERLEAVED = 0
 CHUNKED = 1
+DYNAMIC = -1  # WorkPartition::DYNAMIC: chunks of edges claimed from an atomic cursor
 
 _P = C.c_void_p
 _bind = _lib.bind

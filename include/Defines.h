@@ -13,8 +13,9 @@ typedef uint64_t embIdxT;
 constexpr int WAVE_SIZE = 64;
 
 // WorkPartition (include/ops/DeviceIterator.cuh:47-54): how edge ranges are split across
-// groups. DYNAMIC is declared there and rejected by a static_assert; here it is accepted by no
-// launcher either.
+// groups. DYNAMIC is declared there and rejected by a static_assert (:141); here it is
+// implemented: groups claim chunks of edges from a device-wide atomic cursor.
 enum WorkPartition { DYNAMIC = -1, INTERLEAVED = 0, CHUNKED = 1 };
 static constexpr WorkPartition interleavedPolicy = WorkPartition::INTERLEAVED;
 static constexpr WorkPartition chunkedPolicy = WorkPartition::CHUNKED;
+static constexpr WorkPartition dynamicPolicy = WorkPartition::DYNAMIC;

@@ -234,6 +234,9 @@ enum {
   GARY_TC_NUM_STRATEGIES = 6
 };
 enum {
+  GARY_PARTITION_DYNAMIC = -1,    /* WorkPartition::DYNAMIC (DeviceIterator.cuh:47-51, declared
+                                     there but rejected by a static_assert at :141): groups claim
+                                     chunks of edges from a device-wide atomic cursor */
   GARY_PARTITION_INTERLEAVED = 0, /* WorkPartition::INTERLEAVED, DeviceIterator.cuh:117-131 */
   GARY_PARTITION_CHUNKED = 1      /* WorkPartition::CHUNKED */
 };

@@ -29,13 +29,16 @@ inline size_t getIndexCacheSMem(int blockDim, size_t cacheSize) {
 template <bool useCache = true, size_t N = 32, WorkPartition Policy = interleavedPolicy>
 void triangleCountHom(DeviceGraph dataGraph, uint64_t *count, int gridDim = 0, int blockDim = 256,
                       void *stream = nullptr) {
-  static_assert(Policy == INTERLEAVED || Policy == CHUNKED, "Unsupported work partitioning policy");
+  static_assert(Policy == INTERLEAVED || Policy == CHUNKED || Policy == DYNAMIC,
+                "Unsupported work partitioning policy");
   gary_launch_cfg cfg{};
   cfg.block = blockDim;
   cfg.grid = gridDim;
   cfg.group = WAVE_SIZE;
   cfg.cache_depth = (int32_t)N;
-  cfg.partition = Policy == CHUNKED ? GARY_PARTITION_CHUNKED : GARY_PARTITION_INTERLEAVED;
+  cfg.partition = Policy == CHUNKED   ? GARY_PARTITION_CHUNKED
+                  : Policy == DYNAMIC ? GARY_PARTITION_DYNAMIC
+                                      : GARY_PARTITION_INTERLEAVED;
   gary::check(gary_tc_count(&dataGraph, useCache ? GARY_TC_CACHED_BSEARCH : GARY_TC_BSEARCH, &cfg,
                             count, stream));
 }

@@ -100,6 +100,9 @@ int main(int argc, char **argv) {
     graph_query::triangleCountHom<true, 32, chunkedPolicy>(g, counts.data(), 1536, 256);
   });
   run("RunGPUWithCache64", [&] { graph_query::triangleCountHom<true, 64>(g, counts.data()); });
+  run("RunGPUWithCacheDynamic", [&] {
+    graph_query::triangleCountHom<true, 32, dynamicPolicy>(g, counts.data(), 1536, 256);
+  });
   const std::pair<const char *, graph_query::Strategy> strategies[] = {
       {"Strategy::Learned", graph_query::Strategy::Learned},
       {"Strategy::Heuristic", graph_query::Strategy::Heuristic},

@@ -60,7 +60,7 @@ def test_er_4096(er_ctx, strategy):
 
 
 @pytest.mark.parametrize("group", [8, 16, 32, 64])
-@pytest.mark.parametrize("partition", [gq.INTERLEAVED, gq.CHUNKED])
+@pytest.mark.parametrize("partition", [gq.INTERLEAVED, gq.CHUNKED, gq.DYNAMIC])
 @pytest.mark.parametrize("method", [gq.CACHED_BSEARCH, gq.BSEARCH, gq.MERGE])
 def test_er_4096_edge_configs(er_ctx, group, partition, method):
     g, ctx, want = er_ctx
@@ -108,6 +108,18 @@ def rmat16():
 def test_rmat16(rmat16, strategy):
     g, ctx, want = rmat16
     assert gq.triangleCountHom(ctx, strategy) == want
+
+
+@pytest.mark.parametrize("method", [gq.CACHED_BSEARCH, gq.MERGE])
+def test_rmat16_dynamic_partition(rmat16, method):
+    """WorkPartition::DYNAMIC (declared but unsupported in the reference): groups claim chunks of
+    16 edges from a device-wide cursor; the count does not depend on grid or group size, and a
+    second launch restarts the cursor."""
+    g, ctx, want = rmat16
+    for grid, group in ((1, 64), (7, 16), (4096, 8), (0, 32)):
+        cfg = gq.launch_cfg(block=256, grid=grid, group=group, partition=gq.DYNAMIC)
+        assert gq.triangleCountHom(ctx, method, cfg) == want, (grid, group)
+        assert gq.triangleCountHom(ctx, method, cfg) == want, (grid, group)
 
 
 def test_rmat16_hub_kernel_paths(rmat16):
