@@ -68,4 +68,9 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                       const uint8_t *route = nullptr, const float *pcost = nullptr,
                       float cost_big = 0.f, const RouteTable *table = nullptr);
 
+// Triangle listing on the pivot path: every triangle (u, v, w) of a degree-oriented CSR
+// written to out[3 * cap], claims counted in *total (zeroed by the caller).
+int launch_tri_list(const gary_device_graph &g, unsigned long long *total, int32_t *out,
+                    unsigned long long cap, hipStream_t st);
+
 }  // namespace gary

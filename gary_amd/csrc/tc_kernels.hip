@@ -687,6 +687,228 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ======================================================================================
+// Triangle listing on the pivot path
+// ======================================================================================
+//
+// The counting kernel's sets, row streaming and schedule, with every hit (v, w) of pivot u
+// written out as the triple (u, v, w): hits are appended in ballot order to a per-wave LDS
+// stage of kListStage triples, and a full stage claims one output range with a single atomic
+// and is copied out with coalesced stores.
+
+constexpr int kListStage = 256;
+
+struct ListOut {
+  unsigned long long *total;  // triples found (claimed output positions)
+  int32_t *out;               // 3 * cap int32
+  unsigned long long cap;
+};
+
+__device__ __forceinline__ void list_flush(const int32_t *stage, int32_t fill, const ListOut &lo) {
+  wave_sync();  // the staged triples are visible to every lane
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(lo.total, (unsigned long long)fill);
+  base = __shfl(base, 0, kWave);
+  for (int32_t i = lane_id(); i < 3 * fill; i += kWave)
+    if (base + (unsigned long long)(i / 3) < lo.cap)
+      lo.out[3 * base + (unsigned long long)i] = stage[i];
+  wave_sync();  // the stage is read before it is refilled
+}
+
+// Stages this lane's triple when `hit`; every lane of the wave calls it (fill is wave-uniform).
+__device__ __forceinline__ void list_emit(int32_t *stage, int32_t &fill, const ListOut &lo,
+                                          bool hit, int32_t u, int32_t v, int32_t w) {
+  const uint64_t hits = __ballot(hit);
+  if (!hits) return;
+  const int32_t c = __popcll(hits);
+  if (fill + c > kListStage) {
+    list_flush(stage, fill, lo);
+    fill = 0;
+  }
+  if (hit) {
+    const int32_t at = 3 * (fill + lanes_below(hits));
+    stage[at] = u;
+    stage[at + 1] = v;
+    stage[at + 2] = w;
+  }
+  fill += c;
+}
+
+// stream_rows with output: the owning row's vertex travels with each chunk.
+template <int K>
+__device__ __forceinline__ void stream_rows_list(const gary_device_graph &g, int32_t u,
+                                                 const int32_t *__restrict__ L, int32_t d,
+                                                 int32_t r0, int32_t stride, const int32_t *table,
+                                                 const PivotSet &ps, RowSlots &rs, int lane,
+                                                 int32_t *stage, int32_t &fill,
+                                                 const ListOut &lo) {
+  const uint32_t m = (uint32_t)g.csr.m;
+  const int64_t nnz = g.csr.nnz;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  for (int32_t b = r0; b < d; b += stride * kWave) {
+    const int32_t r = b + lane * stride;
+    int32_t dv = 0, v = 0;
+    int64_t pv = 0;
+    if (r < d) {
+      v = L[r];
+      if ((uint32_t)v < m) {
+        dv = g.csr.deg[v];
+        pv = g.csr.indPtr[v];
+      }
+    }
+    const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + 3) >> 2) - (pv >> 2)) : 0;
+    const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
+    const int32_t start = incl - nch;
+    const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    const int32_t cb = (int32_t)(uint32_t)(pv >> 2);
+    const int32_t rlo = (int32_t)(pv & 3);
+    const int32_t rhi = rlo + dv;
+    for (int32_t q0 = 0; q0 < total; q0 += K * kWave) {
+      int4 data[K];
+      int32_t jlo[K], jhi[K], vr[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int32_t w0 = q0 + k * kWave;
+        rs.s[lane] = -1;
+        wave_sync();
+        if (nch > 0 && start < w0 + kWave && incl > w0) rs.s[max(0, start - w0)] = lane;
+        wave_sync();
+        const int32_t row = max(0, wave_inclusive_max_i32(rs.s[lane]));
+        const int32_t j = w0 + lane - __shfl(start, row, kWave);
+        const uint32_t ch = (uint32_t)__shfl(cb, row, kWave) + (uint32_t)j;
+        vr[k] = __shfl(v, row, kWave);
+        jlo[k] = max(0, __shfl(rlo, row, kWave) - 4 * j);
+        jhi[k] = min(4, __shfl(rhi, row, kWave) - 4 * j);
+        data[k] = make_int4(0, 0, 0, 0);
+        if (w0 + lane < total) {
+          const int64_t e0 = 4 * (int64_t)ch;
+          if (e0 + 3 < nnz) {
+            data[k] = *reinterpret_cast<const int4 *>(col + e0);
+          } else {
+            data[k].x = e0 < nnz ? col[e0] : 0;
+            data[k].y = e0 + 1 < nnz ? col[e0 + 1] : 0;
+            data[k].z = e0 + 2 < nnz ? col[e0 + 2] : 0;
+          }
+        } else {
+          jhi[k] = 0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const bool h0 = (jlo[k] <= 0 && 0 < jhi[k]) ? pivot_probe(table, ps, data[k].x) != 0 : false;
+        const bool h1 = (jlo[k] <= 1 && 1 < jhi[k]) ? pivot_probe(table, ps, data[k].y) != 0 : false;
+        const bool h2 = (jlo[k] <= 2 && 2 < jhi[k]) ? pivot_probe(table, ps, data[k].z) != 0 : false;
+        const bool h3 = (jlo[k] <= 3 && 3 < jhi[k]) ? pivot_probe(table, ps, data[k].w) != 0 : false;
+        list_emit(stage, fill, lo, h0, u, vr[k], data[k].x);
+        list_emit(stage, fill, lo, h1, u, vr[k], data[k].y);
+        list_emit(stage, fill, lo, h2, u, vr[k], data[k].z);
+        list_emit(stage, fill, lo, h3, u, vr[k], data[k].w);
+      }
+    }
+  }
+}
+
+// Row by row with binary search in HBM, for a pivot whose set does not fit in LDS.
+__device__ __forceinline__ void fallback_list(const gary_device_graph &g, int32_t u,
+                                              const int32_t *__restrict__ L, int32_t d,
+                                              int32_t r0, int32_t stride, int lane,
+                                              int32_t *stage, int32_t &fill, const ListOut &lo) {
+  const uint32_t m = (uint32_t)g.csr.m;
+  for (int32_t k = r0; k < d; k += stride) {
+    const int32_t v = L[k];
+    if ((uint32_t)v >= m) continue;
+    const int32_t *N = g.csr.colIdx + g.csr.indPtr[v];
+    const int32_t dv = g.csr.deg[v];
+    for (int32_t p0 = 0; p0 < dv; p0 += kWave) {
+      const int32_t p = p0 + lane;
+      const int32_t w = p < dv ? N[p] : 0;
+      bool hit = false;
+      if (p < dv) {
+        int32_t lo_i = 0, hi_i = d;
+        while (lo_i < hi_i) {
+          const int32_t mid = (lo_i + hi_i) >> 1;
+          if (L[mid] < w) lo_i = mid + 1;
+          else hi_i = mid;
+        }
+        hit = lo_i < d && L[lo_i] == w;
+      }
+      list_emit(stage, fill, lo, hit, u, v, w);
+    }
+  }
+}
+
+// The schedule of tc_pivot_kernel (big pivots one workgroup each, then small pivots one wave
+// each, heaviest first), listing instead of counting.
+template <int WAVES, int TABLE_WORDS>
+__global__ void __launch_bounds__(WAVES * 64)
+    tri_list_pivot_kernel(gary_device_graph g, const int32_t *__restrict__ small,
+                          const int32_t *__restrict__ big, const int32_t *__restrict__ counts,
+                          int32_t *__restrict__ cursors, ListOut lo,
+                          const int32_t *__restrict__ split) {
+  __shared__ int32_t table[TABLE_WORDS];
+  __shared__ RowSlots s_rows[WAVES];
+  __shared__ int32_t staged[WAVES][3 * kListStage];
+  const int lane = lane_id();
+  const int wib = threadIdx.x / kWave;
+  RowSlots &rb = s_rows[wib];
+  int32_t *stage = staged[wib];
+  int32_t fill = 0;
+  const int32_t n_small = counts[0], n_big = counts[1];
+  if (n_small == 0 && n_big == 0) return;
+  const uint32_t m = (uint32_t)g.csr.m;
+  while (true) {
+    if (threadIdx.x == 0) table[0] = atomicAdd(&cursors[1], 1);
+    __syncthreads();
+    const int32_t p = table[0];
+    __syncthreads();
+    if (p >= n_big) break;
+    const int32_t u = big[p];
+    const int32_t d = (uint32_t)u < m ? g.csr.deg[u] : 0;
+    if (d >= 2) {
+      const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
+      PivotSet s;
+      if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
+        pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
+        stream_rows_list<kStreamWindows>(g, u, L, d, wib, WAVES, table, s, rb, lane, stage, fill,
+                                         lo);
+      } else {
+        fallback_list(g, u, L, d, wib, WAVES, lane, stage, fill, lo);
+      }
+    }
+    __syncthreads();
+  }
+  int32_t *wt = table + wib * (TABLE_WORDS / WAVES);
+  const int32_t n_heavy = min(split[0], n_small);
+  const int32_t heavy_claims = (n_heavy + kSmallBatchHeavy - 1) / kSmallBatchHeavy;
+  while (true) {
+    int32_t k = 0;
+    if (lane == 0) k = atomicAdd(&cursors[0], 1);
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
+    const int32_t base = k < heavy_claims
+                             ? k * kSmallBatchHeavy
+                             : n_heavy + (k - heavy_claims) * kSmallBatchLight;
+    if (base >= n_small) break;
+    const int32_t end = min(base + (k < heavy_claims ? kSmallBatchHeavy : kSmallBatchLight),
+                            k < heavy_claims ? n_heavy : n_small);
+    for (int32_t p = base; p < end; ++p) {
+      const int32_t u = small[p];
+      if ((uint32_t)u >= m) continue;
+      const int32_t d = g.csr.deg[u];
+      if (d < 2) continue;
+      const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
+      PivotSet s;
+      if (!pivot_plan(L[0], L[d - 1], d, TABLE_WORDS / WAVES, s)) {
+        fallback_list(g, u, L, d, 0, 1, lane, stage, fill, lo);
+        continue;
+      }
+      pivot_build<false>(wt, s, L, d, lane, kWave);
+      stream_rows_list<kStreamWindows>(g, u, L, d, 0, 1, wt, s, rb, lane, stage, fill, lo);
+      wave_sync();
+    }
+  }
+  if (fill > 0) list_flush(stage, fill, lo);
+}
+
 }  // namespace gary
 
 // ======================================================================================
@@ -902,6 +1124,42 @@ int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long 
 }
 
 }  // namespace
+
+int launch_tri_list(const gary_device_graph &g, unsigned long long *total, int32_t *out,
+                    unsigned long long cap, hipStream_t st) {
+  const int32_t m = g.csr.m;
+  if (m == 0 || g.csr.nnz == 0) return GARY_OK;
+  GARY_CHECK_ARG(((uintptr_t)g.csr.colIdx & 15) == 0,
+                 "listing reads colIdx in 16-byte chunks: colIdx must be 16-byte aligned");
+  GARY_CHECK_ARG(g.csr.nnz < (1ll << 34),
+                 "listing indexes colIdx in 32-bit chunks: nnz must be below 2^34");
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  void *ws = nullptr;
+  const size_t stride = ((size_t)m + 63) & ~(size_t)63;
+  GARY_TRY(workspace_get(dev, st, sizeof(int32_t) * (3 * stride + kPivotCounterWords), &ws));
+  int32_t *lists = (int32_t *)ws;
+  int32_t *counters = lists + 3 * stride;
+  const PivotCfg pc;
+  fill_zero_kernel<<<1, 256, 0, st>>>((uint32_t *)counters, kPivotCounterWords);
+  GARY_HIP(hipGetLastError());
+  const int bgrid = (int)std::min<int64_t>(((int64_t)m + 1023) / 1024, 4096);
+  bin_hist_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, counters,
+                                         nullptr, nullptr, 0.f, true);
+  GARY_HIP(hipGetLastError());
+  bin_scatter_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, lists,
+                                            (int64_t)stride, counters, nullptr, nullptr, 0.f, true);
+  GARY_HIP(hipGetLastError());
+  const ListOut lo{total, out, cap};
+  tri_list_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, lo, counters + kSplitOff + 2);
+  GARY_HIP(hipGetLastError());
+  tri_list_pivot_kernel<4, kPivotWords><<<2048, 4 * 64, 0, st>>>(
+      g, lists, lists + stride, counters, counters + 4, lo, counters + kSplitOff);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
 }  // namespace gary
 
 extern "C" {
