@@ -180,12 +180,17 @@ class Bench:
     def c3(self) -> dict:
         a = self.a
         self.selector.load()
+        from gary_amd.device import gen_chung_lu_device
         t0 = time.time()
-        g = self.graph.oriented(*self.graph.chung_lu_4m())
-        log(f"chung_lu_4m: m={g.nnz} host build {time.time() - t0:.2f}s")
-        ctx = self.DeviceGraphCtx(g)
+        n, nt = 4_194_304, 64 * 1_048_576  # graph.chung_lu_4m(), generated and built on the device
+        ds, dd = gen_chung_lu_device(n, nt, 2.1, 50_000.0, self.graph.SEEDS["chung_lu"])
+        ctx = self.DeviceGraphCtx()
+        ctx.buildFromEdges(n, ds, dd, nt, self.graph.ORIENT_DEGREE)
+        del ds, dd
         ctx.populateDevice()
-        leg, ref = {"m": g.nnz}, None
+        nnz = ctx.deviceGraph().csr.nnz
+        log(f"chung_lu_4m: m={nnz} device generation and build {time.time() - t0:.2f}s")
+        leg, ref = {"m": nnz}, None
         for nm in ("heuristic", "learned", "bitmap"):
             ms, t, _ = self.time_count(ctx, self.names[nm], max(1, min(a.steps, 10)), 2,
                                        self.autotune.cfg_for("c3", nm))

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "common.h"
 #include "device_util.h"
@@ -73,6 +74,20 @@ __global__ void __launch_bounds__(kBlock) gen_rmat_kernel(uint64_t seed, int sca
     gary::rmat_tuple(seed, i, scale, tA, tAB, tABC, &u, &v);
     src[i] = (int32_t)u;
     dst[i] = (int32_t)v;
+  }
+}
+
+// ---- Chung-Lu tuples ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) gen_chung_lu_kernel(uint64_t seed, int32_t n, uint64_t nt,
+                                                              const uint64_t *__restrict__ cum,
+                                                              int32_t *__restrict__ src,
+                                                              int32_t *__restrict__ dst) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < nt;
+       i += (uint64_t)gridDim.x * kBlock) {
+    uint64_t s = gary::stream_state(seed, i);
+    src[i] = gary::chung_lu_draw(cum, n, gary::next64(s));
+    dst[i] = gary::chung_lu_draw(cum, n, gary::next64(s));
   }
 }
 
@@ -632,6 +647,9 @@ int build_csr(int32_t n, int64_t nt, const int32_t *src, const int32_t *dst, int
 
 struct gary_ctx;
 namespace gary {
+// The Chung-Lu cumulative weight table (host_graph.cpp).
+void chung_lu_cum(int32_t n, int64_t ntuples, double gamma, double max_w,
+                  std::vector<uint64_t> &cum);
 // Installs device-built CSR arrays into a context (defined with the context).
 int ctx_adopt(gary_ctx *ctx, int32_t m, int64_t nnz, int64_t *indPtr, int32_t *colIdx);
 int ctx_device(gary_ctx *ctx, int *device);
@@ -652,6 +670,29 @@ int gary_gen_rmat_device(int32_t scale, int64_t ntuples, double a, double b, dou
   gen_rmat_kernel<<<grid_for((uint64_t)ntuples, 16384), kBlock, 0, (hipStream_t)stream>>>(
       seed, scale, (uint64_t)ntuples, t.tA, t.tAB, t.tABC, src, dst);
   GARY_HIP(hipGetLastError());
+  return GARY_OK;
+  GARY_API_END
+}
+
+int gary_gen_chung_lu_device(int32_t n, int64_t ntuples, double gamma, double max_w,
+                             uint64_t seed, int32_t *src, int32_t *dst, void *stream) {
+  GARY_API_BEGIN
+  GARY_CHECK_ARG(n >= 2 && ntuples >= 0 && gamma > 1.0 && max_w >= 1.0,
+                 "gary_gen_chung_lu_device: bad arguments");
+  GARY_CHECK_ARG(ntuples == 0 || (src && dst), "gary_gen_chung_lu_device: null output");
+  if (ntuples == 0) return GARY_OK;
+  std::vector<uint64_t> cum;
+  gary::chung_lu_cum(n, ntuples, gamma, max_w, cum);
+  hipStream_t s = (hipStream_t)stream;
+  int device = 0;
+  GARY_TRY(gary::current_device(&device));
+  void *ws = nullptr;  // the table, in a workspace slot of its own
+  GARY_TRY(gary::workspace_get(device, (void *)10, sizeof(uint64_t) * cum.size(), &ws));
+  GARY_HIP(hipMemcpyAsync(ws, cum.data(), sizeof(uint64_t) * cum.size(), hipMemcpyHostToDevice, s));
+  gen_chung_lu_kernel<<<grid_for((uint64_t)ntuples, 16384), kBlock, 0, s>>>(
+      seed, n, (uint64_t)ntuples, (const uint64_t *)ws, src, dst);
+  GARY_HIP(hipGetLastError());
+  GARY_HIP(hipStreamSynchronize(s));  // the host table is released on return
   return GARY_OK;
   GARY_API_END
 }

@@ -91,6 +91,44 @@ int bits_for(int64_t n) {
 
 }  // namespace
 
+namespace gary {
+
+// Integer cumulative Chung-Lu weights cum[0..n] (cum[0] = 0, every weight at least 1 unit):
+// w_i = C (i + i0)^(-1/(gamma-1)) with i0 chosen so that w_0 = max_w while the weights sum to
+// 2 * ntuples, scaled to a 2^62 total. Shared by the host and device generators.
+void chung_lu_cum(int32_t n, int64_t ntuples, double gamma, double max_w,
+                  std::vector<uint64_t> &cum) {
+  // w_i = C (i + i0)^(-beta), beta = 1/(gamma-1), with i0 chosen so that w_0 = max_w while the
+  // weights sum to 2*ntuples (bisection on i0; the sum is monotone in i0 at fixed w_0).
+  const double beta = 1.0 / (gamma - 1.0), target = 2.0 * (double)ntuples;
+  auto total_for = [&](double i0) {
+    double cst = max_w * std::pow(i0, beta), t = 0;
+    for (int32_t i = 0; i < n; ++i) t += cst * std::pow((double)i + i0, -beta);
+    return t;
+  };
+  double lo = 1e-6, hi = (double)n * 16;
+  for (int it = 0; it < 200 && hi - lo > 1e-9 * hi; ++it) {
+    double mid = 0.5 * (lo + hi);
+    if (total_for(mid) < target) lo = mid;
+    else hi = mid;
+  }
+  const double i0 = 0.5 * (lo + hi), cst = max_w * std::pow(i0, beta);
+  // Integer cumulative weights: sampling is integer-only from here on.
+  cum.assign((size_t)n + 1, 0);
+  double wsum = 0;
+  for (int32_t i = 0; i < n; ++i) wsum += cst * std::pow((double)i + i0, -beta);
+  const double scale = 4611686018427387904.0 / wsum;  // 2^62 / total
+  double run = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    run += cst * std::pow((double)i + i0, -beta);
+    cum[(size_t)i + 1] = (uint64_t)(run * scale);
+  }
+  for (int32_t i = 0; i < n; ++i)  // keep every weight >= 1 unit so every vertex is drawable
+    if (cum[(size_t)i + 1] <= cum[i]) cum[(size_t)i + 1] = cum[i] + 1;
+}
+
+}  // namespace gary
+
 extern "C" {
 
 int gary_gen_er(int32_t n, int64_t m, uint64_t seed, int32_t *src, int32_t *dst) {
@@ -154,45 +192,13 @@ int gary_gen_chung_lu(int32_t n, int64_t ntuples, double gamma, double max_w, ui
   GARY_CHECK_ARG(n >= 2 && ntuples >= 0 && gamma > 1.0 && max_w >= 1.0,
                  "gary_gen_chung_lu: bad arguments");
   GARY_CHECK_ARG(ntuples == 0 || (src && dst), "gary_gen_chung_lu: null output");
-  // w_i = C (i + i0)^(-beta), beta = 1/(gamma-1), with i0 chosen so that w_0 = max_w while the
-  // weights sum to 2*ntuples (bisection on i0; the sum is monotone in i0 at fixed w_0).
-  const double beta = 1.0 / (gamma - 1.0), target = 2.0 * (double)ntuples;
-  auto total_for = [&](double i0) {
-    double cst = max_w * std::pow(i0, beta), t = 0;
-    for (int32_t i = 0; i < n; ++i) t += cst * std::pow((double)i + i0, -beta);
-    return t;
-  };
-  double lo = 1e-6, hi = (double)n * 16;
-  for (int it = 0; it < 200 && hi - lo > 1e-9 * hi; ++it) {
-    double mid = 0.5 * (lo + hi);
-    if (total_for(mid) < target) lo = mid;
-    else hi = mid;
-  }
-  const double i0 = 0.5 * (lo + hi), cst = max_w * std::pow(i0, beta);
-  // Integer cumulative weights: sampling is integer-only from here on.
-  std::vector<uint64_t> cum((size_t)n + 1, 0);
-  double wsum = 0;
-  for (int32_t i = 0; i < n; ++i) wsum += cst * std::pow((double)i + i0, -beta);
-  const double scale = 4611686018427387904.0 / wsum;  // 2^62 / total
-  double run = 0;
-  for (int32_t i = 0; i < n; ++i) {
-    run += cst * std::pow((double)i + i0, -beta);
-    cum[(size_t)i + 1] = (uint64_t)(run * scale);
-  }
-  for (int32_t i = 0; i < n; ++i)  // keep every weight >= 1 unit so every vertex is drawable
-    if (cum[(size_t)i + 1] <= cum[i]) cum[(size_t)i + 1] = cum[i] + 1;
-  const uint64_t total = cum[(size_t)n];
-  auto draw = [&](uint64_t x) {
-    uint64_t t = (uint64_t)(((unsigned __int128)x * total) >> 64);
-    // largest i with cum[i] <= t
-    size_t i = (size_t)(std::upper_bound(cum.begin(), cum.end(), t) - cum.begin()) - 1;
-    return (int32_t)std::min<size_t>(i, (size_t)n - 1);
-  };
+  std::vector<uint64_t> cum;
+  gary::chung_lu_cum(n, ntuples, gamma, max_w, cum);
   parallel_chunks(ntuples, num_workers(), [&](int, int64_t beg, int64_t end) {
     for (int64_t i = beg; i < end; ++i) {
       uint64_t s = stream_state(seed, (uint64_t)i);
-      src[i] = draw(next64(s));
-      dst[i] = draw(next64(s));
+      src[i] = gary::chung_lu_draw(cum.data(), n, next64(s));
+      dst[i] = gary::chung_lu_draw(cum.data(), n, next64(s));
     }
   });
   return GARY_OK;

@@ -62,4 +62,23 @@ GARY_HD void rmat_tuple(uint64_t seed, uint64_t i, int scale, uint64_t tA, uint6
   *v_out = v;
 }
 
+// Chung-Lu endpoint: t = floor(x * total / 2^64) with total = cum[n], then the largest i with
+// cum[i] <= t (cum: n + 1 non-decreasing integer cumulative weights, cum[0] = 0).
+GARY_HD int32_t chung_lu_draw(const uint64_t *cum, int32_t n, uint64_t x) {
+  const uint64_t total = cum[n];
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint64_t t = __umul64hi(x, total);
+#else
+  const uint64_t t = (uint64_t)(((unsigned __int128)x * total) >> 64);
+#endif
+  int32_t lo = 0, hi = n + 1;  // first index with cum > t is in [lo, hi]
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (cum[mid] <= t) lo = mid + 1;
+    else hi = mid;
+  }
+  const int32_t i = lo - 1;
+  return i < n - 1 ? i : n - 1;
+}
+
 }  // namespace gary

@@ -26,6 +26,8 @@ _bind("gary_ctx_clear", C.c_int, [_P])
 _bind("gary_ctx_build_from_edges", C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, C.c_int, _P])
 _bind("gary_gen_rmat_device", C.c_int, [C.c_int32, C.c_int64, C.c_double, C.c_double,
                                         C.c_double, C.c_uint64, _P, _P, _P])
+_bind("gary_gen_chung_lu_device", C.c_int, [C.c_int32, C.c_int64, C.c_double, C.c_double,
+                                            C.c_uint64, _P, _P, _P])
 _bind("gary_csr_from_edges_device", C.c_int, [C.c_int32, C.c_int64, _P, _P, C.c_int, _P, _P,
                                               C.c_int64, _P, _P])
 
@@ -166,6 +168,16 @@ def gen_rmat_device(scale: int, ntuples: int, a=0.57, b=0.19, c=0.19, seed: int 
     check(lib.gary_gen_rmat_device(scale, ntuples, a, b, c, seed & (2**64 - 1), src.ptr,
                                    dst.ptr, None))
     check(lib.gary_device_synchronize(device))
+    return src, dst
+
+
+def gen_chung_lu_device(n: int, ntuples: int, gamma=2.1, max_w=50_000.0, seed: int = 1,
+                        device: int = 0) -> tuple[DeviceBuffer, DeviceBuffer]:
+    """Chung-Lu tuples generated on the device, equal to graph.gen_chung_lu's bit for bit."""
+    src = DeviceBuffer(max(4 * ntuples, 4), device)
+    dst = DeviceBuffer(max(4 * ntuples, 4), device)
+    check(lib.gary_gen_chung_lu_device(n, ntuples, gamma, max_w, seed & (2**64 - 1), src.ptr,
+                                       dst.ptr, None))
     return src, dst
 
 

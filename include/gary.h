@@ -133,6 +133,11 @@ int gary_snap_read(const char *path, int remove_loops, int32_t *src, int32_t *ds
 int gary_gen_rmat_device(int32_t scale, int64_t ntuples, double a, double b, double c,
                          uint64_t seed, int32_t *src, int32_t *dst, void *stream);
 
+/* gary_gen_chung_lu on the device: the same tuples bit for bit (the cumulative weight table is
+ * computed on the host and uploaded). Returns when the tuples are written. */
+int gary_gen_chung_lu_device(int32_t n, int64_t ntuples, double gamma, double max_w,
+                             uint64_t seed, int32_t *src, int32_t *dst, void *stream);
+
 /* gary_csr_from_edges with device arrays: src/dst (ntuples each) in, indPtr[n+1] and colIdx
  * (capacity cap) out, all device pointers; identical output for every mode. Keys are limited to
  * 2^31 - 1 (ntuples, or 2*ntuples for the symmetric modes). Returns when the CSR is complete

@@ -9,7 +9,8 @@ import pytest
 import oracle
 from gary_amd import _lib, graph
 from gary_amd import graph_query as gq
-from gary_amd.device import DeviceGraphCtx, csr_from_edges_device, gen_rmat_device
+from gary_amd.device import (DeviceGraphCtx, csr_from_edges_device, gen_chung_lu_device,
+                             gen_rmat_device)
 
 pytestmark = pytest.mark.gpu
 MODES = [0, 1, 2, 3]
@@ -71,6 +72,15 @@ def test_device_rmat_tuples_equal_host():
     for scale, nt, seed in ((5, 1000, 3), (16, 1 << 20, graph.SEEDS["rmat20"])):
         hs, hd = graph.gen_rmat(scale, nt, seed=seed)
         ds, dd = gen_rmat_device(scale, nt, seed=seed)
+        assert np.array_equal(ds.to_array(np.int32, nt), hs)
+        assert np.array_equal(dd.to_array(np.int32, nt), hd)
+
+
+def test_device_chung_lu_tuples_equal_host():
+    for n, nt, w, seed in ((5000, 40000, 500.0, 2), (4_194_304, 1 << 22, 50_000.0,
+                                                      graph.SEEDS["chung_lu"])):
+        hs, hd = graph.gen_chung_lu(n, nt, seed=seed, max_w=w)
+        ds, dd = gen_chung_lu_device(n, nt, seed=seed, max_w=w)
         assert np.array_equal(ds.to_array(np.int32, nt), hs)
         assert np.array_equal(dd.to_array(np.int32, nt), hd)
 
