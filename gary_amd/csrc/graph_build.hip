@@ -109,13 +109,10 @@ __global__ void __launch_bounds__(kBlock) rs_hist_kernel(const uint64_t *__restr
     const uint32_t i = base + (uint32_t)k * kWave + (uint32_t)lane;
     dig[k] = i < n ? (uint32_t)(keys[i] >> shift) & (kBins - 1) : 0u;
   }
+  // counts only (no ranks): one LDS atomic add per key into the wave's histogram
 #pragma unroll
-  for (int k = 0; k < kIpt; ++k) {
-    const bool valid = base + (uint32_t)k * kWave + (uint32_t)lane < n;
-    const uint64_t p = digit_peers(dig[k], valid);
-    if (valid && lanes_below(p) == 0) h[w][dig[k]] += (uint32_t)__popcll(p);
-    gary::wave_sync();
-  }
+  for (int k = 0; k < kIpt; ++k)
+    if (base + (uint32_t)k * kWave + (uint32_t)lane < n) atomicAdd(&h[w][dig[k]], 1u);
   __syncthreads();
   for (int d = threadIdx.x; d < kBins; d += kBlock) {
     uint32_t s = 0;
