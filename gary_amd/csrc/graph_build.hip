@@ -122,17 +122,24 @@ __global__ void __launch_bounds__(kBlock) rs_hist_kernel(const uint64_t *__restr
   }
 }
 
-// Stable scatter of one tile: out[offset(digit, tile) + rank within tile] = key.
+// Stable scatter of one tile: out[offset(digit, tile) + rank within tile] = key. The tile is
+// first reordered by digit in LDS, so the stores of consecutive threads go to consecutive
+// addresses of a digit's run (whole cache lines) instead of 64 lanes spread over many runs.
 __global__ void __launch_bounds__(kBlock) rs_scatter_kernel(const uint64_t *__restrict__ in,
                                                             uint64_t *__restrict__ out,
                                                             uint32_t n, int shift,
                                                             const uint32_t *__restrict__ off,
                                                             uint32_t tiles) {
   __shared__ uint32_t h[kWaves][kBins];
+  __shared__ uint32_t dstart[kBins];  // first tile-local position of each digit
+  __shared__ uint32_t gbase[kBins];   // output offset of each digit's run from this tile
+  __shared__ uint32_t wtot[kWaves];
+  __shared__ uint64_t sorted[kTile];
   for (int i = threadIdx.x; i < kWaves * kBins; i += kBlock) (&h[0][0])[i] = 0;
   __syncthreads();
   const int w = threadIdx.x / kWave, lane = lane_id();
-  const uint32_t base = blockIdx.x * (uint32_t)kTile + (uint32_t)w * kWaveSpan;
+  const uint32_t tile0 = blockIdx.x * (uint32_t)kTile;
+  const uint32_t base = tile0 + (uint32_t)w * kWaveSpan;
   uint64_t key[kIpt];
   uint32_t rank[kIpt];
 #pragma unroll
@@ -155,23 +162,38 @@ __global__ void __launch_bounds__(kBlock) rs_scatter_kernel(const uint64_t *__re
     gary::wave_sync();
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kBins; d += kBlock) {
-    uint32_t run = off[(size_t)d * tiles + blockIdx.x];
+  // digit d (one per thread: kBlock == kBins): the waves' counts become their prefixes within
+  // the digit, and a block scan of the digit totals gives the digit's first tile position
+  static_assert(kBlock == kBins, "one digit per thread");
+  const int d0 = threadIdx.x;
+  uint32_t cnt = 0;
 #pragma unroll
-    for (int q = 0; q < kWaves; ++q) {
-      const uint32_t c = h[q][d];
-      h[q][d] = run;
-      run += c;
-    }
+  for (int q = 0; q < kWaves; ++q) {
+    const uint32_t c = h[q][d0];
+    h[q][d0] = cnt;
+    cnt += c;
   }
+  const uint32_t incl = gary::wave_inclusive_scan_u32(cnt);
+  if (lane == kWave - 1) wtot[w] = incl;
+  gbase[d0] = off[(size_t)d0 * tiles + blockIdx.x];
+  __syncthreads();
+  uint32_t before = 0;
+  for (int q = 0; q < w; ++q) before += wtot[q];
+  dstart[d0] = before + incl - cnt;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kIpt; ++k) {
-    const uint32_t i = base + (uint32_t)k * kWave + (uint32_t)lane;
-    if (i < n) {
+    if (base + (uint32_t)k * kWave + (uint32_t)lane < n) {
       const uint32_t d = (uint32_t)(key[k] >> shift) & (kBins - 1);
-      out[h[w][d] + rank[k]] = key[k];
+      sorted[dstart[d] + h[w][d] + rank[k]] = key[k];
     }
+  }
+  __syncthreads();
+  const uint32_t valid = min((uint32_t)kTile, n - tile0);
+  for (uint32_t j = threadIdx.x; j < valid; j += kBlock) {
+    const uint64_t k = sorted[j];
+    const uint32_t d = (uint32_t)(k >> shift) & (kBins - 1);
+    out[gbase[d] + (j - dstart[d])] = k;
   }
 }
 
