@@ -16,6 +16,9 @@ Further legs (--legs, default all):
          per-graph uint64 counts;
   hom4   K4 / diamond / 4-cycle homomorphism counts on R-MAT scale 22 (wall clock per call:
          each call returns its counts to the host).
+  ops    the search and intersection operators on the reference's SearchTest / SetOpTest shapes
+         (a sorted list of 4 Mi ints, 32 Ki keys) and with 4 Mi keys, HIP events around repeated
+         device-resident calls, beside the oracle's sequential set intersection;
   patterns  triangle listing (every triangle written as an int32 triple) on R-MAT 20, and
          k-clique counts (k = 4, 5) on R-MAT 20 (ranked symmetric CSR built on the device);
   build  graph construction on the device (R-MAT tuples, symmetrize, dedup, orient, CSR) for
@@ -355,6 +358,77 @@ class Bench:
             log(f"patterns: {c} {k}-cliques in {ms:.1f} ms")
         return leg
 
+    def ops(self) -> dict:
+        """Search and intersection operators (include/ops/Search.cuh, Intersection.cuh) on the
+        reference's test shapes (test/SearchTest.cu:201-208, test/SetOpTest.cu:62-68): sorted
+        int32 lists generated here from a fixed seed."""
+        import oracle
+        from gary_amd import ops
+        from gary_amd._lib import check, lib
+        from gary_amd.device import DeviceBuffer
+        np = self.np
+        rng = np.random.default_rng(0x6A7E00F0)
+        lst = np.sort(rng.integers(1 << 29, 1 << 30, 4 << 20, dtype=np.int64)).astype(np.int32)
+        dl = DeviceBuffer.from_array(lst)
+        res = DeviceBuffer(4 << 20)
+        leg = {"workload": "sorted list of 4,194,304 int32 in [2^29, 2^30]; keys uniform in the "
+                           "same range (SearchTest); sets A of 4,194,304 in [2^26, 2^30] and B "
+                           "(SetOpTest)"}
+
+        def timed(f, reps):
+            f()
+            self.gq.synchronize()
+            self.timer.start()
+            for _ in range(reps):
+                f()
+            return self.timer.stop() / reps
+
+        for nk in (32 << 10, 4 << 20):
+            keys = np.sort(rng.integers(1 << 29, 1 << 30, nk, dtype=np.int64)).astype(np.int32)
+            dk = DeviceBuffer.from_array(keys)
+            want = np.isin(keys, lst)
+            for name, mode in (("plain", ops.SEARCH_PLAIN), ("two_phase", ops.SEARCH_2PHASE)):
+                check(lib.gary_search(dl.ptr, lst.size, dk.ptr, nk, res.ptr, mode, 32, None))
+                got = res.to_array(np.uint8, nk).astype(bool)
+                if not np.array_equal(got, want):
+                    raise SystemExit(f"search {name} with {nk} keys differs from numpy")
+                ms = timed(lambda: check(lib.gary_search(dl.ptr, lst.size, dk.ptr, nk, res.ptr,
+                                                         mode, 32, None)), 20)
+                leg[f"search_{name}_{nk}_us"] = ms * 1e3
+                leg[f"search_{name}_{nk}_keys_per_s"] = nk / (ms * 1e-3)
+            log(f"ops: search {nk} keys: plain {leg[f'search_plain_{nk}_us']:.1f} us, two-phase "
+                f"{leg[f'search_two_phase_{nk}_us']:.1f} us")
+        a = np.sort(rng.integers(1 << 26, 1 << 30, 4 << 20, dtype=np.int64)).astype(np.int32)
+        a = np.unique(a)
+        da = DeviceBuffer.from_array(a)
+        cnt = DeviceBuffer(8)
+        for nb in (32 << 10, 4 << 20):
+            b = np.unique(np.sort(rng.integers(1 << 26, 1 << 30, nb, dtype=np.int64)).astype(np.int32))
+            db = DeviceBuffer.from_array(b)
+            want = oracle.intersect_count(a, b)
+            c0 = time.perf_counter()
+            for _ in range(5):
+                oracle.intersect_count(a, b)
+            cpu_us = (time.perf_counter() - c0) / 5 * 1e6
+            leg[f"isect_{nb}_cpu_us"] = cpu_us
+            for name, mth in (("bsearch", ops.ISECT_BSEARCH), ("cached", ops.ISECT_CACHED_BSEARCH),
+                              ("merge", ops.ISECT_MERGE), ("bitmap", ops.ISECT_BITMAP)):
+                cnt.zero()
+                check(lib.gary_intersect_count(da.ptr, a.size, db.ptr, b.size, mth, 32, cnt.ptr,
+                                               None))
+                if int(cnt.to_array(np.uint64, 1)[0]) != want:
+                    raise SystemExit(f"intersection {name} with |B|={nb} differs from the oracle")
+                ms = timed(lambda: check(lib.gary_intersect_count(
+                    da.ptr, a.size, db.ptr, b.size, mth, 32, cnt.ptr, None)), 20)
+                bytes_alg = 4 * (a.size + b.size)
+                leg[f"isect_{nb}_{name}_us"] = ms * 1e3
+                leg[f"isect_{nb}_{name}_frac"] = bytes_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+            best = min(("bsearch", "cached", "merge", "bitmap"),
+                       key=lambda x: leg[f"isect_{nb}_{x}_us"])
+            log(f"ops: intersect |A|={a.size} |B|={b.size}: best {best} "
+                f"{leg[f'isect_{nb}_{best}_us']:.1f} us (cpu {cpu_us:.0f} us)")
+        return leg
+
     def cpu(self) -> dict:
         import oracle
         a, g = self.a, self.g
@@ -392,7 +466,7 @@ def main(argv=None) -> int:
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--strategy", default="learned")
-    ap.add_argument("--legs", default="headline,c3,batch,hom4,build,patterns")
+    ap.add_argument("--legs", default="headline,c3,batch,hom4,build,patterns,ops")
     ap.add_argument("--build-graphs", default="rmat20_degree,rmat22_ranked",
                     help="graphs of the build leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -436,6 +510,8 @@ def main(argv=None) -> int:
         out["build"] = b.build()
     if "patterns" in legs:
         out["patterns"] = b.patterns()
+    if "ops" in legs:
+        out["ops"] = b.ops()
     if a.calibrate_copy:
         b.calibrate_copy(a.calibrate_copy)
     out["cpu_baseline"] = b.cpu() if "headline" in legs and not a.no_cpu else None

@@ -127,38 +127,86 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Merge-path count over the whole grid: thread t merges diagonal segment t of merge(A, B).
-__device__ __forceinline__ int32_t mp_split(const int32_t *__restrict__ A, int32_t na,
-                                            const int32_t *__restrict__ B, int32_t nb, int64_t k) {
-  int64_t lo = k > nb ? k - nb : 0, hi = k < na ? k : na;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (A[mid] <= B[k - 1 - mid]) lo = mid + 1;
-    else hi = mid;
+// Merge-path split: the number of A elements among the first k of merge(A, B) (ties take A
+// first), i.e. the first i with A[i] > B[k - 1 - i]. One wave finds it in about log64 steps: the
+// lanes test 64 evenly spaced candidates at once, and the search narrows to the gap after the
+// last candidate that holds.
+__device__ int32_t mp_split_wave(const int32_t *__restrict__ A, int32_t na,
+                                 const int32_t *__restrict__ B, int32_t nb, int64_t k) {
+  const int lane = lane_id();
+  int64_t lo = k > nb ? k - nb : 0, hi = k < na ? k : na;  // answer in [lo, hi]
+  while (hi - lo > kWave) {
+    const int64_t step = (hi - lo + kWave - 1) / kWave;
+    const int64_t i = lo + lane * step;
+    const bool holds = i < hi && A[i] <= B[k - 1 - i];
+    const uint64_t yes = __ballot(holds);
+    if (!yes) {
+      hi = lo;
+    } else {
+      const int t = 63 - __clzll((long long)yes);  // last candidate that holds
+      const int64_t at = lo + t * step;
+      hi = min(hi, at + step);
+      lo = at + 1;
+    }
   }
-  return (int32_t)lo;
+  const int64_t i = lo + lane;
+  const bool holds = i < hi && A[i] <= B[k - 1 - i];
+  return (int32_t)(lo + __popcll(__ballot(holds)));
 }
 
+constexpr int kMergePer = 16;                   // merge-path steps per thread
+constexpr int kMergeTile = kBlock * kMergePer;  // per workgroup
+
+// Merge-path count in tiles: a workgroup takes kMergeTile consecutive merge-path steps, finds
+// where they start and end in A and B (waves 0 and 1, one split each), loads both ranges into
+// LDS with coalesced reads, and each thread merges kMergePer steps of the tile from LDS. A pair
+// is counted when B[j] is taken right after an equal A element (the element before the tile's
+// first A is kept in LDS for that test).
 __global__ void __launch_bounds__(kBlock)
     isect_merge_kernel(const int32_t *__restrict__ A, int32_t na, const int32_t *__restrict__ B,
                        int32_t nb, unsigned long long *count) {
+  __shared__ int32_t sA[kMergeTile + 1];  // sA[0] = A[i0 - 1]
+  __shared__ int32_t sB[kMergeTile];
+  __shared__ int32_t split[2];
   const int64_t L = (int64_t)na + nb;
-  const int64_t T = (int64_t)gridDim.x * kBlock;
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t per = (L + T - 1) / T;
-  const int64_t k0 = min(t * per, L), k1 = min(k0 + per, L);
+  const int w = threadIdx.x / kWave;
   uint32_t c = 0;
-  if (k0 < k1) {
-    int32_t i = mp_split(A, na, B, nb, k0);
-    int32_t j = (int32_t)(k0 - i);
-    for (int64_t k = k0; k < k1; ++k) {
-      if (i < na && (j >= nb || A[i] <= B[j])) {
-        ++i;
-      } else {
-        c += (i > 0 && A[i - 1] == B[j]);
-        ++j;
+  for (int64_t d0 = (int64_t)blockIdx.x * kMergeTile; d0 < L;
+       d0 += (int64_t)gridDim.x * kMergeTile) {
+    const int64_t d1 = min(d0 + (int64_t)kMergeTile, L);
+    if (w < 2) {
+      const int32_t s = mp_split_wave(A, na, B, nb, w == 0 ? d0 : d1);
+      if (lane_id() == 0) split[w] = s;
+    }
+    __syncthreads();
+    const int32_t i0 = split[0], i1 = split[1];
+    const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
+    const int32_t la = i1 - i0, lb = j1 - j0;
+    for (int32_t t = threadIdx.x; t < la; t += kBlock) sA[1 + t] = A[i0 + t];
+    for (int32_t t = threadIdx.x; t < lb; t += kBlock) sB[t] = B[j0 + t];
+    if (threadIdx.x == 0) sA[0] = i0 > 0 ? A[i0 - 1] : 0;
+    __syncthreads();
+    const int32_t ll = la + lb;
+    const int32_t k0 = min((int32_t)threadIdx.x * kMergePer, ll);
+    const int32_t k1 = min(k0 + kMergePer, ll);
+    if (k0 < k1) {
+      int32_t lo = max(0, k0 - lb), hi = min(k0, la);
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (sA[1 + mid] <= sB[k0 - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+      }
+      int32_t i = lo, j = k0 - lo;
+      for (int32_t k = k0; k < k1; ++k) {
+        if (i < la && (j >= lb || sA[1 + i] <= sB[j])) {
+          ++i;
+        } else {
+          c += (i0 + i > 0 && sA[i] == sB[j]);
+          ++j;
+        }
       }
     }
+    __syncthreads();  // the tile's LDS is reloaded by the next iteration
   }
   wave_atomic_add(count, c);
 }
@@ -370,8 +418,9 @@ int gary_intersect_count(const int32_t *d_a, int64_t na, const int32_t *d_b, int
   hipStream_t st = (hipStream_t)stream;
   auto *cnt = (unsigned long long *)d_count;
   if (method == GARY_ISECT_MERGE) {
-    isect_merge_kernel<<<grid_for((na + nb) / 16 + 1), kBlock, 0, st>>>(d_a, (int32_t)na, d_b,
-                                                                       (int32_t)nb, cnt);
+    const int64_t tiles = (na + nb + kMergeTile - 1) / kMergeTile;
+    isect_merge_kernel<<<(int)std::min<int64_t>(tiles, 8192), kBlock, 0, st>>>(
+        d_a, (int32_t)na, d_b, (int32_t)nb, cnt);
   } else if (method == GARY_ISECT_BITMAP) {
     isect_bitmap_kernel<4096><<<2048, kBlock, 0, st>>>(d_a, (int32_t)na, d_b, (int32_t)nb, cnt);
   } else {
