@@ -574,8 +574,8 @@ int build_csr(int32_t n, int64_t nt, const int32_t *src, const int32_t *dst, int
   const uint64_t sentinel = (vb == 0) ? 0 : ((1ull << (2 * vb)) - 1);
   const uint32_t nkeys = (uint32_t)(2 * nt);
   const uint64_t cap_keys = std::max<uint64_t>(nkeys, (uint64_t)n);
-  // Scratch: the library's workspace of (device, stream), carved into the sort's two key buffers
-  // and counters, so repeated builds allocate nothing.
+  // Scratch: one library workspace slot per (device, stream), carved into the sort's two key
+  // buffers and counters, so repeated builds allocate nothing.
   int device = 0;
   GARY_TRY(gary::current_device(&device));
   Carve plan;
@@ -588,7 +588,11 @@ int build_csr(int32_t n, int64_t nt, const int32_t *src, const int32_t *dst, int
   const size_t o_deg = plan.take<uint32_t>((size_t)n + 1), o_newid = plan.take<int32_t>((size_t)n);
   const size_t o_rk = plan.take<uint64_t>((size_t)n), o_flag = plan.take<int32_t>(1);
   char *ws = nullptr;
-  GARY_TRY(gary::workspace_get(device, (void *)s, plan.bytes, (void **)&ws));
+  // A scratch slot of its own per stream (the stream handle plus an odd offset: never a stream
+  // handle, never one of the library's small fixed keys), so a build never reallocates the
+  // workspace that the counting calls of the stream use (and that a captured graph of them
+  // points into).
+  GARY_TRY(gary::workspace_get(device, (char *)s + 4097, plan.bytes, (void **)&ws));
   uint64_t *ka = (uint64_t *)(ws + o_ka), *kb = (uint64_t *)(ws + o_kb);
   uint32_t *bh = (uint32_t *)(ws + o_bh), *part = (uint32_t *)(ws + o_part);
   uint32_t *start = (uint32_t *)(ws + o_start), *len = (uint32_t *)(ws + o_len);

@@ -7,6 +7,9 @@
  *
  * Counts are always uint64_t. Device pointers are HIP device pointers of the device the
  * context / call is bound to; `stream` is a hipStream_t passed as void* (NULL = null stream).
+ * Scratch memory is owned by the library and kept between calls: per (device, stream) for the
+ * triangle counts and the device build, per device for the synchronous calls (4-vertex counts,
+ * k-cliques, listing, selector training), so the caller serializes those on one device.
  *
  * Reference interfaces replaced (hirayaku/Gary, paths relative to the reference root):
  *   DeviceCSR / DeviceCOO / DeviceGraph ........ include/DeviceGraph.cuh:15-72
@@ -139,9 +142,11 @@ int gary_gen_chung_lu_device(int32_t n, int64_t ntuples, double gamma, double ma
                              uint64_t seed, int32_t *src, int32_t *dst, void *stream);
 
 /* gary_csr_from_edges with device arrays: src/dst (ntuples each) in, indPtr[n+1] and colIdx
- * (capacity cap) out, all device pointers; identical output for every mode. Keys are limited to
- * 2^31 - 1 (ntuples, or 2*ntuples for the symmetric modes). Returns when the CSR is complete
- * (it reads the deduplicated edge count back to the host for *nnz_out). */
+ * (capacity cap) out, all device pointers; identical output for every mode. Both arcs of every
+ * tuple are sorted, so 2*ntuples is limited to 2^31 - 1. Returns when the CSR is complete (it
+ * reads the deduplicated edge count back to the host for *nnz_out). Scratch (about 2 x 16 bytes
+ * per tuple) is a library buffer per (device, stream), kept for later builds and separate from
+ * the counting calls' one. */
 int gary_csr_from_edges_device(int32_t n, int64_t ntuples, const int32_t *src,
                                const int32_t *dst, int mode, int64_t *indPtr, int32_t *colIdx,
                                int64_t cap, int64_t *nnz_out, void *stream);
@@ -264,7 +269,9 @@ int64_t gary_index_cache_smem(int32_t block, int64_t cache_size);
 
 /* Triangle count of an acyclically oriented graph: Σ over arcs (u,v) of |N+(u) ∩ N+(v)|.
  * Accumulates into the device counter *d_count (the caller zeroes it), on `stream`. cfg may be
- * NULL. The graph's COO (populate) is required by the edge-centric strategies. */
+ * NULL. The graph's COO (populate) is required by the edge-centric strategies. Scratch comes
+ * from a library buffer per (device, stream) that grows for larger graphs: a hipGraph captured
+ * from this call points into it, so re-capture after a call on that stream with a larger graph. */
 int gary_tc_count(const gary_device_graph *g, int strategy, const gary_launch_cfg *cfg,
                   uint64_t *d_count, void *stream);
 /* Synchronous convenience: returns the count in *h_count. */
