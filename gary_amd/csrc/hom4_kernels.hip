@@ -519,14 +519,18 @@ __device__ __forceinline__ int32_t lower_bound(const int32_t *__restrict__ N, in
   return l;
 }
 
-// One workgroup per vertex u: counts wedges u-v-w (v, w < u) into this block's dense counter
-// slice (m entries, all zero between vertices), listing each endpoint w the first time its
-// counter leaves zero; then reads and clears the listed counters, adding C(c, 2). The second
-// pass touches the distinct endpoints only (a few percent of the wedges at hub vertices).
+// One workgroup per light vertex u (2 <= lower degree < kC4HeavyLo, from the compacted list
+// `light`): counts wedges u-v-w (v, w < u) into this block's dense counter slice (m entries, all
+// zero between vertices), listing each endpoint w the first time its counter leaves zero; then
+// reads and clears the listed counters, adding C(c, 2). The second pass touches the distinct
+// endpoints only (a few percent of the wedges at hub vertices). (Claiming every vertex id from
+// the cursor instead, most of them with nothing to count, cost a global atomic and two
+// barriers per id: 105 ms of the C4 count on R-MAT 22.)
 __global__ void __launch_bounds__(kHomBlock)
     c4_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
+              const int32_t *__restrict__ light, const int32_t *__restrict__ nlight,
               uint32_t *__restrict__ counters, int32_t *__restrict__ touched_all,
-              int32_t *__restrict__ cursor, unsigned long long *c4_acc, int32_t heavy_lo) {
+              int32_t *__restrict__ cursor, unsigned long long *c4_acc) {
   __shared__ int32_t s_item, s_nt;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t m = g.csr.m;
@@ -534,17 +538,19 @@ __global__ void __launch_bounds__(kHomBlock)
   uint32_t *cnt = counters + (int64_t)blockIdx.x * m;
   int32_t *touched = touched_all + (int64_t)blockIdx.x * m;
   uint64_t acc = 0;
+  const int32_t n_items = *nlight;
   while (true) {
     if (threadIdx.x == 0) {
       s_item = atomicAdd(cursor, 1);
       s_nt = 0;
     }
     __syncthreads();
-    const int32_t u = s_item;
-    if (u >= m) break;
+    const int32_t item = s_item;
+    if (item >= n_items) break;
+    const int32_t u = light[item];
     const int32_t nl = lo_len[u];
     const int32_t *Nu = col + g.csr.indPtr[u];
-    if (nl >= 2 && nl < heavy_lo) {  // vertices with heavy_lo or more lower neighbours: c4_window_kernel
+    {
       for (int32_t k = wib; k < nl; k += kHomWaves) {
         const int32_t v = Nu[k];
         const int32_t *Nv = col + g.csr.indPtr[v];
@@ -601,16 +607,17 @@ constexpr int kC4Waves = kC4Block / kWave;
 constexpr int kC4Touch = 2048;  // endpoints listed per window; a window with more is swept whole
 constexpr int kC4Steps = 4;     // 64-endpoint steps whose loads are issued together
 
-// heavy[0 .. *nheavy) = the vertices with at least heavy_lo lower neighbours (any order);
-// *max_lo = their largest lower-neighbour count.
-__global__ void heavy_list_kernel(const int32_t *__restrict__ lo_len, int32_t m, int32_t heavy_lo,
-                                  int32_t *__restrict__ heavy, int32_t *__restrict__ nheavy,
-                                  int32_t *__restrict__ max_lo) {
+// heavy[0 .. *nheavy) = the vertices whose lower-neighbour count lies in [lo_min, lo_max), in
+// any order; *max_lo (when given) = their largest lower-neighbour count.
+__global__ void heavy_list_kernel(const int32_t *__restrict__ lo_len, int32_t m, int32_t lo_min,
+                                  int32_t lo_max, int32_t *__restrict__ heavy,
+                                  int32_t *__restrict__ nheavy, int32_t *__restrict__ max_lo) {
   const int lane = lane_id();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < m;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t u = base + lane;
-    const bool h = u < m && lo_len[u] >= heavy_lo;
+    const int32_t l = u < m ? lo_len[u] : 0;
+    const bool h = u < m && l >= lo_min && l < lo_max;
     const uint64_t mask = __ballot(h);
     if (!mask) continue;
     int32_t pos = 0;
@@ -618,7 +625,7 @@ __global__ void heavy_list_kernel(const int32_t *__restrict__ lo_len, int32_t m,
     pos = __shfl(pos, 0, kWave);
     if (h) {
       heavy[pos + lanes_below(mask)] = (int32_t)u;
-      atomicMax(max_lo, lo_len[u]);
+      if (max_lo) atomicMax(max_lo, l);
     }
   }
 }
@@ -795,7 +802,7 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   const size_t b_rows = need_local ? al(sizeof(uint32_t) * (size_t)scratch_words * grid_local) : 0;
   const size_t b_cnt = need_c4 ? al(sizeof(uint32_t) * (size_t)m * grid_c4) : 0;
   const size_t b_touched = need_c4 ? al(sizeof(int32_t) * (size_t)m * grid_c4) : 0;
-  const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * (size_t)m) : 0;
+  const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * 2 * (size_t)m) : 0;  // heavy, light lists
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)3,
                          b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_cnt +
@@ -810,8 +817,9 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   uint32_t *cnts = (uint32_t *)p; p += b_cnt;
   int32_t *touched = (int32_t *)p; p += b_touched;
   int32_t *heavy = (int32_t *)p;
+  int32_t *light = heavy + m;
   // [0] bad, [1] unsupported, [2..3] cursors, [4] #heavy, [5] max lower degree of a heavy
-  // vertex, [6] window-item cursor
+  // vertex, [6] window-item cursor, [7] #light
   int32_t *bad = (int32_t *)misc;
   HomTotals *tot = (HomTotals *)(misc + 64);    // tri, k4
   unsigned long long *sums = (unsigned long long *)(misc + 128);  // t2 lo/hi, c2 lo/hi, c4 lo/hi
@@ -829,11 +837,14 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   hom_sums_kernel<<<1024, 256, 0, st>>>(*g, up_start, up_len, tcount, sums, sums + 2);
   GARY_HIP(hipGetLastError());
   if (need_c4) {
-    heavy_list_kernel<<<std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
-        lo_len, m, kC4HeavyLo, heavy, bad + 4, bad + 5);
+    const int lgrid = (int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096);
+    heavy_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, kC4HeavyLo, INT32_MAX, heavy, bad + 4,
+                                             bad + 5);
     GARY_HIP(hipGetLastError());
-    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, cnts, touched, bad + 3, sums + 4,
-                                              kC4HeavyLo);
+    heavy_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, 2, kC4HeavyLo, light, bad + 7, nullptr);
+    GARY_HIP(hipGetLastError());
+    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, light, bad + 7, cnts, touched, bad + 3,
+                                              sums + 4);
     GARY_HIP(hipGetLastError());
     int32_t hv[2];  // #heavy, max lower degree: sizes the per-workgroup cursor slices
     GARY_HIP(hipMemcpyAsync(hv, bad + 4, sizeof hv, hipMemcpyDeviceToHost, st));
