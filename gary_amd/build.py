@@ -87,6 +87,21 @@ def build_cpp_test() -> Path:
     return CPP_TEST_BIN
 
 
+CONVERT_SRC = PKG / "cli" / "convert.cpp"
+CONVERT_BIN = PKG / "gary_convert"
+
+
+def build_convert() -> Path:
+    """The graph conversion tool (host code, g++) linked against libgary.so."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(CONVERT_SRC),
+           f"-L{PKG}", "-lgary", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib",
+           "-o", str(CONVERT_BIN)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gary_convert build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return CONVERT_BIN
+
+
 def build_oracle() -> Path:
     r = subprocess.run(["make", "-B", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
@@ -100,6 +115,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-oracle", action="store_true")
     a = ap.parse_args(argv)
     print(build_lib(a.jobs))
+    print(build_convert())
     if not a.no_oracle:
         print(build_oracle())
         print(build_cpp_test())

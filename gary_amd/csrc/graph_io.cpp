@@ -253,4 +253,27 @@ int gary_snap_read(const char *path, int remove_loops, int32_t *src, int32_t *ds
   GARY_API_END
 }
 
+// GraphCOO::toCSR_ (src/Graph.cpp:63-91): rows by source id (counting sort), each row's columns
+// sorted ascending; duplicates and loops stay. Equal (row, col) pairs are indistinguishable, so
+// the result is the same whatever the order of the input.
+int gary_coo_to_csr(int32_t m, int64_t nnz, const int32_t *row, const int32_t *col,
+                    int64_t *indPtr, int32_t *colIdx) {
+  GARY_API_BEGIN
+  GARY_CHECK_ARG(m >= 0 && nnz >= 0 && indPtr && (nnz == 0 || (row && col && colIdx)),
+                 "gary_coo_to_csr: bad arguments");
+  std::vector<int64_t> next((size_t)m + 1, 0);
+  for (int64_t e = 0; e < nnz; ++e) {
+    if (row[e] < 0 || row[e] >= m || col[e] < 0 || col[e] >= m)
+      return gary::set_error(GARY_ERR_INVALID, "gary_coo_to_csr: edge %lld (%d, %d) out of [0, %d)",
+                             (long long)e, row[e], col[e], m);
+    ++next[(size_t)row[e] + 1];
+  }
+  for (int32_t v = 0; v < m; ++v) next[(size_t)v + 1] += next[v];
+  std::copy(next.begin(), next.end(), indPtr);
+  for (int64_t e = 0; e < nnz; ++e) colIdx[next[row[e]]++] = col[e];
+  for (int32_t v = 0; v < m; ++v) std::sort(colIdx + indPtr[v], colIdx + indPtr[v + 1]);
+  return GARY_OK;
+  GARY_API_END
+}
+
 }  // extern "C"

@@ -139,6 +139,8 @@ _lib.bind("gary_graph_file_write_coo", C.c_int, [C.c_char_p, C.c_int32, C.c_int6
                                                  C.c_void_p])
 _lib.bind("gary_snap_read", C.c_int, [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64,
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int32)])
+_lib.bind("gary_coo_to_csr", C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p])
 
 FILE_CSR, FILE_COO = 0, 1
 
@@ -193,3 +195,15 @@ def load_from_snap(path, remove_loops: bool = True) -> tuple[int, np.ndarray, np
     check(lib.gary_snap_read(_path(path), int(remove_loops), ptr(src), ptr(dst), nt.value,
                              C.byref(nt), C.byref(n)))
     return n.value, src[:nt.value].copy(), dst[:nt.value].copy()
+
+
+def coo_to_csr(m: int, row, col) -> GraphCSR:
+    """GraphCOO::toCSR_ (src/Graph.cpp:63-91): rows by source id, each sorted ascending,
+    duplicates and loops kept (no dedup, unlike csr_from_edges)."""
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    nnz = int(row.shape[0])
+    indPtr = np.empty(int(m) + 1, np.int64)
+    colIdx = np.empty(max(nnz, 1), np.int32)
+    check(lib.gary_coo_to_csr(int(m), nnz, ptr(row), ptr(col), ptr(indPtr), ptr(colIdx)))
+    return GraphCSR(indPtr, colIdx[:nnz].copy())

@@ -127,6 +127,12 @@ int gary_graph_file_write_coo(const char *path, int32_t m, int64_t nnz, const in
  * least 1), then with arrays of cap >= *ntuples entries. */
 int gary_snap_read(const char *path, int remove_loops, int32_t *src, int32_t *dst, int64_t cap,
                    int64_t *ntuples, int32_t *n);
+/* GraphCOO::toCSR_ (src/Graph.cpp:63-91) on the host: indPtr[m+1] and colIdx[nnz] with rows by
+ * source id and each row sorted ascending; duplicates and loops are kept (no dedup, unlike
+ * gary_csr_from_edges). Ids outside [0, m) fail with GARY_ERR_INVALID. The reference's `convert`
+ * tool (convert.cpp:13-78) is `gary_amd/gary_convert`, built on these file functions. */
+int gary_coo_to_csr(int32_t m, int64_t nnz, const int32_t *row, const int32_t *col,
+                    int64_t *indPtr, int32_t *colIdx);
 
 /* ---- device graph construction (the same builds on the GPU; the reference has host code only:
  *      src/Graph.cpp:19-91, common/ParRadixSort.h:218-306, include/Graph.h:242-248) -------- */
