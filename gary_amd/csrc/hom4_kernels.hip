@@ -30,12 +30,37 @@
 namespace gary {
 namespace {
 
-constexpr int kHomWaves = 4;
+constexpr int kHomWaves = 4;  // c4_kernel
 constexpr int kHomBlock = kHomWaves * 64;
-constexpr int kSetWords = 4096;    // LDS words for the position-returning set of N+(u)
-constexpr int kRowLdsWords = 4095; // LDS words for the local bitset graph (the kernel's LDS is
-                                   // then exactly 40 KiB: four workgroups per CU)
-constexpr int kMaxHomDeg = 2048;   // largest out-degree handled (LDS column counters)
+constexpr int kSetWords = 4096;   // LDS words for the position-returning set of N+(u)
+constexpr int kMaxHomDeg = 2048;  // largest out-degree handled
+// Pivot passes (local graphs of N+(u)) run twice, on two lists of pivots split by out-degree:
+// pivots whose local graph fits a 40 KiB LDS pool take four 512-thread workgroups per CU; the
+// rest take one 1,024-thread workgroup per CU holding the whole 160 KiB (rows up to d+ = 1,053
+// stay in LDS; beyond, they go to a per-workgroup slice of global scratch, where every hit is a
+// global atomic). Pool sizes are in 32-bit words and leave room for the other LDS arrays of
+// each kernel.
+constexpr int kSmallWaves = 8;
+constexpr int kBigWaves = 16;
+constexpr int kLdsWords = 40960;  // 160 KiB per CU
+constexpr int kHomSmallPool = kLdsWords / 4 - 1;                         // + s_item
+constexpr int kHomBigPool = kLdsWords - 1;                               // + s_item
+constexpr int kKcSmallPool = kLdsWords / 4 - kSmallWaves * 64 - 1;      // + stage, s_item
+constexpr int kKcBigPool = kLdsWords - kBigWaves * 64 - 1;              // + stage, s_item
+constexpr int kStreamUnroll = 4;  // N+(v) loads a lane issues before resolving them
+
+// LDS words beyond the set for a pivot of out-degree d: bitset rows (2 * ceil(d / 64) words
+// each) and, for the 4-vertex pass, d column counters.
+__host__ __device__ constexpr int64_t local_words(int32_t d, bool counters) {
+  return (int64_t)d * (((d + 63) >> 6) * 2) + (counters ? d : 0);
+}
+
+// Smallest out-degree whose local graph does not fit `pool` (the first pivot of the big list).
+constexpr int32_t first_big(int32_t pool, bool counters) {
+  int32_t d = 1;
+  while (d <= kMaxHomDeg && kSetWords + local_words(d, counters) <= pool) ++d;
+  return d;
+}
 
 __device__ __forceinline__ uint32_t hslot(int32_t w, int32_t log_t) {
   return ((uint32_t)w * 0x9E3779B1u) >> (32 - log_t);
@@ -198,33 +223,65 @@ __global__ void hom_split_kernel(gary_device_graph g, int64_t *__restrict__ up_s
   if (lane_id() == 0 && s) atomicAdd(asym, (unsigned long long)s);
 }
 
+// list[0 .. *count) = the vertices u with lo <= val[u] < hi, in any order (per-wave runs
+// ascend); *max_val (when given) = the largest such val[u]. Builds the pivot lists of the local
+// graph passes (val = out-degree) and the vertex classes of the 4-cycle pass (val = lower
+// degree).
+__global__ void range_list_kernel(const int32_t *__restrict__ val, int32_t m, int32_t lo,
+                                  int32_t hi, int32_t *__restrict__ list,
+                                  int32_t *__restrict__ count, int32_t *__restrict__ max_val) {
+  const int lane = lane_id();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < m;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = base + lane;
+    const int32_t x = u < m ? val[u] : 0;
+    const bool in = u < m && x >= lo && x < hi;
+    const uint64_t mask = __ballot(in);
+    if (!mask) continue;
+    int32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(count, __popcll(mask));
+    pos = __shfl(pos, 0, kWave);
+    if (in) {
+      list[pos + lanes_below(mask)] = (int32_t)u;
+      if (max_val) atomicMax(max_val, x);
+    }
+  }
+}
+
 // ---- pivot pass: triangles, K4, per-edge triangle counts ---------------------------------------
 
 struct HomTotals {
   unsigned long long tri, k4;
 };
 
-// One workgroup per pivot (atomic cursor over all vertices). rows: local bitset graph, in LDS
-// when it fits, else in this block's slice of `row_scratch` (global memory).
-__global__ void __launch_bounds__(kHomBlock)
+// One workgroup per pivot, claimed from the list `items` (count *nitems; the big list from its
+// end, where the highest-ranked and densest pivots are). LDS pool: the position set, then d
+// column counters, then the local bitset rows when they fit, else this block's slice of
+// `row_scratch` (global memory).
+template <int WAVES, int POOL, bool BIG>
+__global__ void __launch_bounds__(WAVES * kWave)
     hom_local_kernel(gary_device_graph g, const int64_t *__restrict__ up_start,
-                     const int32_t *__restrict__ up_len, uint32_t *__restrict__ tcount,
+                     const int32_t *__restrict__ up_len, const int32_t *__restrict__ items,
+                     const int32_t *__restrict__ nitems, uint32_t *__restrict__ tcount,
                      int32_t *__restrict__ cursor, uint32_t *__restrict__ row_scratch,
                      int64_t scratch_words, HomTotals *tot, int32_t *__restrict__ unsupported) {
-  __shared__ int32_t set[kSetWords];
-  __shared__ uint32_t rows_lds[kRowLdsWords];
-  __shared__ int32_t colcnt[kMaxHomDeg];
+  constexpr int kBlock = WAVES * kWave;
+  __shared__ int32_t pool[POOL];
   __shared__ int32_t s_item;
+  int32_t *const set = pool;
+  int32_t *const colcnt = pool + kSetWords;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t m = g.csr.m;
   const int32_t *__restrict__ col = g.csr.colIdx;
+  const int32_t n_items = *nitems;
   uint64_t tri = 0, k4 = 0;
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
     __syncthreads();
-    const int32_t u = s_item;
-    if (u >= m) break;
-    const int32_t d = up_len[u];
+    const int32_t it = s_item;
+    if (it >= n_items) break;
+    const int32_t u = items[BIG ? n_items - 1 - it : it];
+    const int32_t d = ((uint32_t)u < (uint32_t)m) ? up_len[u] : 0;
     if (d < 2) {
       __syncthreads();
       continue;
@@ -232,51 +289,62 @@ __global__ void __launch_bounds__(kHomBlock)
     const int64_t ubase = up_start[u];
     const int32_t *L = col + ubase;
     PosSet ps;
-    const int32_t W = (d + 63) >> 6;  // u64 words per local row
-    const int64_t row_words = (int64_t)d * W * 2;
-    if (d > kMaxHomDeg || !posset_plan(L[0], L[d - 1], d, kSetWords, ps) ||
-        (row_words > kRowLdsWords && row_words > scratch_words)) {
+    const int32_t W2 = ((d + 63) >> 6) * 2;  // 32-bit words per local row (<= 64)
+    const int64_t row_words = (int64_t)d * W2;
+    const int64_t lds_rows = (int64_t)POOL - kSetWords - d;  // LDS words left for the rows
+    if (d > kMaxHomDeg || lds_rows < 0 || !posset_plan(L[0], L[d - 1], d, kSetWords, ps) ||
+        (row_words > lds_rows && row_words > scratch_words)) {
       if (threadIdx.x == 0) *unsupported = d;
       __syncthreads();
       continue;
     }
-    uint32_t *rows = row_words <= kRowLdsWords ? rows_lds
-                                               : row_scratch + (int64_t)blockIdx.x * scratch_words;
-    for (int64_t i = threadIdx.x; i < row_words; i += kHomBlock) rows[i] = 0;
-    for (int32_t i = threadIdx.x; i < d; i += kHomBlock) colcnt[i] = 0;
+    uint32_t *rows = row_words <= lds_rows ? (uint32_t *)(pool + kSetWords + d)
+                                           : row_scratch + (int64_t)blockIdx.x * scratch_words;
+    for (int64_t i = threadIdx.x; i < row_words; i += kBlock) rows[i] = 0;
+    for (int32_t i = threadIdx.x; i < d; i += kBlock) colcnt[i] = 0;
     posset_build(set, ps, L, d);  // begins and ends with a workgroup barrier
-    // stream the rows: wave wib takes local rows i = wib, wib + 4, ...; lanes stride the row
-    for (int32_t i = wib; i < d; i += kHomWaves) {
+    // stream the rows: wave wib takes local rows i = wib, wib + WAVES, ...; a lane issues
+    // kStreamUnroll loads of N+(v) before looking them up
+    for (int32_t i = wib; i < d; i += WAVES) {
       const int32_t v = L[i];
       if ((uint32_t)v >= (uint32_t)m) continue;
       const int64_t vb = up_start[v];
       const int32_t dv = up_len[v];
-      for (int32_t p = lane; p < dv; p += kWave) {
-        const int32_t j = posset_find(set, ps, col[vb + p]);
-        if (j >= 0) {
-          atomicOr(&rows[((int64_t)i * W + (j >> 6)) * 2 + ((j >> 5) & 1)], 1u << (j & 31));
-          atomicAdd(&colcnt[j], 1);
-          atomicAdd(&tcount[vb + p], 1u);  // t(v, w) from pivot u
+      for (int32_t p0 = lane; p0 < dv; p0 += kStreamUnroll * kWave) {
+        int32_t w[kStreamUnroll];
+#pragma unroll
+        for (int k = 0; k < kStreamUnroll; ++k) {
+          const int32_t p = p0 + k * kWave;
+          w[k] = p < dv ? col[vb + p] : -1;  // -1 lies outside every set
+        }
+#pragma unroll
+        for (int k = 0; k < kStreamUnroll; ++k) {
+          const int32_t j = posset_find(set, ps, w[k]);
+          if (j >= 0) {
+            atomicOr(&rows[(int64_t)i * W2 + (j >> 5)], 1u << (j & 31));
+            atomicAdd(&colcnt[j], 1);
+            atomicAdd(&tcount[vb + p0 + k * kWave], 1u);  // t(v, w) from pivot u
+          }
         }
       }
     }
     __syncthreads();
     // row counts (t(u, v_i)) + column counts (t(u, v_j)); triangles of u = Σ row counts
-    for (int32_t i = threadIdx.x; i < d; i += kHomBlock) {
+    for (int32_t i = threadIdx.x; i < d; i += kBlock) {
       uint32_t rc = 0;
-      for (int32_t w = 0; w < 2 * W; ++w) rc += (uint32_t)__popc(rows[(int64_t)i * W * 2 + w]);
+      for (int32_t w = 0; w < W2; ++w) rc += (uint32_t)__popc(rows[(int64_t)i * W2 + w]);
       tri += rc;
       const uint32_t add = rc + (uint32_t)colcnt[i];
       if (add) atomicAdd(&tcount[ubase + i], add);
     }
     // K4: for every local arc (i, j): popc(row_i & row_j). A wave takes row i; lane l holds its
-    // 32-bit word l (2W <= 64 words), and the row's set bits are enumerated 64 at a time: arc a
-    // belongs to the last word whose exclusive bit prefix is <= a (a search over lanes by
-    // shuffles), and is the (a - prefix)-th set bit of that word. Row j only has bits above j,
-    // so the AND starts at the word holding bit j + 1.
-    for (int32_t i = wib; i < d; i += kHomWaves) {
-      const uint32_t *ri = rows + (int64_t)i * W * 2;
-      const uint32_t word = lane < 2 * W ? ri[lane] : 0u;
+    // 32-bit word l, and the row's set bits are enumerated 64 at a time: arc a belongs to the
+    // last word whose exclusive bit prefix is <= a (a search over lanes by shuffles), and is the
+    // (a - prefix)-th set bit of that word. Row j only has bits above j, so the AND starts at the
+    // word holding bit j + 1.
+    for (int32_t i = wib; i < d; i += WAVES) {
+      const uint32_t *ri = rows + (int64_t)i * W2;
+      const uint32_t word = lane < W2 ? ri[lane] : 0u;
       const int32_t nb = __popc(word);
       const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nb);
       const int32_t excl = incl - nb;
@@ -301,9 +369,9 @@ __global__ void __launch_bounds__(kHomBlock)
           }
         }
         const int32_t j = o * 32 + pos;
-        const uint32_t *rj = rows + (int64_t)j * W * 2;
+        const uint32_t *rj = rows + (int64_t)j * W2;
         uint32_t c = 0;
-        for (int32_t w = (j + 1) >> 5; w < 2 * W; ++w) c += (uint32_t)__popc(ri[w] & rj[w]);
+        for (int32_t w = (j + 1) >> 5; w < W2; ++w) c += (uint32_t)__popc(ri[w] & rj[w]);
         k4 += c;
       }
     }
@@ -434,59 +502,72 @@ struct U128Acc {
   unsigned long long lo, hi;
 };
 
-// One workgroup per pivot (atomic cursor over all vertices), the local graph built as in
-// hom_local_kernel. K = clique size, 3 <= K <= 10.
-template <int K>
-__global__ void __launch_bounds__(kHomBlock)
+// One workgroup per pivot from a pivot list, the local graph built as in hom_local_kernel (LDS
+// pool: the position set, then the rows when they fit). K = clique size, 3 <= K <= 10.
+template <int K, int WAVES, int POOL, bool BIG>
+__global__ void __launch_bounds__(WAVES * kWave)
     kclique_kernel(gary_device_graph g, const int64_t *__restrict__ up_start,
-                   const int32_t *__restrict__ up_len, int32_t *__restrict__ cursor,
+                   const int32_t *__restrict__ up_len, const int32_t *__restrict__ items,
+                   const int32_t *__restrict__ nitems, int32_t *__restrict__ cursor,
                    uint32_t *__restrict__ row_scratch, int64_t scratch_words, U128Acc *total,
                    int32_t *__restrict__ unsupported) {
-  __shared__ int32_t set[kSetWords];
-  __shared__ uint32_t rows_lds[kRowLdsWords];
-  __shared__ uint32_t stage[kHomWaves][kWave];
+  constexpr int kBlock = WAVES * kWave;
+  __shared__ int32_t pool[POOL];
+  __shared__ uint32_t stage[WAVES][kWave];
   __shared__ int32_t s_item;
+  int32_t *const set = pool;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t m = g.csr.m;
   const int32_t *__restrict__ col = g.csr.colIdx;
+  const int32_t n_items = *nitems;
   uint64_t acc = 0;
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
     __syncthreads();
-    const int32_t u = s_item;
-    if (u >= m) break;
-    const int32_t d = up_len[u];
+    const int32_t it = s_item;
+    if (it >= n_items) break;
+    const int32_t u = items[BIG ? n_items - 1 - it : it];
+    const int32_t d = ((uint32_t)u < (uint32_t)m) ? up_len[u] : 0;
     if (d < K - 1) {  // fewer than K - 1 out-neighbours: no K-clique has u as its lowest vertex
       __syncthreads();
       continue;
     }
     const int32_t *L = col + up_start[u];
     PosSet ps;
-    const int32_t W = (d + 63) >> 6;
-    const int32_t w2 = 2 * W;  // 32-bit words per local row
+    const int32_t w2 = ((d + 63) >> 6) * 2;  // 32-bit words per local row (<= 64)
     const int64_t row_words = (int64_t)d * w2;
+    const int64_t lds_rows = (int64_t)POOL - kSetWords;
     if (d > kMaxHomDeg || !posset_plan(L[0], L[d - 1], d, kSetWords, ps) ||
-        (row_words > kRowLdsWords && row_words > scratch_words)) {
+        (row_words > lds_rows && row_words > scratch_words)) {
       if (threadIdx.x == 0) *unsupported = d;
       __syncthreads();
       continue;
     }
-    uint32_t *rows = row_words <= kRowLdsWords ? rows_lds
-                                               : row_scratch + (int64_t)blockIdx.x * scratch_words;
-    for (int64_t i = threadIdx.x; i < row_words; i += kHomBlock) rows[i] = 0;
+    uint32_t *rows = row_words <= lds_rows ? (uint32_t *)(pool + kSetWords)
+                                           : row_scratch + (int64_t)blockIdx.x * scratch_words;
+    for (int64_t i = threadIdx.x; i < row_words; i += kBlock) rows[i] = 0;
     posset_build(set, ps, L, d);  // begins and ends with a workgroup barrier
-    for (int32_t i = wib; i < d; i += kHomWaves) {
+    for (int32_t i = wib; i < d; i += WAVES) {
       const int32_t v = L[i];
       if ((uint32_t)v >= (uint32_t)m) continue;
       const int64_t vb = up_start[v];
       const int32_t dv = up_len[v];
-      for (int32_t p = lane; p < dv; p += kWave) {
-        const int32_t j = posset_find(set, ps, col[vb + p]);
-        if (j >= 0) atomicOr(&rows[(int64_t)i * w2 + (j >> 5)], 1u << (j & 31));
+      for (int32_t p0 = lane; p0 < dv; p0 += kStreamUnroll * kWave) {
+        int32_t w[kStreamUnroll];
+#pragma unroll
+        for (int k = 0; k < kStreamUnroll; ++k) {
+          const int32_t p = p0 + k * kWave;
+          w[k] = p < dv ? col[vb + p] : -1;  // -1 lies outside every set
+        }
+#pragma unroll
+        for (int k = 0; k < kStreamUnroll; ++k) {
+          const int32_t j = posset_find(set, ps, w[k]);
+          if (j >= 0) atomicOr(&rows[(int64_t)i * w2 + (j >> 5)], 1u << (j & 31));
+        }
       }
     }
     __syncthreads();
-    for (int32_t i = wib; i < d; i += kHomWaves) {
+    for (int32_t i = wib; i < d; i += WAVES) {
       const uint32_t S = lane < w2 ? rows[(int64_t)i * w2 + lane] : 0u;
       if constexpr (K == 3) {
         acc += (uint32_t)__popc(S);
@@ -606,29 +687,6 @@ constexpr int kC4Block = 1024;
 constexpr int kC4Waves = kC4Block / kWave;
 constexpr int kC4Touch = 2048;  // endpoints listed per window; a window with more is swept whole
 constexpr int kC4Steps = 4;     // 64-endpoint steps whose loads are issued together
-
-// heavy[0 .. *nheavy) = the vertices whose lower-neighbour count lies in [lo_min, lo_max), in
-// any order; *max_lo (when given) = their largest lower-neighbour count.
-__global__ void heavy_list_kernel(const int32_t *__restrict__ lo_len, int32_t m, int32_t lo_min,
-                                  int32_t lo_max, int32_t *__restrict__ heavy,
-                                  int32_t *__restrict__ nheavy, int32_t *__restrict__ max_lo) {
-  const int lane = lane_id();
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < m;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t u = base + lane;
-    const int32_t l = u < m ? lo_len[u] : 0;
-    const bool h = u < m && l >= lo_min && l < lo_max;
-    const uint64_t mask = __ballot(h);
-    if (!mask) continue;
-    int32_t pos = 0;
-    if (lane == 0) pos = atomicAdd(nheavy, __popcll(mask));
-    pos = __shfl(pos, 0, kWave);
-    if (h) {
-      heavy[pos + lanes_below(mask)] = (int32_t)u;
-      if (max_lo) atomicMax(max_lo, l);
-    }
-  }
-}
 
 __global__ void __launch_bounds__(kC4Block)
     c4_window_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
@@ -783,13 +841,15 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   GARY_TRY(current_device(&dev));
   int ncu = 0;
   GARY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  // hom_local_kernel: 40 KiB of LDS per workgroup, so four per CU. c4_kernel: one per CU; its
-  // wedge atomics saturate the memory system there (half as many: 2.8 s on R-MAT 22, twice as
-  // many: 2.1 s, against 2.06 s).
-  const int grid_local = std::max(1, ncu * 4);
+  // hom_local_kernel: four small-pivot workgroups per CU (40 KiB of LDS each), one big-pivot
+  // workgroup per CU (160 KiB). c4_kernel: one per CU; its wedge atomics saturate the memory
+  // system there (half as many: 2.8 s on R-MAT 22, twice as many: 2.1 s, against 2.06 s).
+  const int grid_small = std::max(1, ncu * 4);
+  const int grid_big = std::max(1, ncu);
   const int grid_c4 = std::max(1, ncu);
   const int grid_win = std::max(1, ncu);  // 140 KiB of LDS each: one per CU
-  const int64_t scratch_words = (int64_t)kMaxHomDeg * (kMaxHomDeg / 64) * 2;
+  const int lgrid = (int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096);
+  const int64_t scratch_words = local_words(kMaxHomDeg, false);
   // workspace layout (bytes, 256-aligned pieces)
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t b_up_start = al(sizeof(int64_t) * (size_t)m);
@@ -799,14 +859,15 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   const size_t b_misc = 256;  // bad flag, cursors, totals, 128-bit sums
   const bool need_local = patterns & (GARY_HOM_K4 | GARY_HOM_DIAMOND);
   const bool need_c4 = patterns & GARY_HOM_C4;
-  const size_t b_rows = need_local ? al(sizeof(uint32_t) * (size_t)scratch_words * grid_local) : 0;
+  const size_t b_rows = need_local ? al(sizeof(uint32_t) * (size_t)scratch_words * grid_big) : 0;
+  const size_t b_piv = need_local ? al(sizeof(int32_t) * 2 * (size_t)m) : 0;  // pivot lists
   const size_t b_cnt = need_c4 ? al(sizeof(uint32_t) * (size_t)m * grid_c4) : 0;
   const size_t b_touched = need_c4 ? al(sizeof(int32_t) * (size_t)m * grid_c4) : 0;
   const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * 2 * (size_t)m) : 0;  // heavy, light lists
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)3,
-                         b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_cnt +
-                             b_touched + b_heavy, &ws));
+                         b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_piv +
+                             b_cnt + b_touched + b_heavy, &ws));
   char *p = (char *)ws;
   int64_t *up_start = (int64_t *)p; p += b_up_start;
   int32_t *up_len = (int32_t *)p; p += b_up_len;
@@ -814,12 +875,14 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   uint32_t *tcount = (uint32_t *)p; p += b_t;
   char *misc = p; p += b_misc;
   uint32_t *rows = (uint32_t *)p; p += b_rows;
+  int32_t *piv_small = (int32_t *)p; p += b_piv;
+  int32_t *piv_big = piv_small + m;
   uint32_t *cnts = (uint32_t *)p; p += b_cnt;
   int32_t *touched = (int32_t *)p; p += b_touched;
   int32_t *heavy = (int32_t *)p;
   int32_t *light = heavy + m;
   // [0] bad, [1] unsupported, [2..3] cursors, [4] #heavy, [5] max lower degree of a heavy
-  // vertex, [6] window-item cursor, [7] #light
+  // vertex, [6] window-item cursor, [7] #light, [8] #small pivots, [9] #big pivots, [10] cursor
   int32_t *bad = (int32_t *)misc;
   HomTotals *tot = (HomTotals *)(misc + 64);    // tri, k4
   unsigned long long *sums = (unsigned long long *)(misc + 128);  // t2 lo/hi, c2 lo/hi, c4 lo/hi
@@ -829,19 +892,26 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   hom_split_kernel<<<std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
       *g, up_start, up_len, lo_len, bad, (unsigned long long *)(misc + 192));
   GARY_HIP(hipGetLastError());
-  if (need_local) {
-    hom_local_kernel<<<grid_local, kHomBlock, 0, st>>>(*g, up_start, up_len, tcount, bad + 2, rows,
-                                                        scratch_words, tot, bad + 1);
+  if (need_local) {  // big pivots first, then the small ones
+    constexpr int32_t dbig = first_big(kHomSmallPool, true);
+    range_list_kernel<<<lgrid, 256, 0, st>>>(up_len, m, 2, dbig, piv_small, bad + 8, nullptr);
+    GARY_HIP(hipGetLastError());
+    range_list_kernel<<<lgrid, 256, 0, st>>>(up_len, m, dbig, INT32_MAX, piv_big, bad + 9, nullptr);
+    GARY_HIP(hipGetLastError());
+    hom_local_kernel<kBigWaves, kHomBigPool, true><<<grid_big, kBigWaves * kWave, 0, st>>>(
+        *g, up_start, up_len, piv_big, bad + 9, tcount, bad + 10, rows, scratch_words, tot, bad + 1);
+    GARY_HIP(hipGetLastError());
+    hom_local_kernel<kSmallWaves, kHomSmallPool, false><<<grid_small, kSmallWaves * kWave, 0, st>>>(
+        *g, up_start, up_len, piv_small, bad + 8, tcount, bad + 2, nullptr, 0, tot, bad + 1);
     GARY_HIP(hipGetLastError());
   }
   hom_sums_kernel<<<1024, 256, 0, st>>>(*g, up_start, up_len, tcount, sums, sums + 2);
   GARY_HIP(hipGetLastError());
   if (need_c4) {
-    const int lgrid = (int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096);
-    heavy_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, kC4HeavyLo, INT32_MAX, heavy, bad + 4,
+    range_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, kC4HeavyLo, INT32_MAX, heavy, bad + 4,
                                              bad + 5);
     GARY_HIP(hipGetLastError());
-    heavy_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, 2, kC4HeavyLo, light, bad + 7, nullptr);
+    range_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, 2, kC4HeavyLo, light, bad + 7, nullptr);
     GARY_HIP(hipGetLastError());
     c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, light, bad + 7, cnts, touched, bad + 3,
                                               sums + 4);
@@ -924,33 +994,47 @@ int gary_kclique_count(const gary_device_graph *g, int k, uint64_t *count, void 
   GARY_TRY(current_device(&dev));
   int ncu = 0;
   GARY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int grid = std::max(1, ncu * 4);
-  const int64_t scratch_words = (int64_t)kMaxHomDeg * (kMaxHomDeg / 64) * 2;
+  const int grid_small = std::max(1, ncu * 4), grid_big = std::max(1, ncu);
+  const int lgrid = (int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096);
+  const int64_t scratch_words = local_words(kMaxHomDeg, false);
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t b_up_start = al(sizeof(int64_t) * (size_t)m);
   const size_t b_len = al(sizeof(int32_t) * (size_t)m);
-  const size_t b_misc = 256;  // [0] bad, [1] unsupported, [2] cursor; +64: total; +192: asym
-  const size_t b_rows = k >= 3 ? al(sizeof(uint32_t) * (size_t)scratch_words * grid) : 0;
+  // [0] bad, [1] unsupported, [2..3] cursors, [4] #small, [5] #big; +64: total; +192: asym
+  const size_t b_misc = 256;
+  const size_t b_rows = k >= 3 ? al(sizeof(uint32_t) * (size_t)scratch_words * grid_big) : 0;
+  const size_t b_piv = k >= 3 ? al(sizeof(int32_t) * 2 * (size_t)m) : 0;  // pivot lists
   void *ws = nullptr;
-  GARY_TRY(workspace_get(dev, (void *)3, b_up_start + 2 * b_len + b_misc + b_rows, &ws));
+  GARY_TRY(workspace_get(dev, (void *)3, b_up_start + 2 * b_len + b_misc + b_rows + b_piv, &ws));
   char *p = (char *)ws;
   int64_t *up_start = (int64_t *)p; p += b_up_start;
   int32_t *up_len = (int32_t *)p; p += b_len;
   int32_t *lo_len = (int32_t *)p; p += b_len;
   char *misc = p; p += b_misc;
-  uint32_t *rows = (uint32_t *)p;
+  uint32_t *rows = (uint32_t *)p; p += b_rows;
+  int32_t *piv_small = (int32_t *)p;
+  int32_t *piv_big = piv_small + m;
   int32_t *bad = (int32_t *)misc;
   U128Acc *total = (U128Acc *)(misc + 64);
   GARY_HIP(hipMemsetAsync(misc, 0, b_misc, st));
   hom_split_kernel<<<std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
       *g, up_start, up_len, lo_len, bad, (unsigned long long *)(misc + 192));
   GARY_HIP(hipGetLastError());
-  if (k >= 3) {
+  if (k >= 3) {  // big pivots first, then the small ones
+    constexpr int32_t dbig = first_big(kKcSmallPool, false);
+    range_list_kernel<<<lgrid, 256, 0, st>>>(up_len, m, k - 1, dbig, piv_small, bad + 4, nullptr);
+    GARY_HIP(hipGetLastError());
+    range_list_kernel<<<lgrid, 256, 0, st>>>(up_len, m, dbig, INT32_MAX, piv_big, bad + 5, nullptr);
+    GARY_HIP(hipGetLastError());
     switch (k) {
 #define GARY_KC(K)                                                                               \
   case K:                                                                                        \
-    kclique_kernel<K><<<grid, kHomBlock, 0, st>>>(*g, up_start, up_len, bad + 2, rows,           \
-                                                   scratch_words, total, bad + 1);               \
+    kclique_kernel<K, kBigWaves, kKcBigPool, true><<<grid_big, kBigWaves * kWave, 0, st>>>(      \
+        *g, up_start, up_len, piv_big, bad + 5, bad + 3, rows, scratch_words, total, bad + 1);   \
+    GARY_HIP(hipGetLastError());                                                                 \
+    kclique_kernel<K, kSmallWaves, kKcSmallPool, false>                                          \
+        <<<grid_small, kSmallWaves * kWave, 0, st>>>(*g, up_start, up_len, piv_small, bad + 4,   \
+                                                     bad + 2, nullptr, 0, total, bad + 1);       \
     break;
       GARY_KC(3) GARY_KC(4) GARY_KC(5) GARY_KC(6) GARY_KC(7) GARY_KC(8) GARY_KC(9) GARY_KC(10)
 #undef GARY_KC
