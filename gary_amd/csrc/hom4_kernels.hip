@@ -691,7 +691,8 @@ constexpr int kC4Steps = 4;     // 64-endpoint steps whose loads are issued toge
 __global__ void __launch_bounds__(kC4Block)
     c4_window_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
                      const int32_t *__restrict__ heavy, const int32_t *__restrict__ nheavy_p,
-                     int32_t jmax, int64_t *__restrict__ cursors, int64_t cursor_stride,
+                     int32_t jmax, int64_t *__restrict__ cursors,
+                     int32_t *__restrict__ next_ep, int64_t stride,
                      int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
   __shared__ uint32_t cnt[kC4Words];
   __shared__ int32_t touch[kC4Touch];  // window offsets whose counter left zero
@@ -701,7 +702,8 @@ __global__ void __launch_bounds__(kC4Block)
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int64_t nheavy = *nheavy_p;
   const int64_t nitems = nheavy * jmax;
-  int64_t *cur = cursors + (int64_t)blockIdx.x * cursor_stride;
+  int64_t *cur = cursors + (int64_t)blockIdx.x * 2 * stride;
+  int32_t *nx = next_ep + (int64_t)blockIdx.x * stride;
   uint64_t acc = 0;
   for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
   if (threadIdx.x == 0) s_ntouch = 0;
@@ -722,8 +724,10 @@ __global__ void __launch_bounds__(kC4Block)
       const int32_t v = Nu[i];
       const int64_t pv = g.csr.indPtr[v];
       const int32_t dv = g.csr.deg[v];
-      cur[2 * i] = pv + lower_bound(col + pv, dv, k0 * kC4Win);
+      const int64_t c = pv + lower_bound(col + pv, dv, k0 * kC4Win);
+      cur[2 * i] = c;
       cur[2 * i + 1] = pv + dv;
+      nx[i] = c < pv + dv ? col[c] : INT32_MAX;
     }
     __syncthreads();
     const bool packed = nl < 65536;  // counts fit 16 bits
@@ -736,16 +740,30 @@ __global__ void __launch_bounds__(kC4Block)
         const int32_t i = i0 + lane;
         int64_t p = 0;
         int32_t len = 0;
-        if (i < nl) {  // endpoints w in [wlo, whi): a prefix of N(v) from the cursor
+        // endpoints w in [wlo, whi): a prefix of N(v) from the cursor. nx[i] caches the next
+        // endpoint, so a neighbour with none in this window costs one 4-byte load.
+        if (i < nl && nx[i] < whi) {
           p = cur[2 * i];
           const int64_t vend = cur[2 * i + 1];
-          const int32_t e0 = p < vend ? col[p] : INT32_MAX;
           const int32_t e1 = p + 1 < vend ? col[p + 1] : INT32_MAX;
           const int32_t e2 = p + 2 < vend ? col[p + 2] : INT32_MAX;
           const int32_t e3 = p + 3 < vend ? col[p + 3] : INT32_MAX;
-          len = (e0 < whi) + (e1 < whi) + (e2 < whi) + (e3 < whi);
-          if (len == 4 && p + 4 < vend) len = 4 + lower_bound(col + p + 4, (int32_t)(vend - p - 4), whi);
+          int32_t after;  // the endpoint following the run
+          if (e1 >= whi) {
+            len = 1;
+            after = e1;
+          } else if (e2 >= whi) {
+            len = 2;
+            after = e2;
+          } else if (e3 >= whi) {
+            len = 3;
+            after = e3;
+          } else {
+            len = p + 4 < vend ? 4 + lower_bound(col + p + 4, (int32_t)(vend - p - 4), whi) : 4;
+            after = p + len < vend ? col[p + len] : INT32_MAX;
+          }
           cur[2 * i] = p + len;
+          nx[i] = after;
         }
         const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)len);
         const int32_t start = incl - len;
@@ -921,13 +939,15 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
     GARY_HIP(hipStreamSynchronize(st));
     if (hv[0] > 0) {
       const int64_t stride = std::max<int32_t>(hv[1], 1);
-      void *cws = nullptr;
-      GARY_TRY(workspace_get(dev, (void *)8, 2 * sizeof(int64_t) * (size_t)stride * grid_win, &cws));
+      void *cws = nullptr;  // per workgroup: cursor and end (int64) and next endpoint (int32)
+      GARY_TRY(workspace_get(dev, (void *)8, 3 * sizeof(int64_t) * (size_t)stride * grid_win, &cws));
       const int32_t windows = (m + kC4Win - 1) / kC4Win;
       const int32_t jmax = (windows + kC4Group - 1) / kC4Group;
       GARY_CHECK_ARG((int64_t)hv[0] * jmax < INT32_MAX, "gary_hom4_count: too many window items");
       c4_window_kernel<<<grid_win, kC4Block, 0, st>>>(*g, lo_len, heavy, bad + 4, jmax,
-                                                      (int64_t *)cws, 2 * stride, bad + 6,
+                                                      (int64_t *)cws,
+                                                      (int32_t *)((int64_t *)cws + 2 * stride * grid_win),
+                                                      stride, bad + 6,
                                                       sums + 4);
       GARY_HIP(hipGetLastError());
     }
