@@ -588,6 +588,8 @@ __global__ void __launch_bounds__(WAVES * kWave)
 
 // ---- 4-cycles ------------------------------------------------------------------------------------
 
+constexpr int kC4HeavyLo = 256;  // lower neighbours from which a vertex takes the window path
+
 // First index of N(v) with value >= key.
 __device__ __forceinline__ int32_t lower_bound(const int32_t *__restrict__ N, int32_t d,
                                                int32_t key) {
@@ -600,12 +602,112 @@ __device__ __forceinline__ int32_t lower_bound(const int32_t *__restrict__ N, in
   return l;
 }
 
-// One workgroup per light vertex u (2 <= lower degree < kC4HeavyLo, from the compacted list
-// `light`): counts wedges u-v-w (v, w < u) into this block's dense counter slice (m entries, all
-// zero between vertices), listing each endpoint w the first time its counter leaves zero; then
-// reads and clears the listed counters, adding C(c, 2). The second pass touches the distinct
-// endpoints only (a few percent of the wedges at hub vertices). (Claiming every vertex id from
-// the cursor instead, most of them with nothing to count, cost a global atomic and two
+// Light vertices with few wedges: one wave per vertex u (static round robin over the list),
+// counting the endpoints of its wedges in the wave's own LDS hash table (kLightSlots keys, 8-bit
+// counts: a count never exceeds the lower degree, which is below 256). A vertex with more than
+// kLightMaxWedges wedges goes to `overflow` for c4_kernel. The runs N(v) ∩ [0, u) of the (at most
+// 255) lower neighbours are flattened over the wave as in the window kernel. On R-MAT 22 this
+// takes 85% of the 499K light vertices (8% of their wedges); one workgroup per vertex spent most
+// of its time on per-vertex barriers and latency.
+constexpr int kLightWaves = 4;
+constexpr int kLightSlots = 1024;
+constexpr int kLightMaxWedges = 512;  // load factor <= 1/2: probing always ends
+static_assert(kLightSlots == 1 << 10, "hslot(e, 10) indexes the table");
+static_assert(kC4HeavyLo <= 4 * kWave && kC4HeavyLo <= 256, "neighbours per lane; 8-bit counts");
+
+__global__ void __launch_bounds__(kLightWaves * kWave)
+    c4_light_wave_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
+                         const int32_t *__restrict__ light, const int32_t *__restrict__ nlight,
+                         int32_t *__restrict__ overflow, int32_t *__restrict__ noverflow,
+                         unsigned long long *c4_acc) {
+  __shared__ int32_t keys_all[kLightWaves][kLightSlots];
+  __shared__ uint32_t cnt_all[kLightWaves][kLightSlots / 4];
+  __shared__ int32_t slots_all[kLightWaves][kWave];
+  const int lane = lane_id(), wib = threadIdx.x / kWave;
+  int32_t *keys = keys_all[wib];
+  uint32_t *cnt = cnt_all[wib];
+  int32_t *slots = slots_all[wib];
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  for (int32_t x = lane; x < kLightSlots; x += kWave) keys[x] = -1;
+  for (int32_t x = lane; x < kLightSlots / 4; x += kWave) cnt[x] = 0u;
+  wave_sync();
+  uint64_t acc = 0;
+  const int64_t n_items = *nlight;
+  for (int64_t it = global_wave_id(); it < n_items; it += num_global_waves()) {
+    const int32_t u = light[it];
+    const int32_t nl = lo_len[u];  // 2 <= nl < kC4HeavyLo = 4 * kWave
+    const int32_t *Nu = col + g.csr.indPtr[u];
+    int64_t pv[kC4HeavyLo / kWave];
+    int32_t cut[kC4HeavyLo / kWave];
+    int32_t mine = 0;
+#pragma unroll
+    for (int r = 0; r < kC4HeavyLo / kWave; ++r) {
+      const int32_t k = r * kWave + lane;
+      pv[r] = 0;
+      cut[r] = 0;
+      if (k < nl) {
+        const int32_t v = Nu[k];
+        pv[r] = g.csr.indPtr[v];
+        cut[r] = lower_bound(col + pv[r], g.csr.deg[v], u);  // N(v) ∩ [0, u)
+        mine += cut[r];
+      }
+    }
+    if (wave_sum_u64((uint64_t)mine) > (uint64_t)kLightMaxWedges) {
+      if (lane == 0) overflow[atomicAdd(noverflow, 1)] = u;
+      continue;
+    }
+#pragma unroll
+    for (int r = 0; r < kC4HeavyLo / kWave; ++r) {
+      if (r * kWave >= nl) break;
+      const int32_t len = cut[r];
+      const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)len);
+      const int32_t start = incl - len;
+      const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+      for (int32_t w0 = 0; w0 < total; w0 += kWave) {
+        slots[lane] = -1;
+        wave_sync();
+        if (len > 0 && start < w0 + kWave && incl > w0) slots[max(0, start - w0)] = lane;
+        wave_sync();
+        const int32_t row = max(0, wave_inclusive_max_i32(slots[lane]));
+        const int64_t rp = __shfl(pv[r], row, kWave);
+        const int32_t rs = __shfl(start, row, kWave);
+        const int32_t e = w0 + lane < total ? col[rp + (w0 + lane - rs)] : -1;
+        wave_sync();  // slots are rewritten by the next step
+        if (e >= 0) {
+          uint32_t h = hslot(e, 10);
+          while (true) {
+            const int32_t old = atomicCAS(&keys[h], -1, e);
+            if (old == -1 || old == e) {
+              atomicAdd(&cnt[h >> 2], 1u << (8 * (h & 3)));
+              break;
+            }
+            h = (h + 1) & (kLightSlots - 1);
+          }
+        }
+      }
+    }
+    wave_sync();
+    for (int32_t x = lane; x < kLightSlots; x += kWave) {
+      if (keys[x] != -1) {
+        const uint64_t c = (cnt[x >> 2] >> (8 * (x & 3))) & 0xffu;
+        acc += c * (c - 1) / 2;
+        keys[x] = -1;
+      }
+    }
+    wave_sync();
+    for (int32_t x = lane; x < kLightSlots / 4; x += kWave) cnt[x] = 0u;
+    wave_sync();
+  }
+  const uint64_t sacc = wave_sum_u64(acc);
+  if (lane == 0) add_u128(c4_acc, sacc);
+}
+
+// One workgroup per light vertex u (2 <= lower degree < kC4HeavyLo) that has too many wedges
+// for c4_light_wave_kernel, from the list that kernel writes: counts wedges u-v-w (v, w < u)
+// into this block's dense counter slice (m entries, all zero between vertices), listing each
+// endpoint w the first time its counter leaves zero; then reads and clears the listed counters,
+// adding C(c, 2). The second pass touches the distinct endpoints only. (Claiming every vertex id
+// from the cursor instead, most of them with nothing to count, cost a global atomic and two
 // barriers per id: 105 ms of the C4 count on R-MAT 22.)
 __global__ void __launch_bounds__(kHomBlock)
     c4_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
@@ -682,7 +784,6 @@ __global__ void __launch_bounds__(kHomBlock)
 // cursor: its first four elements are read together, and only a run longer than that is ended
 // by a binary search. The runs of 64 neighbours v are flattened over a wave (as in the triangle
 // kernel's row streaming) so that every lane counts one endpoint per step.
-constexpr int kC4HeavyLo = 256; // lower neighbours from which a vertex takes the window path
 constexpr int kC4Words = 32768; // 128 KiB of LDS counters: one workgroup per CU
 constexpr int kC4Win = 65536;   // ids per window: 16-bit counters, two per word
 constexpr int kC4Group = 16;    // windows per work item (each item starts its cursors with a
@@ -885,7 +986,8 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   const size_t b_piv = need_local ? al(sizeof(int32_t) * 2 * (size_t)m) : 0;  // pivot lists
   const size_t b_cnt = need_c4 ? al(sizeof(uint32_t) * (size_t)m * grid_c4) : 0;
   const size_t b_touched = need_c4 ? al(sizeof(int32_t) * (size_t)m * grid_c4) : 0;
-  const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * 2 * (size_t)m) : 0;  // heavy, light lists
+  // heavy, light and light-overflow vertex lists
+  const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * 3 * (size_t)m) : 0;
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)3,
                          b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_piv +
@@ -903,8 +1005,10 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   int32_t *touched = (int32_t *)p; p += b_touched;
   int32_t *heavy = (int32_t *)p;
   int32_t *light = heavy + m;
+  int32_t *light_over = light + m;
   // [0] bad, [1] unsupported, [2..3] cursors, [4] #heavy, [5] max lower degree of a heavy
-  // vertex, [6] window-item cursor, [7] #light, [8] #small pivots, [9] #big pivots, [10] cursor
+  // vertex, [6] window-item cursor, [7] #light, [8] #small pivots, [9] #big pivots, [10] cursor,
+  // [11] #light vertices with too many wedges for the wave path
   int32_t *bad = (int32_t *)misc;
   HomTotals *tot = (HomTotals *)(misc + 64);    // tri, k4
   unsigned long long *sums = (unsigned long long *)(misc + 128);  // t2 lo/hi, c2 lo/hi, c4 lo/hi
@@ -935,8 +1039,11 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
     GARY_HIP(hipGetLastError());
     range_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, 2, kC4HeavyLo, light, bad + 7, nullptr);
     GARY_HIP(hipGetLastError());
-    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, light, bad + 7, cnts, touched, bad + 3,
-                                              sums + 4);
+    c4_light_wave_kernel<<<std::max(1, ncu * 7), kLightWaves * kWave, 0, st>>>(
+        *g, lo_len, light, bad + 7, light_over, bad + 11, sums + 4);  // 21 KiB of LDS: 7 per CU
+    GARY_HIP(hipGetLastError());
+    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, light_over, bad + 11, cnts, touched,
+                                              bad + 3, sums + 4);
     GARY_HIP(hipGetLastError());
     int32_t hv[2];  // #heavy, max lower degree: sizes the per-workgroup cursor slices
     GARY_HIP(hipMemcpyAsync(hv, bad + 4, sizeof hv, hipMemcpyDeviceToHost, st));
