@@ -702,6 +702,108 @@ __global__ void __launch_bounds__(kLightWaves * kWave)
   if (lane == 0) add_u128(c4_acc, sacc);
 }
 
+// Light vertices with more wedges (up to kBlockMaxWedges): one workgroup per vertex, counting in
+// an LDS hash table of packed words (vertex id << 8 | count; ids below 2^24 - 1, counts below 256
+// as above) sized per vertex to the next power of two >= 2 * wedges. The wedges of u form one
+// flat range [0, W): thread t takes endpoints t, t + block, ..., finding its lower neighbour by
+// a binary search over the run offsets in LDS. A vertex with more wedges goes to `overflow`.
+constexpr int kBlockThreads = 1024;
+constexpr int kBlockSlots = 32768;  // 128 KiB
+constexpr int kBlockMaxWedges = kBlockSlots / 2;
+constexpr uint32_t kBlockEmpty = 0xffffffffu;
+constexpr int32_t kBlockMaxId = (1 << 24) - 2;  // largest vertex id a packed key holds
+
+__global__ void __launch_bounds__(kBlockThreads)
+    c4_light_block_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
+                          const int32_t *__restrict__ items, const int32_t *__restrict__ nitems,
+                          int32_t *__restrict__ overflow, int32_t *__restrict__ noverflow,
+                          int32_t *__restrict__ cursor, unsigned long long *c4_acc) {
+  __shared__ uint32_t table[kBlockSlots];
+  __shared__ int64_t pv_s[kC4HeavyLo];
+  __shared__ int32_t off_s[kC4HeavyLo + 1];  // exclusive prefix of the run lengths
+  __shared__ int32_t s_item;
+  const int lane = lane_id();
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  for (int32_t x = threadIdx.x; x < kBlockSlots; x += kBlockThreads) table[x] = kBlockEmpty;
+  uint64_t acc = 0;
+  const int32_t n_items = *nitems;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    if (it >= n_items) break;
+    const int32_t u = items[it];
+    const int32_t nl = lo_len[u];  // 2 <= nl < kC4HeavyLo
+    const int32_t *Nu = col + g.csr.indPtr[u];
+    if (threadIdx.x < kC4HeavyLo) {
+      const int32_t k = threadIdx.x;
+      int32_t cut = 0;
+      if (k < nl) {
+        const int32_t v = Nu[k];
+        pv_s[k] = g.csr.indPtr[v];
+        cut = lower_bound(col + pv_s[k], g.csr.deg[v], u);  // N(v) ∩ [0, u)
+      }
+      off_s[k + 1] = cut;
+    }
+    if (threadIdx.x == 0) off_s[0] = 0;
+    __syncthreads();
+    if (threadIdx.x < kWave) {  // inclusive scan of off_s[1 .. kC4HeavyLo]
+      uint32_t carry = 0;
+      for (int32_t c0 = 0; c0 < kC4HeavyLo; c0 += kWave) {
+        const uint32_t v = (uint32_t)off_s[c0 + lane + 1];
+        const uint32_t inc = wave_inclusive_scan_u32(v);
+        off_s[c0 + lane + 1] = (int32_t)(carry + inc);
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+      }
+    }
+    __syncthreads();
+    const int32_t W = off_s[nl];
+    if (W > kBlockMaxWedges) {
+      if (threadIdx.x == 0) overflow[atomicAdd(noverflow, 1)] = u;
+      __syncthreads();
+      continue;
+    }
+    int32_t log_t = 10;
+    while ((1 << log_t) < 2 * W) ++log_t;
+    const uint32_t mask = (1u << log_t) - 1u;
+    for (int32_t e = threadIdx.x; e < W; e += kBlockThreads) {
+      int32_t lo = 0, hi = nl - 1;  // last k with off_s[k] <= e
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (off_s[mid] <= e) lo = mid;
+        else hi = mid - 1;
+      }
+      const int32_t w = col[pv_s[lo] + (e - off_s[lo])];
+      const uint32_t key = (uint32_t)w << 8;
+      uint32_t h = hslot(w, log_t);
+      while (true) {
+        uint32_t cur = table[h];
+        if (cur == kBlockEmpty) {
+          cur = atomicCAS(&table[h], kBlockEmpty, key | 1u);
+          if (cur == kBlockEmpty) break;
+        }
+        if ((cur & ~0xffu) == key) {
+          atomicAdd(&table[h], 1u);
+          break;
+        }
+        h = (h + 1) & mask;
+      }
+    }
+    __syncthreads();
+    for (int32_t x = threadIdx.x; x <= (int32_t)mask; x += kBlockThreads) {
+      const uint32_t t = table[x];
+      if (t != kBlockEmpty) {
+        const uint64_t c = t & 0xffu;
+        acc += c * (c - 1) / 2;
+        table[x] = kBlockEmpty;
+      }
+    }
+    __syncthreads();
+  }
+  const uint64_t sacc = wave_sum_u64(acc);
+  if (lane == 0) add_u128(c4_acc, sacc);
+}
+
 // One workgroup per light vertex u (2 <= lower degree < kC4HeavyLo) that has too many wedges
 // for c4_light_wave_kernel, from the list that kernel writes: counts wedges u-v-w (v, w < u)
 // into this block's dense counter slice (m entries, all zero between vertices), listing each
@@ -986,8 +1088,8 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   const size_t b_piv = need_local ? al(sizeof(int32_t) * 2 * (size_t)m) : 0;  // pivot lists
   const size_t b_cnt = need_c4 ? al(sizeof(uint32_t) * (size_t)m * grid_c4) : 0;
   const size_t b_touched = need_c4 ? al(sizeof(int32_t) * (size_t)m * grid_c4) : 0;
-  // heavy, light and light-overflow vertex lists
-  const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * 3 * (size_t)m) : 0;
+  // heavy and light vertex lists, and the light vertices each light kernel passes on
+  const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * 4 * (size_t)m) : 0;
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)3,
                          b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_piv +
@@ -1006,9 +1108,11 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   int32_t *heavy = (int32_t *)p;
   int32_t *light = heavy + m;
   int32_t *light_over = light + m;
+  int32_t *light_over2 = light_over + m;
   // [0] bad, [1] unsupported, [2..3] cursors, [4] #heavy, [5] max lower degree of a heavy
   // vertex, [6] window-item cursor, [7] #light, [8] #small pivots, [9] #big pivots, [10] cursor,
-  // [11] #light vertices with too many wedges for the wave path
+  // [11] #light vertices with too many wedges for the wave path, [12] #those with too many for
+  // the workgroup hash path, [13] cursor
   int32_t *bad = (int32_t *)misc;
   HomTotals *tot = (HomTotals *)(misc + 64);    // tri, k4
   unsigned long long *sums = (unsigned long long *)(misc + 128);  // t2 lo/hi, c2 lo/hi, c4 lo/hi
@@ -1042,8 +1146,16 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
     c4_light_wave_kernel<<<std::max(1, ncu * 7), kLightWaves * kWave, 0, st>>>(
         *g, lo_len, light, bad + 7, light_over, bad + 11, sums + 4);  // 21 KiB of LDS: 7 per CU
     GARY_HIP(hipGetLastError());
-    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, light_over, bad + 11, cnts, touched,
-                                              bad + 3, sums + 4);
+    const int32_t *rest = light_over, *nrest = bad + 11;
+    if (m - 1 <= kBlockMaxId) {  // packed keys hold every id
+      c4_light_block_kernel<<<std::max(1, ncu), kBlockThreads, 0, st>>>(
+          *g, lo_len, light_over, bad + 11, light_over2, bad + 12, bad + 13, sums + 4);
+      GARY_HIP(hipGetLastError());
+      rest = light_over2;
+      nrest = bad + 12;
+    }
+    c4_kernel<<<grid_c4, kHomBlock, 0, st>>>(*g, lo_len, rest, nrest, cnts, touched, bad + 3,
+                                              sums + 4);
     GARY_HIP(hipGetLastError());
     int32_t hv[2];  // #heavy, max lower degree: sizes the per-workgroup cursor slices
     GARY_HIP(hipMemcpyAsync(hv, bad + 4, sizeof hv, hipMemcpyDeviceToHost, st));

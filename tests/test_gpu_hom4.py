@@ -86,3 +86,46 @@ def test_hom4_rejects_unranked_or_oriented():
     if np.all(np.diff(og.deg) >= 0):  # only meaningful if the degree order still holds
         with pytest.raises(_lib.GaryError):
             gq.hom4Count(DeviceGraphCtx(og))
+
+
+def _light_paths_graph():
+    """u with 255 neighbours v_i and 300 hub neighbours (a 300-clique, each
+    hub with 600 leaves); every v_i is joined to the same 100 vertices w_j. Ranked by degree, the
+    w_j and u are light (fewer than 256 lower neighbours) with thousands to tens of thousands of
+    wedges each, and the hubs are heavy."""
+    u, V, Wn, H = 0, np.arange(1, 256), np.arange(256, 356), np.arange(356, 656)
+    leaves = 656 + np.arange(300 * 600)
+    src, dst = [], []
+    src += [np.full(255, u)]; dst += [V]
+    vv, ww = np.meshgrid(V, Wn, indexing="ij")
+    src += [vv.ravel()]; dst += [ww.ravel()]
+    src += [np.full(300, u)]; dst += [H]
+    hi, hj = np.triu_indices(300, 1)
+    src += [H[hi]]; dst += [H[hj]]
+    src += [np.repeat(H, 600)]; dst += [leaves]
+    n = 656 + 300 * 600
+    return n, np.concatenate(src).astype(np.int32), np.concatenate(dst).astype(np.int32)
+
+
+def test_hom4_light_vertex_paths():
+    """Every light-vertex 4-cycle path (wave hash, workgroup hash, HBM counters) and the window
+    path on one graph, against the oracle."""
+    n, s, d = _light_paths_graph()
+    g, ctx = _ranked_ctx(n, s, d)
+    ip, ci = g.indPtr, g.colIdx
+    rows = np.repeat(np.arange(g.m), np.diff(ip))
+    lo = np.bincount(rows[ci < rows], minlength=g.m)
+
+    def wedges(x):  # Σ over lower neighbours v of |N(v) ∩ [0, x)|
+        return sum(int(np.searchsorted(ci[ip[v]:ip[v + 1]], x)) for v in ci[ip[x]:ip[x + 1]]
+                   if v < x)
+
+    w = np.array([wedges(x) for x in np.where((lo >= 2) & (lo < 256))[0]])
+    # wave hash (<= 512 wedges), workgroup hash (<= 16,384), HBM counters (more); heavy windows
+    assert (w <= 512).sum() > 0 and ((w > 512) & (w <= 16384)).sum() > 0 and (w > 16384).sum() > 0
+    assert (lo >= 256).sum() > 0
+    assert g.nnz == 2 * (255 + 25500 + 300 + 300 * 299 // 2 + 300 * 600)
+    want = oracle.hom4(g.indPtr, g.colIdx)
+    assert want["c4"] > 0
+    assert gq.hom4Count(ctx) == want
+    assert gq.hom4Count(ctx, gq.HOM_C4)["c4"] == want["c4"]
