@@ -693,8 +693,10 @@ __global__ void __launch_bounds__(256)
 //
 // The counting kernel's sets, row streaming and schedule, with every hit (v, w) of pivot u
 // written out as the triple (u, v, w): hits are appended in ballot order to a per-wave LDS
-// stage of kListStage triples, and a full stage claims one output range with a single atomic
-// and is copied out with coalesced stores.
+// stage of kListStage triples; a full stage is copied to the wave's own chunk of global scratch
+// (no atomic), and a full chunk claims one output range with a single atomic and is copied out
+// with coalesced loads and stores. Claiming per stage, 1.66 M atomics on one address for the
+// 424 M triangles of R-MAT 20, serialised at one L2 channel and took two thirds of the time.
 
 constexpr int kListStage = 256;
 
@@ -702,36 +704,55 @@ struct ListOut {
   unsigned long long *total;  // triples found (claimed output positions)
   int32_t *out;               // 3 * cap int32
   unsigned long long cap;
+  int32_t *scratch;           // 3 * chunk int32 per wave: blockIdx.x * waves + wave
+  int32_t chunk;              // triples per wave chunk, >= kListStage
 };
 
-__device__ __forceinline__ void list_flush(const int32_t *stage, int32_t fill, const ListOut &lo) {
-  wave_sync();  // the staged triples are visible to every lane
+// A wave's listing state: its LDS stage and its scratch chunk (fills are wave-uniform).
+struct ListWave {
+  int32_t *stage;
+  int32_t fill;
+  int32_t *chunk;
+  int32_t cfill;
+};
+
+// Claims an output range for the wave's chunk and copies it out.
+__device__ __forceinline__ void list_chunk_flush(ListWave &lw, const ListOut &lo) {
+  // the chunk was stored by every lane of this wave: make those stores visible to its loads
+  __threadfence_block();
   unsigned long long base = 0;
-  if (lane_id() == 0) base = atomicAdd(lo.total, (unsigned long long)fill);
+  if (lane_id() == 0) base = atomicAdd(lo.total, (unsigned long long)lw.cfill);
   base = __shfl(base, 0, kWave);
-  for (int32_t i = lane_id(); i < 3 * fill; i += kWave)
-    if (base + (unsigned long long)(i / 3) < lo.cap)
-      lo.out[3 * base + (unsigned long long)i] = stage[i];
+  for (int32_t i = lane_id(); i < 3 * lw.cfill; i += kWave)
+    if (base + (unsigned long long)(i / 3) < lo.cap) lo.out[3 * base + (unsigned long long)i] = lw.chunk[i];
+  lw.cfill = 0;
+}
+
+// Moves the LDS stage to the scratch chunk (flushing the chunk first when it is too full).
+__device__ __forceinline__ void list_flush(ListWave &lw, const ListOut &lo) {
+  wave_sync();  // the staged triples are visible to every lane
+  if (lw.cfill + lw.fill > lo.chunk) list_chunk_flush(lw, lo);
+  int32_t *dst = lw.chunk + 3 * lw.cfill;
+  for (int32_t i = lane_id(); i < 3 * lw.fill; i += kWave) dst[i] = lw.stage[i];
+  lw.cfill += lw.fill;
+  lw.fill = 0;
   wave_sync();  // the stage is read before it is refilled
 }
 
-// Stages this lane's triple when `hit`; every lane of the wave calls it (fill is wave-uniform).
-__device__ __forceinline__ void list_emit(int32_t *stage, int32_t &fill, const ListOut &lo,
-                                          bool hit, int32_t u, int32_t v, int32_t w) {
+// Stages this lane's triple when `hit`; every lane of the wave calls it.
+__device__ __forceinline__ void list_emit(ListWave &lw, const ListOut &lo, bool hit, int32_t u,
+                                          int32_t v, int32_t w) {
   const uint64_t hits = __ballot(hit);
   if (!hits) return;
   const int32_t c = __popcll(hits);
-  if (fill + c > kListStage) {
-    list_flush(stage, fill, lo);
-    fill = 0;
-  }
+  if (lw.fill + c > kListStage) list_flush(lw, lo);
   if (hit) {
-    const int32_t at = 3 * (fill + lanes_below(hits));
-    stage[at] = u;
-    stage[at + 1] = v;
-    stage[at + 2] = w;
+    const int32_t at = 3 * (lw.fill + lanes_below(hits));
+    lw.stage[at] = u;
+    lw.stage[at + 1] = v;
+    lw.stage[at + 2] = w;
   }
-  fill += c;
+  lw.fill += c;
 }
 
 // stream_rows with output: the owning row's vertex travels with each chunk.
@@ -740,8 +761,7 @@ __device__ __forceinline__ void stream_rows_list(const gary_device_graph &g, int
                                                  const int32_t *__restrict__ L, int32_t d,
                                                  int32_t r0, int32_t stride, const int32_t *table,
                                                  const PivotSet &ps, RowSlots &rs, int lane,
-                                                 int32_t *stage, int32_t &fill,
-                                                 const ListOut &lo) {
+                                                 ListWave &lw, const ListOut &lo) {
   const uint32_t m = (uint32_t)g.csr.m;
   const int64_t nnz = g.csr.nnz;
   const int32_t *__restrict__ col = g.csr.colIdx;
@@ -799,10 +819,10 @@ __device__ __forceinline__ void stream_rows_list(const gary_device_graph &g, int
         const bool h1 = (jlo[k] <= 1 && 1 < jhi[k]) ? pivot_probe(table, ps, data[k].y) != 0 : false;
         const bool h2 = (jlo[k] <= 2 && 2 < jhi[k]) ? pivot_probe(table, ps, data[k].z) != 0 : false;
         const bool h3 = (jlo[k] <= 3 && 3 < jhi[k]) ? pivot_probe(table, ps, data[k].w) != 0 : false;
-        list_emit(stage, fill, lo, h0, u, vr[k], data[k].x);
-        list_emit(stage, fill, lo, h1, u, vr[k], data[k].y);
-        list_emit(stage, fill, lo, h2, u, vr[k], data[k].z);
-        list_emit(stage, fill, lo, h3, u, vr[k], data[k].w);
+        list_emit(lw, lo, h0, u, vr[k], data[k].x);
+        list_emit(lw, lo, h1, u, vr[k], data[k].y);
+        list_emit(lw, lo, h2, u, vr[k], data[k].z);
+        list_emit(lw, lo, h3, u, vr[k], data[k].w);
       }
     }
   }
@@ -812,7 +832,7 @@ __device__ __forceinline__ void stream_rows_list(const gary_device_graph &g, int
 __device__ __forceinline__ void fallback_list(const gary_device_graph &g, int32_t u,
                                               const int32_t *__restrict__ L, int32_t d,
                                               int32_t r0, int32_t stride, int lane,
-                                              int32_t *stage, int32_t &fill, const ListOut &lo) {
+                                              ListWave &lw, const ListOut &lo) {
   const uint32_t m = (uint32_t)g.csr.m;
   for (int32_t k = r0; k < d; k += stride) {
     const int32_t v = L[k];
@@ -832,7 +852,7 @@ __device__ __forceinline__ void fallback_list(const gary_device_graph &g, int32_
         }
         hit = lo_i < d && L[lo_i] == w;
       }
-      list_emit(stage, fill, lo, hit, u, v, w);
+      list_emit(lw, lo, hit, u, v, w);
     }
   }
 }
@@ -851,8 +871,8 @@ __global__ void __launch_bounds__(WAVES * 64)
   const int lane = lane_id();
   const int wib = threadIdx.x / kWave;
   RowSlots &rb = s_rows[wib];
-  int32_t *stage = staged[wib];
-  int32_t fill = 0;
+  ListWave lw{staged[wib], 0,
+              lo.scratch + ((int64_t)blockIdx.x * WAVES + wib) * 3 * (int64_t)lo.chunk, 0};
   const int32_t n_small = counts[0], n_big = counts[1];
   if (n_small == 0 && n_big == 0) return;
   const uint32_t m = (uint32_t)g.csr.m;
@@ -869,10 +889,9 @@ __global__ void __launch_bounds__(WAVES * 64)
       PivotSet s;
       if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
         pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
-        stream_rows_list<kStreamWindows>(g, u, L, d, wib, WAVES, table, s, rb, lane, stage, fill,
-                                         lo);
+        stream_rows_list<kStreamWindows>(g, u, L, d, wib, WAVES, table, s, rb, lane, lw, lo);
       } else {
-        fallback_list(g, u, L, d, wib, WAVES, lane, stage, fill, lo);
+        fallback_list(g, u, L, d, wib, WAVES, lane, lw, lo);
       }
     }
     __syncthreads();
@@ -898,15 +917,16 @@ __global__ void __launch_bounds__(WAVES * 64)
       const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
       PivotSet s;
       if (!pivot_plan(L[0], L[d - 1], d, TABLE_WORDS / WAVES, s)) {
-        fallback_list(g, u, L, d, 0, 1, lane, stage, fill, lo);
+        fallback_list(g, u, L, d, 0, 1, lane, lw, lo);
         continue;
       }
       pivot_build<false>(wt, s, L, d, lane, kWave);
-      stream_rows_list<kStreamWindows>(g, u, L, d, 0, 1, wt, s, rb, lane, stage, fill, lo);
+      stream_rows_list<kStreamWindows>(g, u, L, d, 0, 1, wt, s, rb, lane, lw, lo);
       wave_sync();
     }
   }
-  if (fill > 0) list_flush(stage, fill, lo);
+  if (lw.fill > 0) list_flush(lw, lo);
+  if (lw.cfill > 0) list_chunk_flush(lw, lo);
 }
 
 }  // namespace gary
@@ -927,6 +947,7 @@ constexpr int kPivotWords = 4096;  // 16 KiB: a big pivot's set, or one small-pi
 constexpr int kHubWaves = 8;
 constexpr int kHubWords = 8192;    // 32 KiB hub set
 constexpr int kHubGrid = 512;
+constexpr int kListGrid = 2048;  // main listing launch: four waves per workgroup
 
 // Pivot path: `block` picks the waves per workgroup of the main pivot kernel (128, 256 or 512
 // threads; other values take 256).
@@ -1150,11 +1171,20 @@ int launch_tri_list(const gary_device_graph &g, unsigned long long *total, int32
   bin_scatter_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, lists,
                                             (int64_t)stride, counters, nullptr, nullptr, 0.f, true);
   GARY_HIP(hipGetLastError());
-  const ListOut lo{total, out, cap};
+  // per-wave scratch chunks (the hub launch's waves reuse the main launch's first chunks; the
+  // launches run in stream order). A chunk holds about 1/16384 of the output, 256 to 8,192
+  // triples: at most 768 MiB of scratch, for one claim per chunk.
+  constexpr int64_t kListWaves = std::max(kHubGrid * kHubWaves, kListGrid * 4);
+  int32_t chunk = kListStage;
+  while (chunk < 8192 && (unsigned long long)chunk * 16384ull < cap) chunk *= 2;
+  void *scratch = nullptr;
+  GARY_TRY(workspace_get(dev, (void *)11, sizeof(int32_t) * 3 * (size_t)chunk * kListWaves,
+                         &scratch));
+  const ListOut lo{total, out, cap, (int32_t *)scratch, chunk};
   tri_list_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
       g, lists, lists + 2 * stride, counters + 2, counters + 6, lo, counters + kSplitOff + 2);
   GARY_HIP(hipGetLastError());
-  tri_list_pivot_kernel<4, kPivotWords><<<2048, 4 * 64, 0, st>>>(
+  tri_list_pivot_kernel<4, kPivotWords><<<kListGrid, 4 * 64, 0, st>>>(
       g, lists, lists + stride, counters, counters + 4, lo, counters + kSplitOff);
   GARY_HIP(hipGetLastError());
   return GARY_OK;

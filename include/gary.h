@@ -395,7 +395,8 @@ int gary_kclique_count(const gary_device_graph *g, int k, uint64_t *count, void 
  * *total receives the number of triangles; when it exceeds cap only cap triples are written and
  * the call returns GARY_ERR_OVERFLOW. The intersection-with-output of
  * include/ops/Intersection.cuh:18-50,76-112 applied to the triangle query of
- * include/kernels/GraphQueryHom.cuh:15-43 (which the reference only counts). Synchronous. */
+ * include/kernels/GraphQueryHom.cuh:15-43 (which the reference only counts). Synchronous.
+ * Scratch: a chunk of 256 to 8,192 triples per wave (about cap / 16384), at most 768 MiB. */
 int gary_tri_list(const gary_device_graph *g, int32_t *out, uint64_t cap, uint64_t *total,
                   void *stream);
 
