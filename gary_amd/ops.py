@@ -27,6 +27,12 @@ _lib.bind("gary_search_count", C.c_int, [_P, _I64, _P, _I64, _P, C.c_int, C.c_in
 _lib.bind("gary_search_batched", C.c_int, [C.c_int32, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P])
 _lib.bind("gary_intersect_count", C.c_int, [_P, _I64, _P, _I64, C.c_int, C.c_int, _P, _P])
 _lib.bind("gary_intersect_emit", C.c_int, [_P, _I64, _P, _I64, C.c_int, C.c_int, _P, _P, _P])
+_lib.bind("gary_block_queue_exercise", C.c_int, [C.c_int, _P, C.c_uint32, _P, _P, _P])
+
+QUEUE_EXCLUSIVE = 0
+QUEUE_CONCURRENT = 1
+QUEUE_SLOTS = 128
+QUEUE_WAVES = 16
 
 
 def _dev(a) -> tuple[DeviceBuffer, int]:
@@ -88,3 +94,17 @@ def intersect(a, b, method: int = ISECT_BSEARCH, cache_depth: int = 32) -> np.nd
                                   None))
     k = int(cnt.to_array(np.uint64, 1)[0])
     return out.to_array(np.int32, k)
+
+
+def block_queue_exercise(mode: int, items) -> tuple[np.ndarray, np.ndarray]:
+    """The block-scope queue (CTAQueue, include/ops/DeviceQueue.cuh:33-182) driven on the device:
+    (out, status) as gary_block_queue_exercise describes them."""
+    a = np.ascontiguousarray(items, dtype=np.uint32)
+    d_items = DeviceBuffer.from_array(a)
+    n_out = QUEUE_WAVES * QUEUE_SLOTS if mode == QUEUE_EXCLUSIVE else max(int(a.size), 1)
+    d_out = DeviceBuffer(4 * n_out)
+    d_status = DeviceBuffer(4 * QUEUE_WAVES)
+    check(lib.gary_block_queue_exercise(int(mode), d_items.ptr, int(a.size), d_out.ptr,
+                                        d_status.ptr, None))
+    check(lib.gary_device_synchronize(0))
+    return d_out.to_array(np.uint32, n_out), d_status.to_array(np.int32, QUEUE_WAVES)

@@ -389,6 +389,23 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
  * vertex has more than 2,048 higher-ranked neighbours. */
 int gary_kclique_count(const gary_device_graph *g, int k, uint64_t *count, void *stream);
 
+/* ---- block-scope queue (include/ops/DeviceQueue.cuh:33-182, CTAQueue) ------------------------ */
+
+/* Runs the device queue of gary_amd/csrc/block_queue.h (the reference's CTAQueue: clear / fill /
+ * size / tryEnqX / tryDeqX / tryEnq / tryDeq on a 128-slot ring in LDS) in one workgroup of 16
+ * waves, lane 0 of each wave calling it, as the reference's test does
+ * (test/CTACommTest.cu:91-119). Asynchronous on `stream`; d_out is zeroed first.
+ *   mode 0, exclusive: the queue is filled with d_items[0..n), n <= 128, and every wave dequeues
+ *     with tryDeqX until it is empty: d_out[w*128 + k] = the k-th item wave w took,
+ *     d_status[w] = how many it took (d_out: 16*128 entries).
+ *   mode 1, concurrent: waves 0-7 enqueue their share of d_items[0..n) (values < n) through the
+ *     ring while waves 8-15 dequeue until n items are out; d_out[x] counts the times item x came
+ *     out (n entries), d_status[w] = the items wave w moved, or -1 if a call of that wave was
+ *     stuck or an item was out of range.
+ * Every retry loop in the kernel is bounded, so it always ends. */
+int gary_block_queue_exercise(int mode, const uint32_t *d_items, uint32_t n, uint32_t *d_out,
+                              int32_t *d_status, void *stream);
+
 /* Triangle listing on a degree-oriented CSR (GARY_CSR_ORIENT_DEGREE or ORIENT_ID; needs no COO):
  * every triangle once, as the int32 triple (u, v, w) with v in N+(u) and w in N+(u) ∩ N+(v),
  * written to the device array out[3*cap]. The order of the triples depends on scheduling.
