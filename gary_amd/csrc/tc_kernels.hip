@@ -1172,11 +1172,13 @@ int launch_tri_list(const gary_device_graph &g, unsigned long long *total, int32
                                             (int64_t)stride, counters, nullptr, nullptr, 0.f, true);
   GARY_HIP(hipGetLastError());
   // per-wave scratch chunks (the hub launch's waves reuse the main launch's first chunks; the
-  // launches run in stream order). A chunk holds about 1/16384 of the output, 256 to 8,192
-  // triples: at most 768 MiB of scratch, for one claim per chunk.
+  // launches run in stream order). A chunk holds about 1/16384 of the output, 256 to 2,048
+  // triples: at most 192 MiB of scratch, for one claim per chunk. On R-MAT 20 (424 M triangles)
+  // a 2,048-triple cap took 8.1 ms, 8,192 10.1 ms (the chunk is gone from L2 before it is read
+  // back), 1,024 8.9 ms and 512 12.9 ms (claims on one address again).
   constexpr int64_t kListWaves = std::max(kHubGrid * kHubWaves, kListGrid * 4);
   int32_t chunk = kListStage;
-  while (chunk < 8192 && (unsigned long long)chunk * 16384ull < cap) chunk *= 2;
+  while (chunk < 2048 && (unsigned long long)chunk * 16384ull < cap) chunk *= 2;
   void *scratch = nullptr;
   GARY_TRY(workspace_get(dev, (void *)11, sizeof(int32_t) * 3 * (size_t)chunk * kListWaves,
                          &scratch));

@@ -413,7 +413,7 @@ int gary_block_queue_exercise(int mode, const uint32_t *d_items, uint32_t n, uin
  * the call returns GARY_ERR_OVERFLOW. The intersection-with-output of
  * include/ops/Intersection.cuh:18-50,76-112 applied to the triangle query of
  * include/kernels/GraphQueryHom.cuh:15-43 (which the reference only counts). Synchronous.
- * Scratch: a chunk of 256 to 8,192 triples per wave (about cap / 16384), at most 768 MiB. */
+ * Scratch: a chunk of 256 to 2,048 triples per wave (about cap / 16384), at most 192 MiB. */
 int gary_tri_list(const gary_device_graph *g, int32_t *out, uint64_t cap, uint64_t *total,
                   void *stream);
 
