@@ -105,6 +105,21 @@ __device__ __forceinline__ void wave_atomic_add(unsigned long long *dst, uint64_
   if (lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
 }
 
+// Adds the block's sum of v to *dst with one global atomic (through the LDS word `acc`). Every
+// thread of the block calls it, once, at the end of the kernel. Waves of a persistent kernel
+// finish together, and one global atomic per wave on a single address then serialises at one L2
+// channel (about 10 ns each: 8,192 waves add up to about 80 us).
+__device__ __forceinline__ void block_atomic_add(unsigned long long *dst, uint64_t v,
+                                                 unsigned long long *acc) {
+  __syncthreads();
+  if (threadIdx.x == 0) *acc = 0;
+  __syncthreads();
+  const uint64_t s = wave_sum_u64(v);
+  if (lane_id() == 0 && s) atomicAdd(acc, (unsigned long long)s);
+  __syncthreads();
+  if (threadIdx.x == 0 && *acc) atomicAdd(dst, *acc);
+}
+
 // Block-wide u64 sum into a counter: wave sums -> LDS -> one atomic per workgroup.
 template <int BLOCK>
 __device__ __forceinline__ void block_atomic_add(unsigned long long *dst, uint64_t v,

@@ -45,7 +45,10 @@ __global__ void __launch_bounds__(kBlock)
     if constexpr (COUNT) c += f;
     else results[i] = f;
   }
-  if constexpr (COUNT) wave_atomic_add(count, c);
+  if constexpr (COUNT) {
+    __shared__ unsigned long long s_acc;
+    block_atomic_add(count, c, &s_acc);
+  }
 }
 
 // batchedBinarySearch{,2Phase}Kernel: one wave per item, INTERLEAVED over items.
@@ -123,7 +126,8 @@ __global__ void __launch_bounds__(kBlock)
       block_counts[blockIdx.x] = (int32_t)t;
     }
   } else {
-    wave_atomic_add(count, c);
+    __shared__ unsigned long long s_acc;
+    block_atomic_add(count, c, &s_acc);
   }
 }
 
@@ -208,7 +212,8 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();  // the tile's LDS is reloaded by the next iteration
   }
-  wave_atomic_add(count, c);
+  __shared__ unsigned long long s_acc;
+  block_atomic_add(count, c, &s_acc);
 }
 
 // Bitmap count: each block owns windows of WORDS*32 consecutive ids; it marks the elements of B
@@ -255,7 +260,8 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
   }
-  wave_atomic_add(count, c);
+  __shared__ unsigned long long s_acc;
+  block_atomic_add(count, c, &s_acc);
 }
 
 // Exclusive scan of block counts in one workgroup (nblocks <= 4096) and the total.

@@ -143,7 +143,8 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
         c += isect_group<G, METHOD>(A, g.csr.deg[src], B, g.csr.deg[dst], cache, n_cache, glane);
       }
     }
-    wave_atomic_add(count, c);
+    __shared__ unsigned long long s_acc_dyn;
+    block_atomic_add(count, c, &s_acc_dyn);
     return;
   }
   if constexpr (PART == GARY_PARTITION_CHUNKED) {
@@ -164,7 +165,8 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
     const int32_t *B = g.csr.colIdx + g.csr.indPtr[dst];
     c += isect_group<G, METHOD>(A, g.csr.deg[src], B, g.csr.deg[dst], cache, n_cache, glane);
   }
-  wave_atomic_add(count, c);
+  __shared__ unsigned long long s_acc;
+  block_atomic_add(count, c, &s_acc);
 }
 
 // List-driven variant: one group per arc of `arcs[0 .. *n_arcs)` (arc = index into the CSR).
@@ -190,7 +192,8 @@ __global__ void __launch_bounds__(256) tc_edge_list_kernel(gary_device_graph g, 
     const int32_t *B = g.csr.colIdx + g.csr.indPtr[dst];
     c += isect_group<G, METHOD>(A, g.csr.deg[src], B, g.csr.deg[dst], cache, n_cache, glane);
   }
-  wave_atomic_add(count, c);
+  __shared__ unsigned long long s_acc;
+  block_atomic_add(count, c, &s_acc);
 }
 
 // ======================================================================================
@@ -540,8 +543,12 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       wave_sync();  // the set is rebuilt for the next pivot
     }
   }
-  if (!gid_of) wave_atomic_add(count, c);
-  else if (cur >= 0) wave_atomic_add(&gcount[cur], c);
+  if (!gid_of) {
+    __shared__ unsigned long long s_acc;
+    block_atomic_add(count, c, &s_acc);
+  } else if (cur >= 0) {
+    wave_atomic_add(&gcount[cur], c);
+  }
 }
 
 // Zeroes `n` words (graph-capturable stand-in for a memset).
