@@ -275,8 +275,19 @@ __global__ void __launch_bounds__(WAVES * kWave)
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int32_t n_items = *nitems;
   uint64_t tri = 0, k4 = 0;
+  // thread 0 claims kClaim consecutive items per atomic and hands them out one at a time
+  // through s_item: four small pivots per claim on the one cursor (R-MAT 22 K4 + diamond
+  // 83 -> 81 ms); big pivots one at a time, heaviest first.
+  constexpr int32_t kClaim = BIG ? 1 : 4;
+  int32_t next = 0, end = 0;  // thread 0's claimed range
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
+    if (threadIdx.x == 0) {
+      if (next >= end) {
+        next = atomicAdd(cursor, kClaim);
+        end = next + kClaim;
+      }
+      s_item = next++;
+    }
     __syncthreads();
     const int32_t it = s_item;
     if (it >= n_items) break;
@@ -521,8 +532,19 @@ __global__ void __launch_bounds__(WAVES * kWave)
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int32_t n_items = *nitems;
   uint64_t acc = 0;
+  // thread 0 claims kClaim consecutive items per atomic and hands them out one at a time
+  // through s_item. One at a time here: the clique search of a pivot varies far more than the
+  // local-graph pass does, and claims of four made K5 on R-MAT 20 18% slower.
+  constexpr int32_t kClaim = 1;
+  int32_t next = 0, end = 0;  // thread 0's claimed range
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1);
+    if (threadIdx.x == 0) {
+      if (next >= end) {
+        next = atomicAdd(cursor, kClaim);
+        end = next + kClaim;
+      }
+      s_item = next++;
+    }
     __syncthreads();
     const int32_t it = s_item;
     if (it >= n_items) break;
