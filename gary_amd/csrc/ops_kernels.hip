@@ -424,8 +424,14 @@ int gary_intersect_count(const int32_t *d_a, int64_t na, const int32_t *d_b, int
   hipStream_t st = (hipStream_t)stream;
   auto *cnt = (unsigned long long *)d_count;
   if (method == GARY_ISECT_MERGE) {
+    // four workgroups per CU, grid-striding over the tiles: fewer workgroups adding to the one
+    // counter (4 Mi x 4 Mi: 58.6 us against 67.6 us with one workgroup per tile, 66.9 us with
+    // two per CU)
+    int dev = 0, ncu = 0;
+    GARY_TRY(current_device(&dev));
+    GARY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const int64_t tiles = (na + nb + kMergeTile - 1) / kMergeTile;
-    isect_merge_kernel<<<(int)std::min<int64_t>(tiles, 8192), kBlock, 0, st>>>(
+    isect_merge_kernel<<<(int)std::min<int64_t>(tiles, std::max(1, 4 * ncu)), kBlock, 0, st>>>(
         d_a, (int32_t)na, d_b, (int32_t)nb, cnt);
   } else if (method == GARY_ISECT_BITMAP) {
     isect_bitmap_kernel<4096><<<2048, kBlock, 0, st>>>(d_a, (int32_t)na, d_b, (int32_t)nb, cnt);
