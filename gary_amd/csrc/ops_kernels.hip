@@ -146,11 +146,28 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t step = EMIT ? kBlock : (int64_t)gridDim.x * kBlock;
   for (int64_t i = EMIT ? beg + threadIdx.x : beg; i < end; i += step) {
     const int32_t key = A[i];
-    bool f;
-    if constexpr (METHOD == kIsCached) f = bsearch_2phase(B, nb, cache, n_cache, key);
-    else f = bsearch(B, nb, key);
-    c += f;
-    if constexpr (EMIT) flags[i] = f;
+    bool hit = false;
+    const int32_t *p[1] = {B};
+    int32_t n[1] = {nb};
+    if constexpr (METHOD == kIsCached) {  // phase 1 in the LDS cache
+      int32_t pos = 0;
+      if (bsearch_pos(cache, n_cache, key, pos)) {
+        hit = true;
+        n[0] = 0;
+      } else if (pos < 0) {
+        n[0] = 0;
+      } else {
+        const int32_t bottom = (int32_t)((int64_t)pos * nb / n_cache);
+        p[0] = B + bottom;
+        n[0] = (int32_t)((int64_t)(pos + 1) * nb / n_cache) - bottom;
+      }
+    }
+    const int32_t k[1] = {key};
+    bool f[1];
+    bsearch_multi<1>(p, n, k, f);  // branchless: no early exit, fewer divergent trips
+    const bool found = hit || f[0];
+    c += found;
+    if constexpr (EMIT) flags[i] = found;
   }
   if constexpr (EMIT) {
     const uint32_t s = wave_sum_u32(c);
