@@ -16,6 +16,48 @@
 
 namespace gary {
 
+// Membership of K keys, key[k] searched in the sorted range p[k][0 .. n[k]), resolved in
+// lock-step. The branchless lower bound (P.-V. Khuong and P. Morin, "Array layouts for
+// comparison-based searching", 2017) takes ceil(log2 n) steps whatever the key, so the K loads of
+// a step are independent and in flight together; the early exit of `bsearch` on an equal element
+// gains nothing when most keys are absent. found[k] is the same answer as bsearch gives.
+template <int K>
+__device__ __forceinline__ void bsearch_multi(const int32_t *const (&p)[K], const int32_t (&n)[K],
+                                              const int32_t (&key)[K], bool (&found)[K]) {
+  const int32_t *base[K];
+  int32_t len[K], most = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    base[k] = p[k];
+    len[k] = n[k];
+    most = max(most, len[k]);
+  }
+  while (most > 1) {  // invariant: the lower bound of key[k] lies in [base[k], base[k] + len[k]]
+    most = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (len[k] > 1) {
+        const int32_t h = len[k] >> 1;
+        base[k] = base[k][h] < key[k] ? base[k] + h : base[k];
+        len[k] -= h;
+      }
+      most = max(most, len[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    found[k] = false;
+    if (n[k] > 0) {  // lower bound = base, or base + 1 when *base < key
+      const int32_t v0 = base[k][0];
+      const int32_t v1 = base[k] + 1 < p[k] + n[k] ? base[k][1] : v0;
+      found[k] = v0 == key[k] || (v0 < key[k] && v1 == key[k]);
+    }
+  }
+}
+
+// Membership of key in list[0 .. size) (Search.cuh:48-61). The early exit is kept here: the
+// edge-centric strategies search short degree-oriented lists, where the branchless form
+// (bsearch_multi) measured slower (heuristic strategy on R-MAT 20: 4.77 -> 4.96 ms).
 __device__ __forceinline__ bool bsearch(const int32_t *__restrict__ list, int32_t size, int32_t key) {
   int32_t l = 0, r = size - 1;
   while (r >= l) {
@@ -72,45 +114,6 @@ __device__ __forceinline__ bool bsearch_2phase(const int32_t *__restrict__ list,
   int32_t bottom = (int32_t)((int64_t)pos * size / n_cache);
   int32_t top = (int32_t)((int64_t)(pos + 1) * size / n_cache);
   return bsearch(list + bottom, top - bottom, key);
-}
-
-// Membership of K keys, key[k] searched in the sorted range p[k][0 .. n[k]), resolved in
-// lock-step. The branchless lower bound (P.-V. Khuong and P. Morin, "Array layouts for
-// comparison-based searching", 2017) takes ceil(log2 n) steps whatever the key, so the K loads of
-// a step are independent and in flight together; the early exit of `bsearch` on an equal element
-// gains nothing when most keys are absent. found[k] is the same answer as bsearch gives.
-template <int K>
-__device__ __forceinline__ void bsearch_multi(const int32_t *const (&p)[K], const int32_t (&n)[K],
-                                              const int32_t (&key)[K], bool (&found)[K]) {
-  const int32_t *base[K];
-  int32_t len[K], most = 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    base[k] = p[k];
-    len[k] = n[k];
-    most = max(most, len[k]);
-  }
-  while (most > 1) {  // invariant: the lower bound of key[k] lies in [base[k], base[k] + len[k]]
-    most = 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      if (len[k] > 1) {
-        const int32_t h = len[k] >> 1;
-        base[k] = base[k][h] < key[k] ? base[k] + h : base[k];
-        len[k] -= h;
-      }
-      most = max(most, len[k]);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    found[k] = false;
-    if (n[k] > 0) {  // lower bound = base, or base + 1 when *base < key
-      const int32_t v0 = base[k][0];
-      const int32_t v1 = base[k] + 1 < p[k] + n[k] ? base[k][1] : v0;
-      found[k] = v0 == key[k] || (v0 < key[k] && v1 == key[k]);
-    }
-  }
 }
 
 }  // namespace gary
