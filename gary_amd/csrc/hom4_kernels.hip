@@ -569,6 +569,7 @@ __global__ void __launch_bounds__(WAVES * kWave)
                                            : row_scratch + (int64_t)blockIdx.x * scratch_words;
     for (int64_t i = threadIdx.x; i < row_words; i += kBlock) rows[i] = 0;
     posset_build(set, ps, L, d);  // begins and ends with a workgroup barrier
+    if (threadIdx.x == 0) s_item = 0;  // row counter of the count phase (u is read by now)
     for (int32_t i = wib; i < d; i += WAVES) {
       const int32_t v = L[i];
       if ((uint32_t)v >= (uint32_t)m) continue;
@@ -589,7 +590,15 @@ __global__ void __launch_bounds__(WAVES * kWave)
       }
     }
     __syncthreads();
-    for (int32_t i = wib; i < d; i += WAVES) {
+    // Rows to waves: for K >= 5 one at a time from s_item (a row's clique search varies widely:
+    // K5 on R-MAT 20 139 -> 131 ms), else statically (claims cost K4 about 1%).
+    for (int32_t r = wib;; r += WAVES) {
+      int32_t i = r;
+      if constexpr (K >= 5) {
+        if (lane == 0) i = atomicAdd(&s_item, 1);
+        i = __shfl(i, 0, kWave);
+      }
+      if (i >= d) break;
       const uint32_t S = lane < w2 ? rows[(int64_t)i * w2 + lane] : 0u;
       if constexpr (K == 3) {
         acc += (uint32_t)__popc(S);
