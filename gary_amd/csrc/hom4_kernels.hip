@@ -968,19 +968,22 @@ __global__ void __launch_bounds__(kC4Block)
       nx[i] = c < pv + dv ? col[c] : INT32_MAX;
     }
     __syncthreads();
+    // neighbours per wave: 64, or for fewer than kC4Block neighbours an even share, so every
+    // wave of the (one per CU) workgroup has loads in flight
+    const int32_t per = nl < kC4Block ? (nl + kC4Waves - 1) / kC4Waves : kWave;
     const bool packed = nl < 65536;  // counts fit 16 bits
     const int32_t span = packed ? kC4Win : kC4Words;
     for (int32_t k = k0 * (kC4Win / span); k < k1 * (kC4Win / span); ++k) {
       const int32_t wlo = k * span;
       if (wlo >= u) break;
       const int32_t whi = min(wlo + span, u);
-      for (int32_t i0 = wib * kWave; i0 < nl; i0 += kC4Block) {
+      for (int32_t i0 = wib * per; i0 < nl; i0 += kC4Waves * per) {
         const int32_t i = i0 + lane;
         int64_t p = 0;
         int32_t len = 0;
         // endpoints w in [wlo, whi): a prefix of N(v) from the cursor. nx[i] caches the next
         // endpoint, so a neighbour with none in this window costs one 4-byte load.
-        if (i < nl && nx[i] < whi) {
+        if (lane < per && i < nl && nx[i] < whi) {
           p = cur[2 * i];
           const int64_t vend = cur[2 * i + 1];
           const int32_t e1 = p + 1 < vend ? col[p + 1] : INT32_MAX;
