@@ -77,6 +77,9 @@ _SIGS = {
     "gary_memcpy_d2h": (C.c_int, [C.c_int, _P, _P, C.c_size_t]),
     "gary_memcpy_d2d": (C.c_int, [C.c_int, _P, _P, C.c_size_t]),
     "gary_memset": (C.c_int, [C.c_int, _P, C.c_int, C.c_size_t]),
+    "gary_stream_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "gary_stream_destroy": (C.c_int, [C.c_int, _P]),
+    "gary_stream_synchronize": (C.c_int, [C.c_int, _P]),
 }
 
 

@@ -194,14 +194,17 @@ int gary_batch_create(int device, int32_t ngraphs, const int32_t *graph_vertices
 }
 
 int gary_batch_destroy(gary_batch *b) {
+  GARY_API_BEGIN
   if (!b) return GARY_OK;
   GARY_TRY(use_device(b->device));
   release(b);
   delete b;
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_batch_set_cfg(gary_batch *b, const gary_launch_cfg *cfg) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(b, "gary_batch_set_cfg: null batch");
   gary_launch_cfg c = cfg ? *cfg : gary_launch_cfg{};
   const int bs = c.bin_small ? c.bin_small : gary::kDefaultBinSmall, bh = c.bin_hub ? c.bin_hub : 2048;
@@ -220,35 +223,44 @@ int gary_batch_set_cfg(gary_batch *b, const gary_launch_cfg *cfg) {
   }
   b->cfg = c;
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_batch_device_graph(gary_batch *b, gary_device_graph *out) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(b && out, "gary_batch_device_graph: null pointer");
   *out = view(b);
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_batch_tc_launch(gary_batch *b, int use_graph, void *stream) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(b, "gary_batch_tc_launch: null batch");
   GARY_TRY(use_device(b->device));
   if (!use_graph) return enqueue(b, (hipStream_t)stream);
   GARY_TRY(capture(b));
   GARY_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_batch_counts(gary_batch *b, uint64_t *h_counts, void *stream) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(b && h_counts, "gary_batch_counts: null pointer");
   GARY_TRY(use_device(b->device));
   GARY_HIP(hipMemcpyAsync(h_counts, b->gcount, sizeof(uint64_t) * (size_t)b->ngraphs,
                           hipMemcpyDeviceToHost, (hipStream_t)stream));
   GARY_HIP(hipStreamSynchronize((hipStream_t)stream));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_batch_tc_count(gary_batch *b, uint64_t *h_counts) {
+  GARY_API_BEGIN
   GARY_TRY(gary_batch_tc_launch(b, 1, nullptr));
   return gary_batch_counts(b, h_counts, nullptr);
+  GARY_API_END
 }
 
 }  // extern "C"

@@ -43,6 +43,7 @@ const char *gary_last_error(void) { return gary::last_error(); }
 int gary_version(void) { return 100; }
 
 int gary_device_count(int *count) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(count, "gary_device_count: null");
   int c = 0;
   hipError_t e = hipGetDeviceCount(&c);
@@ -52,60 +53,103 @@ int gary_device_count(int *count) {
   }
   *count = c;
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_device_synchronize(int device) {
+  GARY_API_BEGIN
   GARY_TRY(use_device(device));
   GARY_HIP(hipDeviceSynchronize());
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_malloc(int device, size_t bytes, void **ptr) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(ptr, "gary_malloc: null out pointer");
   GARY_TRY(use_device(device));
   *ptr = nullptr;
   if (bytes == 0) return GARY_OK;
   GARY_HIP(hipMalloc(ptr, bytes));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_free(int device, void *ptr) {
+  GARY_API_BEGIN
   if (!ptr) return GARY_OK;
   GARY_TRY(use_device(device));
   GARY_HIP(hipFree(ptr));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_memcpy_h2d(int device, void *dst, const void *src, size_t bytes) {
+  GARY_API_BEGIN
   if (bytes == 0) return GARY_OK;
   GARY_CHECK_ARG(dst && src, "gary_memcpy_h2d: null pointer");
   GARY_TRY(use_device(device));
   GARY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_memcpy_d2h(int device, void *dst, const void *src, size_t bytes) {
+  GARY_API_BEGIN
   if (bytes == 0) return GARY_OK;
   GARY_CHECK_ARG(dst && src, "gary_memcpy_d2h: null pointer");
   GARY_TRY(use_device(device));
   GARY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_memcpy_d2d(int device, void *dst, const void *src, size_t bytes) {
+  GARY_API_BEGIN
   if (bytes == 0) return GARY_OK;
   GARY_CHECK_ARG(dst && src, "gary_memcpy_d2d: null pointer");
   GARY_TRY(use_device(device));
   GARY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_memset(int device, void *dst, int value, size_t bytes) {
+  GARY_API_BEGIN
   if (bytes == 0) return GARY_OK;
   GARY_CHECK_ARG(dst, "gary_memset: null pointer");
   GARY_TRY(use_device(device));
   GARY_HIP(hipMemset(dst, value, bytes));
   return GARY_OK;
+  GARY_API_END
+}
+
+int gary_stream_create(int device, void **stream) {
+  GARY_API_BEGIN
+  GARY_CHECK_ARG(stream, "gary_stream_create: null out pointer");
+  GARY_TRY(use_device(device));
+  hipStream_t s = nullptr;
+  GARY_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void *)s;
+  return GARY_OK;
+  GARY_API_END
+}
+
+int gary_stream_destroy(int device, void *stream) {
+  GARY_API_BEGIN
+  if (!stream) return GARY_OK;
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipStreamDestroy((hipStream_t)stream));
+  return GARY_OK;
+  GARY_API_END
+}
+
+int gary_stream_synchronize(int device, void *stream) {
+  GARY_API_BEGIN
+  GARY_TRY(use_device(device));
+  GARY_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return GARY_OK;
+  GARY_API_END
 }
 
 }  // extern "C"
@@ -196,6 +240,7 @@ int timing_end(void *stream) {
 extern "C" {
 
 int gary_kernel_timing(int enable) {
+  GARY_API_BEGIN
   std::lock_guard<std::mutex> lock(gary::g_tm_mu);
   for (auto &p : *gary::g_tm_pairs) {
     gary::g_tm_pool->push_back(p.first);
@@ -204,9 +249,11 @@ int gary_kernel_timing(int enable) {
   gary::g_tm_pairs->clear();
   gary::g_tm_on = enable != 0;
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_kernel_timing_read(int *launches, float *total_ms) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(launches && total_ms, "gary_kernel_timing_read: null pointer");
   std::lock_guard<std::mutex> lock(gary::g_tm_mu);
   float tot = 0;
@@ -222,6 +269,7 @@ int gary_kernel_timing_read(int *launches, float *total_ms) {
   *total_ms = tot;
   gary::g_tm_pairs->clear();
   return GARY_OK;
+  GARY_API_END
 }
 
 }  // extern "C"
@@ -234,6 +282,7 @@ struct gary_timer {
 extern "C" {
 
 int gary_timer_create(gary_timer **out) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(out, "gary_timer_create: null out");
   auto *t = new (std::nothrow) gary_timer();
   if (!t) return gary::set_error(GARY_ERR_OOM, "gary_timer_create: out of memory");
@@ -247,28 +296,35 @@ int gary_timer_create(gary_timer **out) {
   }
   *out = t;
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_timer_destroy(gary_timer *t) {
+  GARY_API_BEGIN
   if (!t) return GARY_OK;
   (void)hipEventDestroy(t->start);
   (void)hipEventDestroy(t->stop);
   delete t;
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_timer_start(gary_timer *t, void *stream) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(t, "gary_timer_start: null timer");
   GARY_HIP(hipEventRecord(t->start, (hipStream_t)stream));
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_timer_stop(gary_timer *t, void *stream, float *ms) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(t && ms, "gary_timer_stop: null pointer");
   GARY_HIP(hipEventRecord(t->stop, (hipStream_t)stream));
   GARY_HIP(hipEventSynchronize(t->stop));
   GARY_HIP(hipEventElapsedTime(ms, t->start, t->stop));
   return GARY_OK;
+  GARY_API_END
 }
 
 }  // extern "C"

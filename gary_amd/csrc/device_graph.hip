@@ -122,18 +122,22 @@ int gary_ctx_create(int device, gary_ctx **out) {
 }
 
 int gary_ctx_destroy(gary_ctx *ctx) {
+  GARY_API_BEGIN
   if (!ctx) return GARY_OK;
   GARY_TRY(use_device(ctx->device));
   release(ctx);
   delete ctx;
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_ctx_clear(gary_ctx *ctx) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(ctx, "gary_ctx_clear: null ctx");
   GARY_TRY(use_device(ctx->device));
   release(ctx);
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_ctx_load_csr(gary_ctx *ctx, int32_t m, int64_t nnz, const int64_t *indPtr,
@@ -178,6 +182,7 @@ int gary_ctx_load_csr(gary_ctx *ctx, int32_t m, int64_t nnz, const int64_t *indP
 }
 
 int gary_ctx_populate(gary_ctx *ctx) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(ctx, "gary_ctx_populate: null ctx");
   if (!ctx->loaded) return gary::set_error(GARY_ERR_STATE, "DeviceGraphCtx is not initialized");
   GARY_TRY(use_device(ctx->device));
@@ -190,17 +195,21 @@ int gary_ctx_populate(gary_ctx *ctx) {
     GARY_HIP(hipDeviceSynchronize());
   }
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_ctx_device_graph(gary_ctx *ctx, gary_device_graph *out) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(ctx && out, "gary_ctx_device_graph: null pointer");
   if (!ctx->loaded) return gary::set_error(GARY_ERR_STATE, "DeviceGraphCtx is not initialized");
   out->csr = gary_device_csr{ctx->m, ctx->nnz, ctx->indPtr, ctx->colIdx, ctx->deg};
   out->coo = gary_device_coo{ctx->m, ctx->nnz, ctx->rowIdx, ctx->colIdx, ctx->deg};
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_ctx_copy_to_host(gary_ctx *ctx, int64_t *indPtr, int32_t *colIdx, int32_t *deg) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(ctx, "gary_ctx_copy_to_host: null ctx");
   if (!ctx->loaded) return gary::set_error(GARY_ERR_STATE, "DeviceGraphCtx is not initialized");
   GARY_TRY(use_device(ctx->device));
@@ -213,6 +222,7 @@ int gary_ctx_copy_to_host(gary_ctx *ctx, int64_t *indPtr, int32_t *colIdx, int32
   if (deg && ctx->m > 0)
     GARY_HIP(hipMemcpy(deg, ctx->deg, sizeof(int32_t) * (size_t)ctx->m, hipMemcpyDeviceToHost));
   return GARY_OK;
+  GARY_API_END
 }
 
 }  // extern "C"

@@ -422,6 +422,7 @@ extern "C" {
 
 int gary_search(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64_t nkeys,
                 uint8_t *d_results, int mode, int cache_depth, void *stream) {
+  GARY_API_BEGIN
   using namespace gary;
   GARY_TRY(check_search_args(len, nkeys, mode, cache_depth));
   if (nkeys == 0) return GARY_OK;
@@ -438,10 +439,12 @@ int gary_search(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64
                        cache_depth, nullptr, st);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_search_count(const int32_t *d_list, int64_t len, const int32_t *d_keys, int64_t nkeys,
                       uint64_t *d_count, int mode, int cache_depth, void *stream) {
+  GARY_API_BEGIN
   using namespace gary;
   GARY_TRY(check_search_args(len, nkeys, mode, cache_depth));
   GARY_CHECK_ARG(d_count, "gary_search_count: null counter");
@@ -452,11 +455,13 @@ int gary_search_count(const int32_t *d_list, int64_t len, const int32_t *d_keys,
                       (unsigned long long *)d_count, st);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_search_batched(int32_t batch, const int32_t *const *d_lists, const int64_t *d_list_sizes,
                         const int32_t *const *d_keys, const int64_t *d_key_sizes,
                         uint8_t *const *d_results, int mode, int cache_depth, void *stream) {
+  GARY_API_BEGIN
   using namespace gary;
   GARY_TRY(check_search_args(0, 0, mode, cache_depth));
   GARY_CHECK_ARG(batch >= 0, "gary_search_batched: batch=%d", batch);
@@ -473,10 +478,12 @@ int gary_search_batched(int32_t batch, const int32_t *const *d_lists, const int6
                                                           d_key_sizes, d_results, 0);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_intersect_count(const int32_t *d_a, int64_t na, const int32_t *d_b, int64_t nb,
                          int method, int cache_depth, uint64_t *d_count, void *stream) {
+  GARY_API_BEGIN
   using namespace gary;
   GARY_CHECK_ARG(d_count, "gary_intersect_count: null counter");
   GARY_CHECK_ARG(na >= 0 && nb >= 0 && na <= INT32_MAX && nb <= INT32_MAX,
@@ -514,11 +521,13 @@ int gary_intersect_count(const int32_t *d_a, int64_t na, const int32_t *d_b, int
   }
   GARY_HIP(hipGetLastError());
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_intersect_emit(const int32_t *d_a, int64_t na, const int32_t *d_b, int64_t nb,
                         int method, int cache_depth, int32_t *d_out, uint64_t *d_count,
                         void *stream) {
+  GARY_API_BEGIN
   using namespace gary;
   GARY_CHECK_ARG(d_count, "gary_intersect_emit: null counter");
   GARY_CHECK_ARG(na >= 0 && nb >= 0 && na <= INT32_MAX && nb <= INT32_MAX,
@@ -557,6 +566,7 @@ int gary_intersect_emit(const int32_t *d_a, int64_t na, const int32_t *d_b, int6
   compact_kernel<<<grid, kBlock, 0, st>>>(d_a, (int32_t)na, flags, bcounts, d_out);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
+  GARY_API_END
 }
 
 }  // extern "C"

@@ -21,8 +21,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <sstream>
+#include <string>
 #include <mutex>
 #include <vector>
 
@@ -474,7 +478,7 @@ struct Model {
 
 struct DeviceState {
   RouteTable *heuristic = nullptr;  // device
-  RouteTable *scratch = nullptr;    // device: learned / caller tables
+  RouteTable *sync_table = nullptr; // device: output of the synchronous learned-table call
   BinSums *sums = nullptr;          // device
   Model model;
 };
@@ -512,7 +516,7 @@ int state_for(int dev, DeviceState **out) {
     RouteTable h;
     heuristic_table_host(h);
     GARY_HIP(hipMalloc(&s.heuristic, sizeof(RouteTable)));
-    GARY_HIP(hipMalloc(&s.scratch, sizeof(RouteTable)));
+    GARY_HIP(hipMalloc(&s.sync_table, sizeof(RouteTable)));
     GARY_HIP(hipMalloc(&s.sums, sizeof(BinSums) * kRouteBins * kSumCopies));
     GARY_HIP(hipMemcpy(s.heuristic, &h, sizeof h, hipMemcpyHostToDevice));
   }
@@ -568,59 +572,149 @@ int compute_bin_sums(const gary_device_graph &g, BinSums *sums, hipStream_t st) 
 
 }  // namespace
 
-int launch_routed(const gary_device_graph &g, const RouteTable *table, const PivotCfg &pc,
-                  float cost_big, unsigned long long *cnt, hipStream_t st) {
-  const int32_t m = g.csr.m;
-  if (m == 0 || g.csr.nnz == 0) return GARY_OK;
-  GARY_CHECK_ARG(g.coo.rowIdx, "routed counting needs the COO rowIdx (call populate first)");
+namespace {
+
+// Per-(device, stream) buffers of one routed count: per-arc route bytes, per-pivot predicted
+// costs, the pivot lists and counters, the compacted edge-route arc lists and their counts, and
+// the route table and bin sums of the call. Keeping the table and the sums here (not per device)
+// lets counts on different streams of one device run at the same time.
+struct RoutedWs {
+  uint8_t *route = nullptr;
+  float *pcost = nullptr;
+  int32_t *lists = nullptr;
+  uint32_t *cached = nullptr, *merge = nullptr;
+  int32_t *counts = nullptr;
+  RouteTable *table = nullptr;
+  BinSums *sums = nullptr;
+  size_t stride = 0;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int routed_ws(const gary_device_graph &g, hipStream_t st, RoutedWs &w) {
   int dev = 0;
   GARY_TRY(current_device(&dev));
-  GARY_CHECK_ARG(g.csr.nnz <= (int64_t)UINT32_MAX, "routed counting: more than 2^32-1 arcs");
-  const size_t stride = ((size_t)m + 63) & ~(size_t)63;
-  const size_t b_route = ((size_t)g.csr.nnz + 255) & ~(size_t)255;
-  const size_t b_pcost = sizeof(float) * stride;
-  const size_t b_lists = sizeof(int32_t) * (3 * stride + kPivotCounterWords);
-  const size_t b_arcs = (sizeof(uint32_t) * (size_t)g.csr.nnz + 255) & ~(size_t)255;
+  const int64_t m = std::max<int64_t>(g.csr.m, 0), nnz = std::max<int64_t>(g.csr.nnz, 0);
+  w.stride = ((size_t)m + 63) & ~(size_t)63;
+  const size_t b_route = align256((size_t)nnz);
+  const size_t b_pcost = align256(sizeof(float) * w.stride);
+  const size_t b_lists = align256(sizeof(int32_t) * (3 * w.stride + kPivotCounterWords));
+  const size_t b_arcs = align256(sizeof(uint32_t) * (size_t)nnz);
+  const size_t b_counts = 256;
+  const size_t b_table = align256(sizeof(RouteTable));
+  const size_t b_sums = align256(sizeof(BinSums) * kRouteBins * kSumCopies);
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)((uintptr_t)st ^ 0x5E1Eull),
-                         b_route + b_pcost + b_lists + 2 * b_arcs + 256, &ws));
+                         b_route + b_pcost + b_lists + 2 * b_arcs + b_counts + b_table + b_sums,
+                         &ws));
   char *p = (char *)ws;
-  uint8_t *route = (uint8_t *)p; p += b_route;
-  float *pcost = (float *)p; p += b_pcost;
-  int32_t *lists = (int32_t *)p; p += b_lists;
-  uint32_t *cached = (uint32_t *)p; p += b_arcs;
-  uint32_t *merge = (uint32_t *)p; p += b_arcs;
-  int32_t *counts = (int32_t *)p;
-  fill_zero_kernel_sel<<<1, 64, 0, st>>>((uint32_t *)counts, 2);
+  w.route = (uint8_t *)p; p += b_route;
+  w.pcost = (float *)p; p += b_pcost;
+  w.lists = (int32_t *)p; p += b_lists;
+  w.cached = (uint32_t *)p; p += b_arcs;
+  w.merge = (uint32_t *)p; p += b_arcs;
+  w.counts = (int32_t *)p; p += b_counts;
+  w.table = (RouteTable *)p; p += b_table;
+  w.sums = (BinSums *)p;
+  return GARY_OK;
+}
+
+// Counts with every arc routed by `table` (device pointer, read by the kernels of this call).
+int launch_routed_ws(const gary_device_graph &g, const RoutedWs &w, const RouteTable *table,
+                     const PivotCfg &pc, float cost_big, unsigned long long *cnt, hipStream_t st) {
+  const int32_t m = g.csr.m;
+  fill_zero_kernel_sel<<<1, 64, 0, st>>>((uint32_t *)w.counts, 2);
   const bool by_cost = cost_big > 0.f;
   if (by_cost)
     fill_zero_kernel_sel<<<(int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
-        (uint32_t *)pcost, m);
+        (uint32_t *)w.pcost, m);
   route_kernel<<<(int)((g.csr.nnz + kRouteChunk - 1) / kRouteChunk), 256, 0, st>>>(
-      g, table, route, by_cost ? pcost : nullptr, cached, merge, counts);
+      g, table, w.route, by_cost ? w.pcost : nullptr, w.cached, w.merge, w.counts);
   GARY_HIP(hipGetLastError());
-  GARY_TRY(launch_pivot_core(g, pc, lists, stride, lists + 3 * stride, nullptr, nullptr, 0, cnt,
-                             st, true, route, by_cost ? pcost : nullptr, cost_big, table));
-  return launch_edge_lists(g, cached, counts, merge, counts + 1, cnt, st);
+  GARY_TRY(launch_pivot_core(g, pc, w.lists, w.stride, w.lists + 3 * w.stride, nullptr, nullptr,
+                             0, cnt, st, true, w.route, by_cost ? w.pcost : nullptr, cost_big,
+                             table));
+  return launch_edge_lists(g, w.cached, w.counts, w.merge, w.counts + 1, cnt, st);
 }
 
-// HEURISTIC and AUTO_LEARNED strategies of gary_tc_count.
+// The numbers of the value of top-level "key" in a selector file (a number, or an array of
+// numbers, nested arrays flattened in order). The file is the JSON object that
+// gary_amd/selector.py writes: "params", "cost_big", "allowed", "margin", "fixed_nnz",
+// "fixed_costs" come before any nested report, so the first occurrence of a key is the top-level
+// one. Returns false when the key is absent or its value holds no number.
+bool selector_file_numbers(const std::string &doc, const char *key, std::vector<double> &out) {
+  out.clear();
+  const std::string k = std::string("\"") + key + "\"";
+  size_t at = doc.find(k);
+  if (at == std::string::npos) return false;
+  at = doc.find_first_not_of(" \t\r\n", at + k.size());
+  if (at == std::string::npos || doc[at] != ':') return false;
+  at = doc.find_first_not_of(" \t\r\n", at + 1);
+  if (at == std::string::npos) return false;
+  int depth = 0;
+  do {
+    const char c = doc[at];
+    if (c == '[') {
+      ++depth;
+      ++at;
+    } else if (c == ']') {
+      --depth;
+      ++at;
+    } else if (c == '-' || c == '+' || c == '.' || (c >= '0' && c <= '9')) {
+      char *end = nullptr;
+      const double v = std::strtod(doc.c_str() + at, &end);
+      if (end == doc.c_str() + at) return false;
+      out.push_back(v);
+      at = (size_t)(end - doc.c_str());
+    } else if (c == ',' || c == ' ' || c == '\t' || c == '\r' || c == '\n') {
+      ++at;
+    } else {
+      return false;  // not a number or an array of numbers
+    }
+  } while (depth > 0 && at < doc.size());
+  return depth == 0 && !out.empty();
+}
+
+int check_routed(const gary_device_graph &g) {
+  GARY_CHECK_ARG(g.coo.rowIdx, "routed counting needs the COO rowIdx (call populate first)");
+  GARY_CHECK_ARG(g.csr.nnz <= (int64_t)UINT32_MAX, "routed counting: more than 2^32-1 arcs");
+  return GARY_OK;
+}
+
+}  // namespace
+
+int launch_routed(const gary_device_graph &g, const RouteTable *table, const PivotCfg &pc,
+                  float cost_big, unsigned long long *cnt, hipStream_t st) {
+  if (g.csr.m == 0 || g.csr.nnz == 0) return GARY_OK;
+  GARY_TRY(check_routed(g));
+  RoutedWs w;
+  GARY_TRY(routed_ws(g, st, w));
+  return launch_routed_ws(g, w, table, pc, cost_big, cnt, st);
+}
+
+// HEURISTIC and AUTO_LEARNED strategies of gary_tc_count. The learned table and its bin sums
+// live in the stream's workspace; only the model parameters and the heuristic table are per
+// device (both change only through the selector's set-up calls).
 int launch_selected(const gary_device_graph &g, bool learned, const PivotCfg &pc,
                     unsigned long long *cnt, hipStream_t st) {
+  if (g.csr.m == 0 || g.csr.nnz == 0) return GARY_OK;
+  GARY_TRY(check_routed(g));
   int dev = 0;
   GARY_TRY(current_device(&dev));
   std::lock_guard<std::mutex> lock(g_mu);
   DeviceState *s = nullptr;
   GARY_TRY(state_for(dev, &s));
+  RoutedWs w;
+  GARY_TRY(routed_ws(g, st, w));
   if (!learned || !s->model.trained)
-    return launch_routed(g, s->heuristic, pc, 0.f, cnt, st);
-  GARY_TRY(compute_bin_sums(g, s->sums, st));
-  mlp_table_kernel<<<1, kRouteBins, 0, st>>>(s->model.P, s->sums,
+    return launch_routed_ws(g, w, s->heuristic, pc, 0.f, cnt, st);
+  GARY_TRY(compute_bin_sums(g, w.sums, st));
+  mlp_table_kernel<<<1, kRouteBins, 0, st>>>(s->model.P, w.sums,
                                              log2f(1.f + (float)g.csr.nnz), (float)g.csr.nnz,
                                              s->model.allowed, fixed_of(s->model),
-                                             s->model.margin, s->scratch);
+                                             s->model.margin, w.table);
   GARY_HIP(hipGetLastError());
-  return launch_routed(g, s->scratch, pc, s->model.cost_big, cnt, st);
+  return launch_routed_ws(g, w, w.table, pc, s->model.cost_big, cnt, st);
 }
 
 }  // namespace gary
@@ -643,29 +737,30 @@ int gary_tc_count_routed(const gary_device_graph *g, const uint8_t *h_routes,
   const int bs = cfg && cfg->bin_small ? cfg->bin_small : kDefaultBinSmall;
   const int bh = cfg && cfg->bin_hub ? cfg->bin_hub : 2048;
   const int grid = cfg ? cfg->grid : 0;
-  int dev = 0;
-  GARY_TRY(current_device(&dev));
-  std::lock_guard<std::mutex> lock(g_mu);
-  DeviceState *s = nullptr;
-  GARY_TRY(state_for(dev, &s));
   hipStream_t st = (hipStream_t)stream;
-  GARY_HIP(hipMemcpyAsync(s->scratch, &t, sizeof t, hipMemcpyHostToDevice, st));
+  if (g->csr.m == 0 || g->csr.nnz == 0) return GARY_OK;
+  GARY_TRY(check_routed(*g));
+  RoutedWs w;
+  GARY_TRY(routed_ws(*g, st, w));
+  GARY_HIP(hipMemcpyAsync(w.table, &t, sizeof t, hipMemcpyHostToDevice, st));
   GARY_HIP(hipStreamSynchronize(st));  // `t` lives on this stack frame
   PivotCfg pc;
   pc.bin_small = bs;
   pc.bin_hub = bh;
   pc.grid = grid;
   if (cfg && cfg->block) pc.waves = cfg->block == 128 ? 2 : cfg->block == 512 ? 8 : 4;
-  return launch_routed(*g, s->scratch, pc, cost_big, (unsigned long long *)d_count, st);
+  return launch_routed_ws(*g, w, w.table, pc, cost_big, (unsigned long long *)d_count, st);
   GARY_API_END
 }
 
 int gary_selector_heuristic_table(uint8_t *h_routes) {
+  GARY_API_BEGIN
   GARY_CHECK_ARG(h_routes, "gary_selector_heuristic_table: null pointer");
   gary::RouteTable t;
   gary::heuristic_table_host(t);
   std::memcpy(h_routes, t.route, gary::kRouteBins);
   return GARY_OK;
+  GARY_API_END
 }
 
 int gary_selector_bin_features(const gary_device_graph *g, float *h_features, float *h_arcs) {
@@ -733,6 +828,51 @@ int gary_selector_set_fixed_costs(int32_t npoints, const float *h_nnz, const flo
     for (int o = 0; o < kOut; ++o) s->model.fixed_f[k][o] = h_costs[k * kOut + o];
   }
   return GARY_OK;
+  GARY_API_END
+}
+
+int gary_selector_trained(int *trained) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(trained, "gary_selector_trained: null pointer");
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lock(g_mu);
+  DeviceState *s = nullptr;
+  GARY_TRY(state_for(dev, &s));
+  *trained = s->model.trained ? 1 : 0;
+  return GARY_OK;
+  GARY_API_END
+}
+
+int gary_selector_load(const char *path) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(path, "gary_selector_load: null path");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return set_error(GARY_ERR_INVALID, "gary_selector_load: cannot open %s", path);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string doc = ss.str();
+  std::vector<double> params, cost_big, allowed, margin, fixed_nnz, fixed_costs;
+  if (!selector_file_numbers(doc, "params", params) || params.size() != (size_t)kParams)
+    return set_error(GARY_ERR_INVALID, "gary_selector_load: %s has no \"params\" array of %d "
+                     "numbers", path, kParams);
+  selector_file_numbers(doc, "cost_big", cost_big);
+  selector_file_numbers(doc, "allowed", allowed);
+  selector_file_numbers(doc, "margin", margin);
+  selector_file_numbers(doc, "fixed_nnz", fixed_nnz);
+  selector_file_numbers(doc, "fixed_costs", fixed_costs);
+  GARY_CHECK_ARG(fixed_nnz.size() <= 4 && fixed_costs.size() == fixed_nnz.size() * kOut,
+                 "gary_selector_load: %s: %zu fixed-cost points with %zu costs", path,
+                 fixed_nnz.size(), fixed_costs.size());
+  std::vector<float> p(params.begin(), params.end());
+  std::vector<float> fx(fixed_nnz.begin(), fixed_nnz.end()), fc(fixed_costs.begin(), fixed_costs.end());
+  GARY_TRY(gary_selector_set_params(p.data(), 1, cost_big.empty() ? 0.f : (float)cost_big[0],
+                                    allowed.empty() ? 7u : (uint32_t)allowed[0],
+                                    margin.empty() ? 0.f : (float)margin[0]));
+  return gary_selector_set_fixed_costs((int32_t)fx.size(), fx.empty() ? nullptr : fx.data(),
+                                       fc.empty() ? nullptr : fc.data());
   GARY_API_END
 }
 
@@ -875,10 +1015,10 @@ int gary_selector_learned_table(const gary_device_graph *g, uint8_t *h_routes, f
   GARY_TRY(compute_bin_sums(*g, s->sums, nullptr));
   mlp_table_kernel<<<1, kRouteBins>>>(s->model.P, s->sums, log2f(1.f + (float)g->csr.nnz),
                                       (float)g->csr.nnz, s->model.allowed, fixed_of(s->model),
-                                      s->model.margin, s->scratch);
+                                      s->model.margin, s->sync_table);
   GARY_HIP(hipGetLastError());
   RouteTable t;
-  GARY_HIP(hipMemcpy(&t, s->scratch, sizeof t, hipMemcpyDeviceToHost));
+  GARY_HIP(hipMemcpy(&t, s->sync_table, sizeof t, hipMemcpyDeviceToHost));
   std::memcpy(h_routes, t.route, kRouteBins);
   if (h_costs) std::memcpy(h_costs, t.cost, sizeof(float) * kRouteBins);
   return GARY_OK;

@@ -71,6 +71,31 @@ class DeviceBuffer:
             pass
 
 
+class Stream:
+    """A non-blocking HIP stream of `device` (default priority), created by the library; pass
+    `.handle` as the stream argument of the asynchronous calls."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        h = C.c_void_p(None)
+        check(lib.gary_stream_create(device, C.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        check(lib.gary_stream_synchronize(self.device, self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.gary_stream_destroy(self.device, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceGraphCtx:
     """Owner of one graph's device buffers (DeviceGraphCtx)."""
 

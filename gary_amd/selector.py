@@ -39,6 +39,8 @@ _lib.bind("gary_selector_set_params", C.c_int, [_P, C.c_int, _F, C.c_uint32, _F]
 _lib.bind("gary_selector_get_params", C.c_int, [_P])
 _lib.bind("gary_selector_set_fixed_costs", C.c_int, [C.c_int32, _P, _P])
 _lib.bind("gary_selector_learned_table", C.c_int, [C.POINTER(DeviceGraph), _P, _P])
+_lib.bind("gary_selector_load", C.c_int, [C.c_char_p])
+_lib.bind("gary_selector_trained", C.c_int, [C.POINTER(C.c_int)])
 
 
 def _dg(g):
@@ -251,14 +253,23 @@ def save(path: Path = MODEL_PATH, fixed_nnz=(), fixed_costs=(), **meta) -> None:
     Path(path).write_text(json.dumps(doc, indent=1))
 
 
-def load(path: Path = MODEL_PATH) -> dict | None:
-    """Loads a saved model into the library (enables the learned strategy)."""
+def load(path: Path = MODEL_PATH, required: bool = True) -> dict | None:
+    """Loads a saved model into the library (gary_selector_load; enables the learned strategy)
+    and returns the file's contents. A missing file raises FileNotFoundError, or returns None
+    when required=False."""
     p = Path(path)
     if not p.exists():
+        if required:
+            raise FileNotFoundError(f"no selector model at {p}")
         return None
-    doc = json.loads(p.read_text())
-    set_params(np.array(doc["params"], np.float32), True, float(doc.get("cost_big", 0.0)),
-               int(doc.get("allowed", 7)), float(doc.get("margin", 0.0)))
-    if doc.get("fixed_nnz"):
-        set_fixed_costs(doc["fixed_nnz"], doc["fixed_costs"])
-    return doc
+    check(lib.gary_selector_load(str(p).encode()))
+    if not trained():
+        raise RuntimeError(f"{p} loaded but the learned strategy is not enabled")
+    return json.loads(p.read_text())
+
+
+def trained() -> bool:
+    """Whether GARY_TC_AUTO_LEARNED runs a model on the current device (else the heuristic)."""
+    t = C.c_int(0)
+    check(lib.gary_selector_trained(C.byref(t)))
+    return bool(t.value)

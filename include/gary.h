@@ -203,6 +203,11 @@ int gary_memcpy_h2d(int device, void *dst, const void *src, size_t bytes);
 int gary_memcpy_d2h(int device, void *dst, const void *src, size_t bytes);
 int gary_memcpy_d2d(int device, void *dst, const void *src, size_t bytes);
 int gary_memset(int device, void *dst, int value, size_t bytes);
+/* Streams for callers without a HIP toolchain: a non-blocking stream (default priority) of
+ * `device`, its destruction, and a wait for its queued work. */
+int gary_stream_create(int device, void **stream);
+int gary_stream_destroy(int device, void *stream);
+int gary_stream_synchronize(int device, void *stream);
 
 /* ---- search and intersection operators (include/ops/Search.cuh, Intersection.cuh) ------- */
 enum {
@@ -335,6 +340,14 @@ int gary_selector_train(const float *h_x, const float *h_t, const float *h_mask,
 int gary_selector_set_params(const float *h_params, int trained, float cost_big, uint32_t allowed,
                              float margin);
 int gary_selector_get_params(float *h_params);
+/* Loads a selector file (the JSON object gary_amd/selector.py writes: "params" with
+ * GARY_MLP_PARAMS numbers, optional "cost_big", "allowed", "margin", "fixed_nnz" and
+ * "fixed_costs") into the current device and enables GARY_TC_AUTO_LEARNED; a missing or malformed
+ * file fails with GARY_ERR_INVALID and leaves the model unchanged. */
+int gary_selector_load(const char *path);
+/* *trained = 1 when GARY_TC_AUTO_LEARNED runs a model on the current device, 0 when it falls
+ * back to the heuristic table. */
+int gary_selector_trained(int *trained);
 /* Fixed cost (ns) of using each route at all, as up to 4 points (h_nnz[k] increasing,
  * h_costs[k][3]) interpolated linearly in log2(1 + nnz); the learned table picks the route subset
  * with the least predicted total. */

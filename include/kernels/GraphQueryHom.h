@@ -49,6 +49,16 @@ inline void triangleCount(DeviceGraph dataGraph, Strategy s, uint64_t *count,
   gary::check(gary_tc_count(&dataGraph, (int)s, cfg, count, stream));
 }
 
+// Loads a trained selector file (gary_amd/selector_model.json) so Strategy::Learned runs its
+// model on the current device; without one, Learned runs the heuristic table.
+inline void loadSelector(const std::string &path) { gary::check(gary_selector_load(path.c_str())); }
+
+inline bool selectorTrained() {
+  int t = 0;
+  gary::check(gary_selector_trained(&t));
+  return t != 0;
+}
+
 // triangleCountLegacy(DeviceGraph, unsigned*): 32-bit counter; throws gary::Error with
 // GARY_ERR_OVERFLOW instead of wrapping past 2^32.
 inline void triangleCountLegacy(DeviceGraph dataGraph, unsigned *count, void *stream = nullptr) {

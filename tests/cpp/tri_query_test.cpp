@@ -1,6 +1,7 @@
 // tri_query_test.cpp — the reference's TriQueryTest (test/TriQueryTest.cu:39-153) restated over
 // the C++ façade: plain g++ host code, no HIP compiler, linked against libgary.so and the CPU
-// oracle. Usage: tri_query_test [rmat_scale]   (0 or absent: ER G(4096, 65536))
+// oracle. Usage: tri_query_test [rmat_scale [selector_model.json]]   (scale 0 or absent: ER
+// G(4096, 65536); the model defaults to gary_amd/selector_model.json beside this binary's tree)
 // Prints one line per case and exits non-zero on the first mismatch.
 #include <chrono>
 #include <cstdio>
@@ -103,6 +104,13 @@ int main(int argc, char **argv) {
   run("RunGPUWithCacheDynamic", [&] {
     graph_query::triangleCountHom<true, 32, dynamicPolicy>(g, counts.data(), 1536, 256);
   });
+  {  // Strategy::Learned runs the shipped model, not the heuristic fallback
+    std::string model = argc > 2 ? argv[2] : std::string(argv[0]);
+    if (argc <= 2) model = model.substr(0, model.find_last_of('/') + 1) +
+                           "../../gary_amd/selector_model.json";
+    graph_query::loadSelector(model);
+    expect("loadSelector (trained)", graph_query::selectorTrained() ? 1 : 0, 1, 0.0);
+  }
   const std::pair<const char *, graph_query::Strategy> strategies[] = {
       {"Strategy::Learned", graph_query::Strategy::Learned},
       {"Strategy::Heuristic", graph_query::Strategy::Heuristic},

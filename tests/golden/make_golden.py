@@ -1,7 +1,7 @@
 """Writes tests/golden/graphs.json: small graphs with independently computed answers.
 
-Answers come from networkx (triangles, clique enumeration), numpy integer matrix powers
-(trace(A^3)/6, trace(A^4)) and brute force over all vertex maps (homomorphism counts, n <= 8).
+Answers come from networkx (triangles, the triangle vertex sets, clique enumeration), numpy
+integer matrix powers (trace(A^3)/6, trace(A^4)) and brute force over all vertex maps (homomorphism counts, n <= 8).
 None of them uses this repository's code, so the fixtures pin the oracle and the GPU path.
 
 Run: python tests/golden/make_golden.py
@@ -51,6 +51,15 @@ def describe(name: str, n: int, edges: list[tuple[int, int]], raw=None, brute: b
     for u, v in G.edges():
         t = len(list(nx.common_neighbors(G, u, v)))
         diam += 2 * t * t
+    # every triangle as its sorted vertex triple, by networkx clique enumeration
+    tri_list = []
+    for c in nx.enumerate_all_cliques(G):  # non-decreasing size: stop past the triangles
+        if len(c) > 3:
+            break
+        if len(c) == 3:
+            tri_list.append(sorted(int(x) for x in c))
+    tri_list.sort()
+    assert len(tri_list) == tri_nx, name
     m = G.number_of_edges()
     degs = [d for _, d in G.degree()]
     c4 = (tr4 - 4 * sum(d * (d - 1) // 2 for d in degs) - 2 * m)
@@ -61,6 +70,7 @@ def describe(name: str, n: int, edges: list[tuple[int, int]], raw=None, brute: b
         "m": m, "triangles": tri_nx, "trace_a3": tr3, "k4": k4, "hom_k4": 24 * k4,
         "hom_diamond": diam, "hom_c4": tr4, "c4": c4 // 8,
         "edges_sorted": sorted([min(u, v), max(u, v)] for u, v in G.edges()),
+        "triangle_list": tri_list,
     }
     if brute:
         adj = [[bool(A[i][j]) for j in range(n)] for i in range(n)]

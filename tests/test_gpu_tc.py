@@ -2,16 +2,29 @@
 (bit-exact uint64), through the C ABI."""
 from __future__ import annotations
 
+import json
+
 import numpy as np
 import pytest
 
 import oracle
-from gary_amd import graph, graph_query as gq
-from gary_amd.device import DeviceBuffer, DeviceGraphCtx
+from gary_amd import graph, selector
+from gary_amd import graph_query as gq
+from gary_amd.device import DeviceBuffer, DeviceGraphCtx, gen_chung_lu_device
+from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
 ALL = [gq.BITMAP, gq.HEURISTIC, gq.AUTO_LEARNED, gq.CACHED_BSEARCH, gq.BSEARCH, gq.MERGE]
+TRI_FULL = json.loads((GOLDEN / "tri_full.json").read_text())
+
+
+@pytest.fixture(scope="module", autouse=True)
+def learned_model():
+    """Every AUTO_LEARNED case here runs the shipped model, whatever ran before in the process."""
+    doc = selector.load()
+    assert selector.trained()
+    return doc
 
 
 def _ctx(g: graph.GraphCSR) -> DeviceGraphCtx:
@@ -162,14 +175,64 @@ def test_ctx_roundtrip(er_ctx):
     assert np.array_equal(h.deg, g.deg)
 
 
-def test_rmat20_full(request):
-    """RMAT scale 20, edgefactor 16: GPU == oracle, bit-exact."""
-    n, s, d = graph.rmat(20, 16)
+@pytest.fixture(scope="module")
+def rmat20():
+    n, s, d = graph.rmat(20, 16, seed=graph.SEEDS["rmat20"])
     g = graph.oriented(n, s, d)
-    want = oracle.tri_count(g.indPtr, g.colIdx)
-    ctx = _ctx(g)
-    for strat in (gq.BITMAP, gq.HEURISTIC, gq.CACHED_BSEARCH):
-        assert gq.triangleCountHom(ctx, strat) == want, strat
+    fx = TRI_FULL["rmat20"]
+    assert (g.m, g.nnz, int(g.deg.max())) == (fx["n"], fx["m"], fx["max_out_degree"])
+    return g, _ctx(g), fx["triangles"]
+
+
+@pytest.mark.parametrize("strategy", ALL)
+def test_rmat20_full(rmat20, strategy):
+    """R-MAT scale 20, edgefactor 16 (the headline workload): every strategy, the learned one
+    with the shipped model, equals the oracle's count (tests/golden/tri_full.json), bit-exact."""
+    g, ctx, want = rmat20
+    assert gq.triangleCountHom(ctx, strategy) == want
+
+
+@pytest.fixture(scope="module")
+def chung_lu_4m():
+    """Chung-Lu, 4,194,304 vertices, 67,108,864 tuples, exponent 2.1, weights capped at 50,000:
+    generated and built on the device (the same tuples and CSR as the host build that wrote the
+    fixture)."""
+    fx = TRI_FULL["chung_lu_4m"]
+    n, nt = fx["n"], fx["tuples"]
+    ds, dd = gen_chung_lu_device(n, nt, 2.1, 50_000.0, graph.SEEDS["chung_lu"])
+    ctx = DeviceGraphCtx()
+    ctx.buildFromEdges(n, ds, dd, nt, graph.ORIENT_DEGREE)
+    del ds, dd
+    ctx.populateDevice()
+    dg = ctx.deviceGraph()
+    assert (dg.csr.m, dg.csr.nnz) == (n, fx["m"])
+    return ctx, fx["triangles"]
+
+
+@pytest.mark.parametrize("strategy", [gq.AUTO_LEARNED, gq.HEURISTIC, gq.BITMAP,
+                                      gq.CACHED_BSEARCH])
+def test_chung_lu_4m(chung_lu_4m, strategy):
+    """The skewed Chung-Lu workload: the learned selector, the plain heuristic, the bitmap and the
+    reference's cached two-phase search all give the oracle's count."""
+    ctx, want = chung_lu_4m
+    assert gq.triangleCountHom(ctx, strategy) == want
+
+
+def test_learned_table_is_the_models(rmat20):
+    """AUTO_LEARNED reads the loaded model: its table on R-MAT 20 is the fused-inference table,
+    and a model that forbids the bitmap route yields a table without it."""
+    g, ctx, want = rmat20
+    doc = selector.load()
+    routes, _ = selector.learned_table(ctx)
+    assert set(np.unique(routes)) <= {selector.BITMAP, selector.CACHED, selector.MERGE}
+    try:
+        selector.set_params(np.array(doc["params"], np.float32), True, 0.0, allowed=6,
+                            margin=0.0)
+        routes, _ = selector.learned_table(ctx)
+        assert selector.BITMAP not in set(np.unique(routes))
+        assert gq.triangleCountHom(ctx, gq.AUTO_LEARNED) == want
+    finally:
+        selector.load()
 
 
 def test_hub_fallback_star():

@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 from gary_amd import _lib, graph
+from gary_amd import graph_query as gq
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -106,3 +107,13 @@ def test_ranked_symmetric_properties(golden_graphs):
         h = O.hom4(g.indPtr, g.colIdx)
         assert h["hom_c4"] == rec["hom_c4"] and h["hom_diamond"] == rec["hom_diamond"]
         assert h["k4"] == rec["k4"] and h["tri"] == rec["triangles"]
+
+
+def test_index_cache_smem():
+    """getIndexCacheSMem (src/GraphQueryHom.cu:11-13): waves per block x cache size, in elements,
+    with 64-lane waves (256 threads, N = 32 -> 4 x 32 = 128; the reference's 8 warps give 256)."""
+    assert gq.getIndexCacheSMem(256, 32) == 128
+    assert gq.getIndexCacheSMem(64, 64) == 64
+    assert gq.getIndexCacheSMem(1024, 1) == 16
+    assert gq.getIndexCacheSMem(512, 0) == 0
+    assert gq.getIndexCacheSMem(0, 32) == 0

@@ -214,6 +214,84 @@ def test_oracle_plants_fail():
     assert oracle.tri_count(indPtr, ci) == t - 2
 
 
+# ---- triangle listing and per-range counts -----------------------------------------------------
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_tri_list_matches_networkx_sets(golden_graphs, mode):
+    """or_tri_list (the emit variants of include/ops/Intersection.cuh:18-50,76-112 applied to the
+    triangle query) lists each triangle of the networkx fixture exactly once, as (u, v, w) with
+    v in N+(u) and w in N+(u) and N+(v)."""
+    for rec in golden_graphs:
+        n = rec["n"]
+        src, dst = np.array(rec["src"], np.int32), np.array(rec["dst"], np.int32)
+        ip, ci = oracle.csr_from_edges(n, src, dst, mode=mode)
+        t = oracle.tri_list(ip, ci)
+        assert t.shape == (rec["triangles"], 3), rec["name"]
+        got = sorted(sorted(r) for r in t.tolist())
+        assert got == rec["triangle_list"], rec["name"]
+        arcs = {(u, int(v)) for u in range(n) for v in ci[ip[u]:ip[u + 1]]}
+        for u, v, w in t.tolist():
+            assert (u, v) in arcs and (u, w) in arcs and (v, w) in arcs, rec["name"]
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_tri_list_checksum_matches_networkx_sets(golden_graphs, mode):
+    """The order-independent checksum equals the sum of or_triple_hash over the fixture's
+    triangles, each put in orientation order (lowest, middle, highest): with degree orientation
+    by (degree, id), with id orientation by decreasing id (u -> v iff u > v)."""
+    for rec in golden_graphs:
+        n = rec["n"]
+        src, dst = np.array(rec["src"], np.int32), np.array(rec["dst"], np.int32)
+        ip, ci = oracle.csr_from_edges(n, src, dst, mode=mode)
+        deg = np.zeros(n, np.int64)
+        for a, b in rec["edges_sorted"]:
+            deg[a] += 1
+            deg[b] += 1
+        key = (lambda v: (int(deg[v]), v)) if mode == 0 else (lambda v: -v)
+        want = 0
+        for tri in rec["triangle_list"]:
+            u, v, w = sorted(tri, key=key)
+            want = (want + int(oracle.lib.or_triple_hash(u, v, w))) % (1 << 64)
+        assert oracle.tri_list_checksum(ip, ci) == want, rec["name"]
+
+
+def test_tri_list_checksum_detects_a_changed_triple():
+    """The checksum is not blind to which triangles are listed: moving one arc of K4 changes it."""
+    ip, ci = oracle.csr_from_edges(5, np.array([0, 0, 0, 1, 1, 2]), np.array([1, 2, 3, 2, 3, 3]), 2)
+    c0 = oracle.tri_list_checksum(ip, ci)
+    ip2, ci2 = oracle.csr_from_edges(5, np.array([0, 0, 0, 1, 1, 2, 4, 4]),
+                                     np.array([1, 2, 3, 2, 3, 3, 0, 1]), 2)
+    assert oracle.tri_list_checksum(ip2, ci2) != c0
+
+
+def test_tri_count_range_partitions_and_unions(golden_graphs):
+    """or_tri_count_range counts the triangles whose lowest vertex lies in [vbegin, vend): the
+    counts of any partition of the vertex range add up to the whole count, and on the disjoint
+    union of the golden graphs (graph g on ids [base_g, base_g + n_g)) the range of each graph
+    gives that graph's networkx count."""
+    srcs, dsts, bases, base = [], [], [], 0
+    for rec in golden_graphs:
+        bases.append(base)
+        srcs.append(np.array(rec["src"], np.int64) + base)
+        dsts.append(np.array(rec["dst"], np.int64) + base)
+        base += rec["n"]
+    bases.append(base)
+    ip, ci = oracle.csr_from_edges(base, np.concatenate(srcs).astype(np.int32),
+                                   np.concatenate(dsts).astype(np.int32), 0)
+    total = oracle.tri_count(ip, ci)
+    assert total == sum(r["triangles"] for r in golden_graphs)
+    for k, rec in enumerate(golden_graphs):
+        assert oracle.tri_count_range(ip, ci, bases[k], bases[k + 1]) == rec["triangles"], \
+            rec["name"]
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        cuts = np.sort(rng.integers(0, base + 1, 7))
+        edges = [0, *cuts.tolist(), base]
+        parts = [oracle.tri_count_range(ip, ci, a, b) for a, b in zip(edges, edges[1:])]
+        assert sum(parts) == total
+    assert oracle.tri_count_range(ip, ci, 7, 7) == 0
+
+
 # ---- k-cliques (no reference counterpart; pinned by networkx, closed forms, brute force) -------
 
 def _kc_fixture():
