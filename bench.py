@@ -386,17 +386,22 @@ class Bench:
             keys = np.sort(rng.integers(1 << 29, 1 << 30, nk, dtype=np.int64)).astype(np.int32)
             dk = DeviceBuffer.from_array(keys)
             want = np.isin(keys, lst)
-            for name, mode in (("plain", ops.SEARCH_PLAIN), ("two_phase", ops.SEARCH_2PHASE)):
-                check(lib.gary_search(dl.ptr, lst.size, dk.ptr, nk, res.ptr, mode, 32, None))
+            # two_phase: the reference's 32-entry index cache (test/TriQueryTest.cu:42);
+            # two_phase_1024: a 1,024-entry cache (4 KiB of LDS per workgroup)
+            for name, mode, depth in (("plain", ops.SEARCH_PLAIN, 32),
+                                      ("two_phase", ops.SEARCH_2PHASE, 32),
+                                      ("two_phase_1024", ops.SEARCH_2PHASE, 1024)):
+                check(lib.gary_search(dl.ptr, lst.size, dk.ptr, nk, res.ptr, mode, depth, None))
                 got = res.to_array(np.uint8, nk).astype(bool)
                 if not np.array_equal(got, want):
                     raise SystemExit(f"search {name} with {nk} keys differs from numpy")
                 ms = timed(lambda: check(lib.gary_search(dl.ptr, lst.size, dk.ptr, nk, res.ptr,
-                                                         mode, 32, None)), 20)
+                                                         mode, depth, None)), 20)
                 leg[f"search_{name}_{nk}_us"] = ms * 1e3
                 leg[f"search_{name}_{nk}_keys_per_s"] = nk / (ms * 1e-3)
             log(f"ops: search {nk} keys: plain {leg[f'search_plain_{nk}_us']:.1f} us, two-phase "
-                f"{leg[f'search_two_phase_{nk}_us']:.1f} us")
+                f"{leg[f'search_two_phase_{nk}_us']:.1f} us (1,024-entry cache "
+                f"{leg[f'search_two_phase_1024_{nk}_us']:.1f} us)")
         a = np.sort(rng.integers(1 << 26, 1 << 30, 4 << 20, dtype=np.int64)).astype(np.int32)
         a = np.unique(a)
         da = DeviceBuffer.from_array(a)

@@ -26,21 +26,23 @@ constexpr int kSearchKeys = 4;
 
 // ---- search ------------------------------------------------------------------------------
 
-// binarySearchKernel / binarySearch2PhaseKernel / the count variants: every wave builds its
-// own index cache of the whole list (Search.cuh:103-109), then threads grid-stride the keys.
+// binarySearchKernel / binarySearch2PhaseKernel / the count variants: every workgroup builds one
+// index cache of the whole list (Search.cuh:103-109) for its 256 threads, then threads
+// grid-stride the keys. Phase 1 is a branchless lower bound in the cache for all K keys of a
+// thread at once; phase 2 (and the plain search) halve in HBM down to kSegment candidates and
+// finish with one segment read per key (bsearch_multi_seg).
 template <bool TWO_PHASE, bool COUNT, int K>
 __global__ void __launch_bounds__(kBlock)
     search_kernel(const int32_t *__restrict__ list, int32_t len, const int32_t *__restrict__ keys,
                   int64_t nkeys, uint8_t *__restrict__ results, int32_t n_cache,
                   unsigned long long *count) {
-  extern __shared__ int32_t smem[];
-  int32_t *cache = smem + (threadIdx.x / kWave) * n_cache;
+  extern __shared__ int32_t cache[];
   if constexpr (TWO_PHASE) {
-    build_cache_group(list, len, cache, n_cache, lane_id(), kWave);
-    wave_sync();
+    build_cache_group(list, len, cache, n_cache, threadIdx.x, kBlock);
+    __syncthreads();
   }
-  // K keys per thread and trip, searched in lock-step (bsearch_multi): key r of a trip is
-  // i0 + r * stride, so the key loads and result stores of each r stay coalesced
+  // K keys per thread and trip, searched in lock-step: key r of a trip is i0 + r * stride, so
+  // the key loads and result stores of each r stay coalesced
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   uint32_t c = 0;
   for (int64_t i0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; i0 < nkeys; i0 += K * stride) {
@@ -54,24 +56,26 @@ __global__ void __launch_bounds__(kBlock)
       p[r] = list;
       n[r] = i < nkeys ? len : 0;
       hit[r] = false;
-      if constexpr (TWO_PHASE) {  // phase 1 in the LDS cache (Search.cuh:120-138)
-        if (n[r] > 0) {
-          int32_t pos = 0;
-          if (bsearch_pos(cache, n_cache, key[r], pos)) {
-            hit[r] = true;
-            n[r] = 0;
-          } else if (pos < 0) {
-            n[r] = 0;
-          } else {
-            const int32_t bottom = (int32_t)((int64_t)pos * len / n_cache);
-            const int32_t top = (int32_t)((int64_t)(pos + 1) * len / n_cache);
-            p[r] = list + bottom;
-            n[r] = top - bottom;
-          }
+    }
+    if constexpr (TWO_PHASE) {  // phase 1 in the LDS cache (Search.cuh:120-138)
+      int32_t lb[K];
+      lower_bound_pow2_multi<K>(cache, n_cache, key, lb);
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        // a cache hit, or pos = lb - 1: the last cache entry below the key (none: a miss)
+        hit[r] = n[r] > 0 && lb[r] < n_cache && cache[lb[r]] == key[r];
+        const int32_t pos = lb[r] - 1;
+        if (hit[r] || pos < 0) {
+          n[r] = 0;
+        } else {
+          const int32_t bottom = (int32_t)((int64_t)pos * len / n_cache);
+          const int32_t top = (int32_t)((int64_t)(pos + 1) * len / n_cache);
+          p[r] = list + bottom;
+          n[r] = n[r] > 0 ? top - bottom : 0;
         }
       }
     }
-    bsearch_multi<K>(p, n, key, f);
+    bsearch_multi_seg<K>(p, n, key, f);
 #pragma unroll
     for (int r = 0; r < K; ++r) {
       const int64_t i = i0 + r * stride;
@@ -385,7 +389,7 @@ void launch_search(int grid, int mode, const int32_t *list, int64_t len, const i
                    int64_t nkeys, uint8_t *results, int cache_depth, unsigned long long *cnt,
                    hipStream_t st) {
   const bool multi = nkeys > (int64_t)grid * kBlock;
-  const size_t smem = kWavesPerBlock * cache_depth * sizeof(int32_t);
+  const size_t smem = (size_t)cache_depth * sizeof(int32_t);
   if (mode == GARY_SEARCH_2PHASE) {
     if (multi)
       search_kernel<true, COUNT, kSearchKeys><<<grid, kBlock, smem, st>>>(

@@ -95,7 +95,7 @@ __device__ __forceinline__ uint32_t isect_group(const int32_t *__restrict__ A, i
     } else {
       build_cache_group(B, nb, cache, n_cache, glane, G);
       wave_sync();
-      for (int32_t k = glane; k < na; k += G) c += bsearch_2phase(B, nb, cache, n_cache, A[k]);
+      for (int32_t k = glane; k < na; k += G) c += bsearch_2phase_seg(B, nb, cache, n_cache, A[k]);
       wave_sync();  // the cache is rebuilt for the next edge
     }
   }
@@ -212,11 +212,13 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t w, int32_t log_t) {
 // Membership set of one pivot list N+(u) in LDS.
 //
 // Bitmap mode (log_t == 0), a two-level rank bitmap over x = w - base, x < span:
-//   coarse[cwords] : bit c set iff some element has x >> s == c
-//   prefix[cwords] : number of set coarse bits before word i
-//   exact[...]     : one block of 2^s bits per set coarse bit, in coarse-bit order
-// Membership costs at most three LDS reads and no loop; elements of one sorted row share coarse
-// words, so neighbouring lanes mostly read the same addresses.
+//   word[cwords] : low 16 bits: bit c set iff some element has (x >> s) == 16 * i + c;
+//                  high 16 bits: number of set coarse bits in the words before i
+//   exact[...]   : one block of 2^s bits per set coarse bit, in coarse-bit order
+// A probe reads one word (coarse bit and rank together) and, when the coarse bit is set, one
+// exact word; elements of one sorted row share coarse words, so neighbouring lanes mostly read
+// the same addresses. (Coarse bits and ranks in separate arrays took one more dependent read
+// per probe: 3.59 -> 3.50 ms per R-MAT 20 count with them packed.)
 // Hash mode (log_t > 0): open addressing over 2^log_t slots (empty = -1, linear probing), used
 // when the bitmap does not fit.
 struct PivotSet {
@@ -235,8 +237,8 @@ __device__ __forceinline__ bool pivot_plan(int32_t lo, int32_t hi, int32_t d, in
   int32_t best_s = 5;
   int64_t best = INT64_MAX;
   for (int32_t s = 5; s <= 20; ++s) {
-    const int64_t cw = ((int64_t)ps.span + (1ll << (s + 5)) - 1) >> (s + 5);
-    const int64_t w = 2 * cw + ((int64_t)d << (s - 5));
+    const int64_t cw = ((int64_t)ps.span + (1ll << (s + 4)) - 1) >> (s + 4);
+    const int64_t w = cw + ((int64_t)d << (s - 5));
     if (w < best) {
       best = w;
       best_s = s;
@@ -244,7 +246,7 @@ __device__ __forceinline__ bool pivot_plan(int32_t lo, int32_t hi, int32_t d, in
   }
   if (best <= table_words) {
     ps.s = best_s;
-    ps.cwords = (int32_t)(((int64_t)ps.span + (1ll << (best_s + 5)) - 1) >> (best_s + 5));
+    ps.cwords = (int32_t)(((int64_t)ps.span + (1ll << (best_s + 4)) - 1) >> (best_s + 4));
     ps.log_t = 0;
     ps.words = (int32_t)best;
     return true;
@@ -289,21 +291,21 @@ __device__ __forceinline__ void pivot_build(int32_t *t, const PivotSet &ps,
     return;
   }
   uint32_t *coarse = (uint32_t *)t;
-  int32_t *prefix = t + ps.cwords;
-  uint32_t *exact = (uint32_t *)(t + 2 * ps.cwords);
+  uint32_t *exact = (uint32_t *)(t + ps.cwords);
   const int32_t s = ps.s;
   for (int32_t i = tid; i < d; i += nthr) {
     const uint32_t x = (uint32_t)(L[i] - ps.base);
-    if (x < ps.span) atomicOr(&coarse[x >> (s + 5)], 1u << ((x >> s) & 31));
+    if (x < ps.span) atomicOr(&coarse[x >> (s + 4)], 1u << ((x >> s) & 15));
   }
   pivot_sync<BLOCK>();
-  if (tid < kWave) {
+  if (tid < kWave) {  // rank prefix of the set coarse bits, into the high half of each word
     uint32_t carry = 0;
     for (int32_t c0 = 0; c0 < ps.cwords; c0 += kWave) {
       const int32_t i = c0 + tid;
-      const uint32_t v = i < ps.cwords ? (uint32_t)__popc(coarse[i]) : 0u;
+      const uint32_t w = i < ps.cwords ? coarse[i] : 0u;
+      const uint32_t v = (uint32_t)__popc(w);
       const uint32_t incl = wave_inclusive_scan_u32(v);
-      if (i < ps.cwords) prefix[i] = (int32_t)(carry + incl - v);
+      if (i < ps.cwords) coarse[i] = w | ((carry + incl - v) << 16);
       carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
     }
   }
@@ -311,8 +313,8 @@ __device__ __forceinline__ void pivot_build(int32_t *t, const PivotSet &ps,
   for (int32_t i = tid; i < d; i += nthr) {
     const uint32_t x = (uint32_t)(L[i] - ps.base);
     if (x >= ps.span) continue;
-    const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
-    const uint32_t r = (uint32_t)prefix[cw] + (uint32_t)__popc(coarse[cw] & ((1u << b) - 1));
+    const uint32_t word = coarse[x >> (s + 4)], b = (x >> s) & 15;
+    const uint32_t r = (word >> 16) + (uint32_t)__popc(word & ((1u << b) - 1));
     atomicOr(&exact[(r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)], 1u << (x & 31));
   }
   pivot_sync<BLOCK>();
@@ -323,11 +325,10 @@ __device__ __forceinline__ uint32_t pivot_probe(const int32_t *t, const PivotSet
   if (x >= ps.span) return 0;
   if (ps.log_t == 0) {
     const int32_t s = ps.s;
-    const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
-    const uint32_t word = (uint32_t)t[cw];
+    const uint32_t word = (uint32_t)t[x >> (s + 4)], b = (x >> s) & 15;
     if (!((word >> b) & 1u)) return 0;
-    const uint32_t r = (uint32_t)t[ps.cwords + cw] + (uint32_t)__popc(word & ((1u << b) - 1));
-    const uint32_t e = (uint32_t)t[2 * ps.cwords + (r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)];
+    const uint32_t r = (word >> 16) + (uint32_t)__popc(word & ((1u << b) - 1));
+    const uint32_t e = (uint32_t)t[ps.cwords + (r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)];
     return (e >> (x & 31)) & 1u;
   }
   const uint32_t mask = (1u << ps.log_t) - 1;
@@ -340,30 +341,81 @@ __device__ __forceinline__ uint32_t pivot_probe(const int32_t *t, const PivotSet
   }
 }
 
+constexpr int kStreamWindows = 2;  // listing: 16-byte-chunk windows loaded before any is probed
+constexpr int kChunk = 8;          // counting: elements per streamed chunk (32 bytes)
+
 // Per-wave LDS scratch of the flattened row streaming: one slot per lane.
 struct RowSlots {
   int32_t s[kWave];
 };
 
+// The bitmap and hash probes of pivot_probe, apart, so the element loops below are compiled
+// once per mode with no per-element mode branch.
+__device__ __forceinline__ uint32_t probe_bitmap(const int32_t *t, const PivotSet &ps, int32_t w) {
+  const uint32_t x = (uint32_t)(w - ps.base);
+  if (x >= ps.span) return 0;
+  const int32_t s = ps.s;
+  const uint32_t word = (uint32_t)t[x >> (s + 4)], b = (x >> s) & 15;
+  if (!((word >> b) & 1u)) return 0;
+  const uint32_t r = (word >> 16) + (uint32_t)__popc(word & ((1u << b) - 1));
+  const uint32_t e = (uint32_t)t[ps.cwords + (r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)];
+  return (e >> (x & 31)) & 1u;
+}
+
+__device__ __forceinline__ uint32_t probe_hash(const int32_t *t, const PivotSet &ps, int32_t w) {
+  const uint32_t mask = (1u << ps.log_t) - 1;
+  uint32_t h = hash_slot(w, ps.log_t);
+  while (true) {
+    const int32_t v = t[h];
+    if (v == w) return 1;
+    if (v == -1) return 0;
+    h = (h + 1) & mask;
+  }
+}
+
+template <int CH, bool HASH>
+__device__ __forceinline__ uint32_t probe_chunk(const int4 (&data)[CH / 4], int32_t jlo, int32_t jhi,
+                                                const int32_t *table, const PivotSet &ps) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int h = 0; h < CH / 4; ++h) {
+    const int32_t el[4] = {data[h].x, data[h].y, data[h].z, data[h].w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int pos = 4 * h + q;
+      if (jlo <= pos && pos < jhi)
+        c += HASH ? probe_hash(table, ps, el[q]) : probe_bitmap(table, ps, el[q]);
+    }
+  }
+  return c;
+}
+
 // The rows r0, r0 + stride, ... (< d) of pivot list L, 64 rows per batch. A batch is flattened
-// into the 16-byte chunks that overlap its rows; each window of 64 chunks gives lane l chunk
-// w0 + l, so a wave's loads cover one contiguous stretch of colIdx, and K windows are loaded
-// before any is probed. The row owning chunk w0 + l is the last row starting at or before it:
-// each row with chunks writes its lane number into the slot of its first chunk in the window
-// (the row running into the window writes slot 0), and an inclusive max-scan over the slots
-// hands every lane its row; the row's start, first chunk and element window come from that
-// row's lane by shuffles. Elements of a chunk outside its row are masked; the array's last
-// partial chunk is read element-wise. Chunk indices are 32-bit (colIdx below 2^34 elements).
-template <int K>
+// into the aligned chunks of kChunk elements (32 bytes) that overlap its rows; each window of 64
+// chunks gives lane l chunk w0 + l, so a wave's loads cover one contiguous stretch of colIdx.
+// The row owning chunk w0 + l is the last row starting at or before it: each row with chunks in
+// the window writes (tag | its lane) into the LDS slot of its first chunk there (the row running
+// into the window writes slot 0), and an inclusive max-scan over the slots hands every lane its
+// row. The tag grows by window, so slots left by earlier windows lose the max and need no reset.
+// One shuffle each then gives the lane its chunk in colIdx and the range of its valid slots.
+// Elements of a chunk outside its row are skipped. A chunk is read as two aligned 16-byte
+// halves; the second only when it holds an index below nnz, so no read leaves the 16-byte
+// blocks that hold colIdx elements. Chunk indices are 32-bit (colIdx below 2^34 elements).
+// (16-byte chunks with two windows in flight: 3.38 ms per R-MAT 20 count; 32-byte chunks with
+// a reset slot array and four shuffles: 3.21 ms; this form: 3.10 ms.)
 __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
                                                 const int32_t *__restrict__ L, int32_t d,
                                                 int32_t r0, int32_t stride, const int32_t *table,
                                                 const PivotSet &ps, RowSlots &rs, int lane,
                                                 const uint8_t *__restrict__ Lroute) {
+  constexpr int CH = kChunk;
+  constexpr int LG = 3;
   const uint32_t m = (uint32_t)g.csr.m;
-  const int64_t nnz = g.csr.nnz;
+  const uint64_t nnz = (uint64_t)g.csr.nnz;
   const int32_t *__restrict__ col = g.csr.colIdx;
+  const bool hash = ps.log_t != 0;
   uint32_t c = 0;
+  int32_t tag = 1 << 30;  // forces a slot reset before the first window
   for (int32_t b = r0; b < d; b += stride * kWave) {
     const int32_t r = b + lane * stride;
     int32_t dv = 0;
@@ -375,49 +427,46 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
         pv = g.csr.indPtr[v];
       }
     }
-    const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + 3) >> 2) - (pv >> 2)) : 0;
+    const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + CH - 1) >> LG) - (pv >> LG)) : 0;
     const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
-    const int32_t start = incl - nch;  // the row's first chunk in the batch
+    const int32_t start = incl - nch;
     const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
-    const int32_t cb = (int32_t)(uint32_t)(pv >> 2);  // the row's first chunk in colIdx
-    const int32_t rlo = (int32_t)(pv & 3);             // its first element's slot in that chunk
-    const int32_t rhi = rlo + dv;                      // one past its last element's slot
-    for (int32_t q0 = 0; q0 < total; q0 += K * kWave) {
-      int4 data[K];
-      int32_t jlo[K], jhi[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int32_t w0 = q0 + k * kWave;
+    // chunk of batch position x: base_r + x; its valid slots: [A - CH*x, A + dv - CH*x)
+    // (unsigned: the terms may wrap, the results for the row's own positions fit int32)
+    const uint32_t base_r = (uint32_t)(pv >> LG) - (uint32_t)start;
+    const uint32_t A = (uint32_t)(pv & (CH - 1)) + (uint32_t)CH * (uint32_t)start;
+    const uint32_t Bv = A + (uint32_t)dv;
+    for (int32_t w0 = 0; w0 < total; w0 += kWave) {
+      if (tag >= (1 << 30)) {  // stale slots could outrank new tags: clear them
         rs.s[lane] = -1;
+        tag = 0;
         wave_sync();
-        if (nch > 0 && start < w0 + kWave && incl > w0) rs.s[max(0, start - w0)] = lane;
-        wave_sync();
-        const int32_t row = max(0, wave_inclusive_max_i32(rs.s[lane]));
-        const int32_t j = w0 + lane - __shfl(start, row, kWave);  // chunk index within the row
-        const uint32_t ch = (uint32_t)__shfl(cb, row, kWave) + (uint32_t)j;
-        jlo[k] = max(0, __shfl(rlo, row, kWave) - 4 * j);
-        jhi[k] = min(4, __shfl(rhi, row, kWave) - 4 * j);
-        data[k] = make_int4(0, 0, 0, 0);
-        if (w0 + lane < total) {
-          const int64_t e0 = 4 * (int64_t)ch;
-          if (e0 + 3 < nnz) {
-            data[k] = *reinterpret_cast<const int4 *>(col + e0);
-          } else {
-            data[k].x = e0 < nnz ? col[e0] : 0;
-            data[k].y = e0 + 1 < nnz ? col[e0 + 1] : 0;
-            data[k].z = e0 + 2 < nnz ? col[e0 + 2] : 0;
-          }
-        } else {
-          jhi[k] = 0;
-        }
       }
+      // slot of the window position where each row with chunks here begins (0 for the row
+      // running into the window), tagged so that slots left by earlier windows lose the max
+      if (nch > 0 && start < w0 + kWave && incl > w0) rs.s[max(0, start - w0)] = tag | lane;
+      wave_sync();
+      const int32_t row = wave_inclusive_max_i32(rs.s[lane]) & (kWave - 1);
+      wave_sync();
+      tag += kWave;
+      const int32_t x = w0 + lane;
+      const uint32_t ch = (uint32_t)__shfl((int32_t)base_r, row, kWave) + (uint32_t)x;
+      const int32_t jlo = (int32_t)((uint32_t)__shfl((int32_t)A, row, kWave) - (uint32_t)CH * (uint32_t)x);
+      int32_t jhi = (int32_t)((uint32_t)__shfl((int32_t)Bv, row, kWave) - (uint32_t)CH * (uint32_t)x);
+      int4 data[CH / 4];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        c += (jlo[k] <= 0 && 0 < jhi[k]) ? pivot_probe(table, ps, data[k].x) : 0u;
-        c += (jlo[k] <= 1 && 1 < jhi[k]) ? pivot_probe(table, ps, data[k].y) : 0u;
-        c += (jlo[k] <= 2 && 2 < jhi[k]) ? pivot_probe(table, ps, data[k].z) : 0u;
-        c += (jlo[k] <= 3 && 3 < jhi[k]) ? pivot_probe(table, ps, data[k].w) : 0u;
+      for (int h = 0; h < CH / 4; ++h) data[h] = make_int4(0, 0, 0, 0);
+      if (x < total) {
+        const uint64_t e0 = (uint64_t)ch << LG;
+        data[0] = *reinterpret_cast<const int4 *>(col + e0);
+#pragma unroll
+        for (int h = 1; h < CH / 4; ++h)
+          if (e0 + 4 * h < nnz) data[h] = *reinterpret_cast<const int4 *>(col + e0 + 4 * h);
+      } else {
+        jhi = 0;
       }
+      c += hash ? probe_chunk<CH, true>(data, jlo, jhi, table, ps)
+                : probe_chunk<CH, false>(data, jlo, jhi, table, ps);
     }
   }
   return c;
@@ -454,7 +503,6 @@ constexpr int kSmallBatchHeavy = 4;
 constexpr int kSmallBatchLight = 16;
 constexpr int kLightLog = 4;
 static_assert(kLightLog >= 1 && kLightLog <= 15, "light-pivot threshold");
-constexpr int kStreamWindows = 2;  // 64-chunk windows loaded before any is probed
 
 template <int WAVES, int TABLE_WORDS>
 __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8)))
@@ -490,7 +538,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       PivotSet s;
       if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
         pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
-        c += stream_rows<kStreamWindows>(g, L, d, wib, WAVES, table, s, rb, lane,
+        c += stream_rows(g, L, d, wib, WAVES, table, s, rb, lane,
                                   route ? route + g.csr.indPtr[u] : nullptr);
       } else {
         c += pivot_fallback(g, L, d, wib, WAVES, lane, route ? route + g.csr.indPtr[u] : nullptr);
@@ -538,7 +586,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
         continue;
       }
       pivot_build<false>(wt, s, L, d, lane, kWave);
-      c += stream_rows<kStreamWindows>(g, L, d, 0, 1, wt, s, rb, lane,
+      c += stream_rows(g, L, d, 0, 1, wt, s, rb, lane,
                                 route ? route + g.csr.indPtr[u] : nullptr);
       wave_sync();  // the set is rebuilt for the next pivot
     }
