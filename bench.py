@@ -6,9 +6,11 @@ launch of the counting path, including the per-call pivot binning and, for the l
 its features and inference). Prints one JSON line: edges/s of the headline strategy, per-strategy
 times, the HBM roofline fraction of the dominant kernel under the merge byte model
 B_alg = 4*W + 32*m (W = sum over arcs (u,v) of d+(u) + d+(v)), and the reference's CPU algorithm
-(cpuTriCount restated in the oracle) timed on this host.
+(cpuTriCount restated in the oracle) timed on this host, with the CPU model and thread count.
 
-Further legs (--legs, default all):
+ms_per_step is the mean of the K back-to-back steps; ms_per_step_median the median of K counts
+each timed alone. Further legs (--legs, default all):
+  c1     Erdos-Renyi G(4096, 65536): every strategy and the CPU count;
   c3     the learned selector against the plain degree-threshold heuristic (and bitmap-only, and
          the reference dispatcher's cases as routes) on the Chung-Lu graph (4,194,304 vertices,
          ~64M edges, exponent 2.1, max weight 50,000);
@@ -103,6 +105,33 @@ class Bench:
         assert total % steps == 0
         return ms, total // steps, (kms / launches if launches else None)
 
+    def step_median(self, ctx, strategy: int, steps: int, cfg=None) -> float:
+        """Median ms of `steps` counts each timed alone (HIP events around one count, the host
+        waiting between counts): the per-step figure beside the back-to-back mean."""
+        gq, cnt = self.gq, self.cnt
+        ts = []
+        for _ in range(max(steps, 10)):
+            self.timer.start()
+            gq.triangleCountHomAsync(ctx, cnt, strategy, cfg)
+            ts.append(self.timer.stop())
+        ts.sort()
+        return ts[len(ts) // 2]
+
+    def time_heuristic_tuned(self, ctx, workload: str, want: int, steps: int, cfg=None):
+        """ms per count of the heuristic strategy with the autotuner's thresholds for this
+        workload (None when the tuner has no entry), the thresholds restored afterwards."""
+        th = self.autotune.heuristic_for(workload)
+        if th is None:
+            return None, None
+        try:
+            self.selector.set_heuristic(**th)
+            ms, t, _ = self.time_count(ctx, self.gq.HEURISTIC, steps, 1, cfg)
+        finally:
+            self.selector.set_heuristic(**self.selector.HEURISTIC_DEFAULT)
+        if t != want:
+            raise SystemExit(f"tuned heuristic counted {t} != {want}")
+        return ms, th
+
     def time_dispatch(self, ctx, want: int) -> float:
         """ms per count with the reference dispatcher's cases as routes (no bitmap route)."""
         tab = self.selector.dispatch_table()
@@ -144,23 +173,38 @@ class Bench:
                 kernel_ms[nm] = kms
             log(f"{nm}: {ms:.3f} ms/step" + (f", triangles={tri}" if nm == a.strategy else ""))
         self.dispatch_ms = None if a.only else self.time_dispatch(ctx, tri)
+        medians = {nm: self.step_median(ctx, self.names[nm], a.steps,
+                                        self.autotune.cfg_for(wl, nm) if wl else None)
+                   for nm in results if nm in ("learned", "heuristic", "bitmap")}
+        self.heur_tuned = (None, None) if a.only or not wl else \
+            self.time_heuristic_tuned(ctx, wl, tri, a.steps, self.autotune.cfg_for(wl, "heuristic"))
+        if self.heur_tuned[0] is not None:
+            log(f"heuristic (tuned thresholds {self.heur_tuned[1]}): {self.heur_tuned[0]:.3f} ms/step")
         pivot_path = a.strategy in ("learned", "heuristic", "bitmap")
         k_ms = kernel_ms.get(a.strategy, results[a.strategy])
         k_bytes = B_alg_main if pivot_path else B_alg
         achieved = k_bytes / (k_ms * 1e-3) / 1e9
-        traffic = None
+        traffic, traffic_src = None, None
         if pivot_path and a.scale == 20 and a.edgefactor == 16:
-            from gary_amd.profiling import read_traffic
-            traffic = read_traffic(ROOT / "profiles" / "tc_pivot_rmat20_traffic.md")
+            # not measured by this run: the calibrated FETCH_SIZE / WRITE_SIZE passes of the
+            # committed profile, named with the commit they profiled
+            from gary_amd.profiling import read_commit, read_traffic
+            prof = ROOT / "profiles" / "tc_pivot_rmat20_traffic.md"
+            traffic = read_traffic(prof)
+            if traffic is not None:
+                traffic_src = f"profiles/{prof.name} (source commit {read_commit(prof)})"
         self.g, self.tri = g, tri
         ms = results[a.strategy]
         return {
             "value": g.nnz / (ms * 1e-3), "ms_per_step": ms,
+            "ms_per_step_median": medians.get(a.strategy),
+            "strategies_ms_median": medians,
             "config": {"workload": f"rmat{a.scale}-ef{a.edgefactor} (a=.57,b=.19,c=.19) "
                                    "symmetrized, dedup, no loops, degree-oriented int32 CSR",
                        "n": n, "m": g.nnz, "triangles": tri, "strategy": a.strategy},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                         "traffic_source": traffic_src,
                          "kernel": "tc_pivot_kernel<4,4096>" if pivot_path else "tc_edge_kernel",
                          "kernel_ms": k_ms,
                          "bytes_model": "sum over arcs (u,v) of 4*(d+(u)+d+(v))+32",
@@ -203,8 +247,28 @@ class Bench:
             log(f"chung_lu_4m {nm}: {ms:.3f} ms")
         leg["triangles"] = ref
         leg["reference_dispatch_ms"] = self.time_dispatch(ctx, ref)
+        ht_ms, ht = self.time_heuristic_tuned(ctx, "c3", ref, max(1, min(a.steps, 10)),
+                                              self.autotune.cfg_for("c3", "heuristic"))
+        leg["heuristic_tuned_ms"], leg["heuristic_tuned_thresholds"] = ht_ms, ht
         leg["learned_vs_plain"] = leg["heuristic_ms"] / leg["learned_ms"]
+        leg["learned_vs_tuned_heuristic"] = ht_ms / leg["learned_ms"] if ht_ms else None
+        leg["learned_vs_bitmap"] = leg["bitmap_ms"] / leg["learned_ms"]
         leg["learned_vs_reference_dispatch"] = leg["reference_dispatch_ms"] / leg["learned_ms"]
+        leg["edges_per_s"] = nnz / (leg["learned_ms"] * 1e-3)
+        if not a.no_cpu:  # cpuTriCount restated, on the same CSR copied back to the host
+            import oracle
+            g = ctx.getHostCSR()
+            threads = oracle.omp_threads(a.cpu_threads)
+            ts = []
+            for _ in range(2):
+                c0 = time.perf_counter()
+                tri = oracle.tri_count(g.indPtr, g.colIdx, threads)
+                ts.append(time.perf_counter() - c0)
+            if tri != ref:
+                raise SystemExit(f"chung_lu_4m: CPU count {tri} != GPU count {ref}")
+            t = min(ts)
+            leg["cpu_baseline"] = self.cpu_record(nnz / t, threads, f"full Chung-Lu count, best of 2, {t:.2f}s each")
+            log(f"chung_lu_4m cpu baseline: {t:.2f}s with {threads} threads")
         return leg
 
     def batch(self) -> dict:
@@ -232,13 +296,36 @@ class Bench:
         ach = b_all / (ms * 1e-3) / 1e9
         log(f"batch2000: {ms:.3f} ms per launch, {g.nnz / (ms * 1e-3):.3e} edges/s")
         B.close()
-        return {"workload": "2,000 seeded graphs, n ~ U[64, 20000], avg degree ~ U[4, 64], half ER "
-                            "half R-MAT(.57,.19,.19,.05), degree-oriented int32 CSR, one hipGraph "
-                            "replay per step, uint64 count per graph",
-                "graphs": 2000, "m": g.nnz, "ms_per_launch": ms,
-                "edges_per_s": g.nnz / (ms * 1e-3), "triangles": one,
-                "roofline": {"achieved": ach, "frac": ach / HBM_PEAK_GBS,
-                             "algorithmic_bytes": b_all}}
+        leg = {"workload": "2,000 seeded graphs, n ~ U[64, 20000], avg degree ~ U[4, 64], half ER "
+                           "half R-MAT(.57,.19,.19,.05), degree-oriented int32 CSR, one hipGraph "
+                           "replay per step, uint64 count per graph",
+               "graphs": 2000, "m": g.nnz, "ms_per_launch": ms,
+               "edges_per_s": g.nnz / (ms * 1e-3), "triangles": one,
+               "roofline": {"achieved": ach, "frac": ach / HBM_PEAK_GBS,
+                            "algorithmic_bytes": b_all}}
+        if not a.no_cpu:
+            # cpuTriCount per graph, graphs in parallel, on every 10th graph of the union
+            import oracle
+            threads = oracle.omp_threads(a.cpu_threads)
+            picks = list(range(0, 2000, 10))
+            ptrs, cols, base = [np.zeros(1, np.int64)], [], [0]
+            for i in picks:  # graph i's rows of the union CSR, its ids moved to base[-1]
+                lo, hi = int(vbase[i]), int(vbase[i + 1])
+                e0, e1 = int(g.indPtr[lo]), int(g.indPtr[hi])
+                ptrs.append(g.indPtr[lo + 1:hi + 1] - e0 + ptrs[-1][-1])
+                cols.append((g.colIdx[e0:e1].astype(np.int64) - lo + base[-1]).astype(np.int32))
+                base.append(base[-1] + hi - lo)
+            sub = self.graph.GraphCSR(np.concatenate(ptrs), np.concatenate(cols))
+            c0 = time.perf_counter()
+            per = oracle.tri_count_batch(np.array(base, np.int32), sub.indPtr, sub.colIdx, threads)
+            t = time.perf_counter() - c0
+            if not np.array_equal(per, per_graph[picks]):
+                raise SystemExit("batch2000: CPU per-graph counts differ from the GPU's")
+            leg["cpu_baseline"] = self.cpu_record(
+                sub.nnz / t, threads, f"every 10th graph (200 graphs, {sub.nnz} arcs), "
+                                      f"parallel over graphs, one run, {t:.2f}s")
+            log(f"batch2000 cpu baseline: 200 graphs in {t:.2f}s with {threads} threads")
+        return leg
 
     def hom4(self) -> dict:
         gq, graph = self.gq, self.graph
@@ -263,7 +350,56 @@ class Bench:
             log(f"rmat22 hom4 {nm}: {leg[f'{nm}_ms']:.1f} ms")
         leg["counts"] = vals
         leg["edges_per_s"] = (g.nnz // 2) / (leg["all_ms"] * 1e-3)
+        # per-pattern algorithmic bytes (m undirected edges, d full degree, d+ upper degree):
+        #   diamond 4*sum d^2 + 32m; 4-cycles 4*sum d^2 + 16n; 4-cliques at least 4*W + 32m
+        #   (W = sum over arcs (u,v) of d+(u) + d+(v); the full K4 formula adds the common
+        #   out-neighbour terms, which need per-arc triangle counts)
+        np = self.np
+        d = np.diff(g.indPtr).astype(np.int64)
+        row = np.repeat(np.arange(n, dtype=np.int64), d)
+        upper = g.colIdx > row
+        dplus = np.bincount(row[upper], minlength=n).astype(np.int64)
+        W = int((dplus[row[upper]] + dplus[g.colIdx[upper]]).sum())
+        m_und = g.nnz // 2
+        sd2 = int((d * d).sum())
+        patterns = {"diamond": (4 * sd2 + 32 * m_und, leg["k4_diamond_ms"]),
+                    "c4": (4 * sd2 + 16 * n, leg["c4_ms"]),
+                    "k4_lower_bound": (4 * W + 32 * m_und, leg["k4_diamond_ms"])}
+        leg["roofline"] = {k: {"algorithmic_bytes": b, "ms": t,
+                               "achieved": b / (t * 1e-3) / 1e9,
+                               "frac": b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                           for k, (b, t) in patterns.items()}
+        if not self.a.no_cpu:
+            leg["cpu_sample"] = self.hom4_cpu_sample()
         return leg
+
+    def hom4_cpu_sample(self) -> dict:
+        """The CPU hom4 path (the oracle's oriented-wedge counts, OpenMP) on R-MAT 20 beside the
+        GPU call on the same graph: RMAT-22 takes minutes on the CPU, so the sample is the same
+        generator two scales down."""
+        import oracle
+        gq, graph = self.gq, self.graph
+        n, s, d = graph.rmat(20, 16, seed=graph.SEEDS["rmat20"])
+        g = graph.csr_from_edges(n, s, d, graph.SYMMETRIC_RANKED)
+        ctx = self.DeviceGraphCtx(g)
+        want = gq.hom4Count(ctx, gq.HOM_ALL)
+        t1 = time.perf_counter()
+        for _ in range(3):
+            gq.hom4Count(ctx, gq.HOM_ALL)
+        gpu_ms = (time.perf_counter() - t1) * 1e3 / 3
+        threads = oracle.omp_threads(self.a.cpu_threads)
+        c0 = time.perf_counter()
+        h = oracle.hom4_ranked(g.indPtr, g.colIdx, threads)
+        t = time.perf_counter() - c0
+        if h != want:
+            raise SystemExit(f"rmat20 hom4: CPU {h} != GPU {want}")
+        m_und = g.nnz // 2
+        log(f"rmat20 hom4: GPU {gpu_ms:.1f} ms, CPU {t:.2f}s with {threads} threads")
+        return {"workload": "rmat20-ef16 ranked symmetric, all six counts", "undirected_edges": m_und,
+                "gpu_ms": gpu_ms, "gpu_edges_per_s": m_und / (gpu_ms * 1e-3),
+                "cpu_baseline": self.cpu_record(m_und / t, threads,
+                                                f"full R-MAT 20 hom4 (oracle.hom4_ranked), one run, "
+                                                f"{t:.1f}s")}
 
     def build(self) -> dict:
         """Device graph construction against the host build of the same tuples (wall clock,
@@ -433,6 +569,20 @@ class Bench:
                 f"{leg[f'isect_{nb}_{best}_us']:.1f} us (cpu {cpu_us:.0f} us)")
         return leg
 
+    @staticmethod
+    def cpu_model() -> str:
+        try:
+            for line in Path("/proc/cpuinfo").read_text().splitlines():
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+        except OSError:
+            pass
+        return "unknown"
+
+    def cpu_record(self, value: float, threads: int, sample: str) -> dict:
+        return {"value": value, "unit": "edges/s", "cores": threads, "kind": "port",
+                "cpu_model": self.cpu_model(), "sample": sample}
+
     def cpu(self) -> dict:
         import oracle
         a, g = self.a, self.g
@@ -446,10 +596,46 @@ class Bench:
         if tri != self.tri:
             raise SystemExit(f"CPU count {tri} != GPU count {self.tri}")
         t = sorted(times)[4]
-        log(f"cpu baseline: {t:.3f}s with {threads} threads")
-        return {"value": g.nnz / t, "unit": "edges/s", "cores": threads, "kind": "port",
-                "sample": f"full rmat{a.scale} count (cpuTriCount restated, OpenMP dynamic), "
-                          f"median of 9 after 1 warm-up, {t:.2f}s each"}
+        log(f"cpu baseline: {t:.3f}s with {threads} threads ({self.cpu_model()})")
+        return self.cpu_record(g.nnz / t, threads,
+                               f"full rmat{a.scale} count (cpuTriCount restated, OpenMP dynamic), "
+                               f"median of 9 after 1 warm-up, {t:.2f}s each")
+
+    def c1(self) -> dict:
+        """Erdős–Rényi G(4096, 65536), degree-oriented: every strategy on the GPU (median of
+        per-count times, each count being launch-bound at this size) and the CPU count."""
+        import oracle
+        a, gq, graph = self.a, self.gq, self.graph
+        self.selector.load()
+        g = graph.oriented(*graph.er_4096())
+        ctx = self.DeviceGraphCtx(g)
+        ctx.populateDevice()
+        want = oracle.tri_count(g.indPtr, g.colIdx)
+        leg = {"workload": "Erdos-Renyi G(n=4096, m=65536), seed 0x6A7E0001, degree-oriented "
+                           "int32 CSR", "m": g.nnz, "triangles": want, "strategies_ms": {}}
+        for nm, sid in self.names.items():
+            cfg = self.autotune.cfg_for("er4096", nm)
+            ms, t, _ = self.time_count(ctx, sid, max(a.steps, 10), 2, cfg)
+            if t != want:
+                raise SystemExit(f"er4096: {nm} counted {t} != {want}")
+            leg["strategies_ms"][nm] = self.step_median(ctx, sid, max(a.steps, 10), cfg)
+        best = min(leg["strategies_ms"], key=leg["strategies_ms"].get)
+        leg["best"] = best
+        leg["edges_per_s"] = g.nnz / (leg["strategies_ms"][best] * 1e-3)
+        log(f"er4096: " + ", ".join(f"{k} {v * 1e3:.0f} us" for k, v in leg["strategies_ms"].items()))
+        if not a.no_cpu:
+            threads = oracle.omp_threads(a.cpu_threads)
+            oracle.tri_count(g.indPtr, g.colIdx, threads)
+            ts = []
+            for _ in range(21):
+                c0 = time.perf_counter()
+                oracle.tri_count(g.indPtr, g.colIdx, threads)
+                ts.append(time.perf_counter() - c0)
+            t = sorted(ts)[10]
+            leg["cpu_baseline"] = self.cpu_record(g.nnz / t, threads,
+                                                  f"full count, median of 21, {t * 1e3:.2f} ms each")
+            log(f"er4096 cpu baseline: {t * 1e3:.2f} ms with {threads} threads")
+        return leg
 
     def calibrate_copy(self, mib: int) -> None:
         from gary_amd._lib import check, lib
@@ -470,7 +656,7 @@ def main(argv=None) -> int:
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--strategy", default="learned")
-    ap.add_argument("--legs", default="headline,c3,batch,hom4,build,patterns,ops")
+    ap.add_argument("--legs", default="headline,c1,c3,batch,hom4,build,patterns,ops")
     ap.add_argument("--build-graphs", default="rmat20_degree,rmat22_ranked",
                     help="graphs of the build leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -497,11 +683,17 @@ def main(argv=None) -> int:
     selector_legs = {}
     if "headline" in legs and not a.only:
         r = out["strategies_ms"]
+        ht_ms, ht = b.heur_tuned
         selector_legs[f"rmat{a.scale}"] = {
             "heuristic_ms": r["heuristic"], "learned_ms": r["learned"],
             "bitmap_ms": r["bitmap"], "reference_dispatch_ms": b.dispatch_ms,
+            "heuristic_tuned_ms": ht_ms, "heuristic_tuned_thresholds": ht,
             "learned_vs_plain": r["heuristic"] / r["learned"],
+            "learned_vs_tuned_heuristic": ht_ms / r["learned"] if ht_ms else None,
+            "learned_vs_bitmap": r["bitmap"] / r["learned"],
             "learned_vs_reference_dispatch": b.dispatch_ms / r["learned"]}
+    if "c1" in legs:
+        out["c1"] = b.c1()
     if "c3" in legs:
         selector_legs["chung_lu_4m"] = b.c3()
     if selector_legs:

@@ -9,7 +9,10 @@ Knobs (gary_launch_cfg):
   run one per wave with a per-wave LDS set) and `bin_hub` (pivots above it get the 8-wave hub
   kernel with a 32 KiB set);
 - edge-centric strategies (cached_bsearch, merge, bsearch): `block`, `group` (lanes per arc),
-  `cache_depth` (LDS index cache entries per group), `partition` and `grid`.
+  `cache_depth` (LDS index cache entries per group), `partition` and `grid`;
+- the heuristic's route thresholds (`heuristic_table`, gary_selector_set_heuristic): the log2
+  out-degree from which a pivot's arcs go to the bitmap, the log2 degree below which both lists
+  count as small (merge), and the log2 length ratio from which the cached binary search is used.
 The search is coordinate descent from the library defaults (two passes over the knobs); every
 candidate's count must equal the default configuration's count, or the tuner stops with an
 error. The result is a JSON file keyed "<workload>/<strategy>"; `cfg_for` reads it.
@@ -37,6 +40,8 @@ PIVOT_SPACE = {"block": [128, 256, 512], "grid": [1024, 2048, 4096, 8192],
 EDGE_SPACE = {"block": [128, 256], "group": [8, 16, 32, 64], "cache_depth": [16, 32, 64, 128],
               "partition": [gq.INTERLEAVED, gq.CHUNKED, gq.DYNAMIC], "grid": [0, 3072, 6144, 12288]}
 PIVOT_STRATEGIES = ("bitmap", "heuristic", "learned")
+HEURISTIC_SPACE = {"bitmap_from": list(range(0, 11)) + [16], "small_max": [2, 3, 4, 5, 6, 7],
+                   "skew": [1, 2, 3, 4, 5, 6]}
 
 
 def to_cfg(d: dict | None) -> LaunchCfg | None:
@@ -120,6 +125,48 @@ def tune_graph(ctx, strategy: str, repeats: int = 5) -> dict:
     return {"cfg": best, "ms": t, "default_ms": t_def, "evals": evals, "count": want}
 
 
+def heuristic_for(workload: str, path: Path = TUNED_PATH) -> dict | None:
+    """The tuned heuristic thresholds of `workload` ({bitmap_from, small_max, skew}), or None."""
+    p = Path(path)
+    if not p.exists():
+        return None
+    rec = json.loads(p.read_text()).get("entries", {}).get(f"{workload}/heuristic_table")
+    return dict(rec["thresholds"]) if rec else None
+
+
+def tune_heuristic(ctx, repeats: int = 5, cfg: LaunchCfg | None = None) -> dict:
+    """Coordinate descent over the heuristic's route thresholds (counts must not change); the
+    library defaults are restored afterwards."""
+    from . import selector
+    d = DeviceBuffer(8)
+    timer = EventTimer()
+
+    def run(th: dict, reps: int) -> tuple[float, int]:
+        selector.set_heuristic(**th)
+        d.zero()
+        gq.triangleCountHomAsync(ctx, d, gq.HEURISTIC, cfg)
+        gq.synchronize()
+        got = int(d.to_array(np.uint64, 1)[0])
+        timer.start()
+        for _ in range(reps):
+            gq.triangleCountHomAsync(ctx, d, gq.HEURISTIC, cfg)
+        return timer.stop() / reps, got
+
+    try:
+        t_def, want = run(selector.HEURISTIC_DEFAULT, repeats)
+
+        def measure(th: dict) -> float:
+            t, got = run(th, repeats)
+            if got != want:
+                raise SystemExit(f"autotune: heuristic thresholds {th} counted {got} != {want}")
+            return t
+
+        best, t, evals = _descend(measure, HEURISTIC_SPACE, dict(selector.HEURISTIC_DEFAULT))
+    finally:
+        selector.set_heuristic(**selector.HEURISTIC_DEFAULT)
+    return {"thresholds": best, "ms": t, "default_ms": t_def, "evals": evals, "count": want}
+
+
 def tune_batch(b, repeats: int = 5) -> dict:
     timer = EventTimer()
     b.set_cfg(None)
@@ -165,7 +212,7 @@ def _graph_for(name: str):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workloads", default="rmat20,batch2000,c3,er4096")
-    ap.add_argument("--strategies", default="bitmap,cached_bsearch,merge")
+    ap.add_argument("--strategies", default="bitmap,cached_bsearch,merge,heuristic_table")
     ap.add_argument("--out", default=str(TUNED_PATH))
     ap.add_argument("--repeats", type=int, default=5)
     a = ap.parse_args(argv)
@@ -191,6 +238,12 @@ def main(argv=None) -> int:
         ctx.populateDevice()
         for st in a.strategies.split(","):
             t1 = time.time()
+            if st == "heuristic_table":
+                rec = tune_heuristic(ctx, a.repeats, cfg_for(wl, "heuristic"))
+                doc["entries"][f"{wl}/{st}"] = rec
+                log(f"{wl}/{st}: {rec['default_ms']:.3f} -> {rec['ms']:.3f} ms "
+                    f"{rec['thresholds']} ({rec['evals']} tables, {time.time() - t1:.1f}s)")
+                continue
             rec = tune_graph(ctx, st, a.repeats)
             doc["entries"][f"{wl}/{st}"] = rec
             log(f"{wl}/{st}: {rec['default_ms']:.3f} -> {rec['ms']:.3f} ms {rec['cfg']} "

@@ -398,7 +398,14 @@ __device__ __forceinline__ float fixed_cost(const FixedCosts &fc, float lx, int 
   return fc.f[fc.n - 1][o];
 }
 
-__global__ void __launch_bounds__(kRouteBins)
+// Fused inference, kSplit threads per bin (kRouteBins * kSplit threads, one workgroup): every
+// thread of a bin computes the first layer, thread q the hidden units [q*kHid/kSplit,
+// (q+1)*kHid/kSplit) of the second and its share of the output sums, which the bin's threads add
+// with shuffles. (One thread per bin, each doing all 1,312 multiply-adds in turn: 17 us.)
+constexpr int kSplit = 4;
+static_assert(kHid % kSplit == 0 && kRouteBins * kSplit <= 1024, "inference split");
+
+__global__ void __launch_bounds__(kRouteBins * kSplit)
     mlp_table_kernel(const float *__restrict__ P, const BinSums *__restrict__ sums, float log_nnz,
                      float nnz, uint32_t allowed, FixedCosts fixed, float margin,
                      RouteTable *__restrict__ table) {
@@ -411,18 +418,35 @@ __global__ void __launch_bounds__(kRouteBins)
     table->mixed = 0;
   }
   __syncthreads();
-  const int b = threadIdx.x;
-  float x[kIn], h1[kHid], h2[kHid], y[kOut], c[kOut];
+  const int b = threadIdx.x / kSplit, q = threadIdx.x % kSplit;
+  float x[kIn], h1[kHid], y[kOut], c[kOut];
   const BinSums s = read_sums(sums, b);
   bin_feature_vector(b, s, log_nnz, x);
-  mlp_forward(sP, x, h1, h2, y);
-  for (int o = 0; o < kOut; ++o) c[o] = exp2f(fminf(y[o], 30.f));
+  for (int k = 0; k < kHid; ++k) {
+    float a = sP[oB1 + k];
+    for (int i = 0; i < kIn; ++i) a += sP[oW1 + k * kIn + i] * x[i];
+    h1[k] = a > 0.f ? a : 0.f;
+  }
+  for (int o = 0; o < kOut; ++o) y[o] = 0.f;
+  for (int k = q * (kHid / kSplit); k < (q + 1) * (kHid / kSplit); ++k) {
+    float a = sP[oB2 + k];
+    for (int i = 0; i < kHid; ++i) a += sP[oW2 + k * kHid + i] * h1[i];
+    const float h2 = a > 0.f ? a : 0.f;
+    for (int o = 0; o < kOut; ++o) y[o] += sP[oW3 + o * kHid + k] * h2;
+  }
+  for (int o = 0; o < kOut; ++o) {
+#pragma unroll
+    for (int d = 1; d < kSplit; d <<= 1) y[o] += __shfl_xor(y[o], d, kWave);
+    y[o] += sP[oB3 + o];
+    c[o] = exp2f(fminf(y[o], 30.f));
+  }
+  // route subsets: the bin's first thread contributes its arcs at the subset's best cost
   for (uint32_t R = 1; R < 8; ++R) {
     if ((R & allowed) != R) continue;
     float best = INFINITY;
     for (int o = 0; o < kOut; ++o)
       if ((R >> o) & 1u) best = fminf(best, c[o]);
-    float contrib = s.arcs * best;
+    float contrib = q == 0 ? s.arcs * best : 0.f;
     contrib = wave_sum_f32(contrib);
     if (lane_id() == 0) atomicAdd(&s_tot[R], contrib);
   }
@@ -450,19 +474,32 @@ __global__ void __launch_bounds__(kRouteBins)
       by = c[o];
       best = o;
     }
-  table->route[b] = (uint8_t)best;  // output o == route code o (bitmap, cached, merge)
-  table->cost[b] = by;
-  const uint64_t bm = __ballot(best == 0);  // bins 64w .. 64w+63 of this wave
+  if (q == 0) {
+    table->route[b] = (uint8_t)best;  // output o == route code o (bitmap, cached, merge)
+    table->cost[b] = by;
+  }
+  // bins 16w .. 16w+15 of this wave (kSplit = 4 lanes each): rows with a bitmap bin
+  const uint64_t bm = __ballot(q == 0 && best == 0);
+  const uint64_t other = __ballot(q == 0 && best != 0);
   if (lane_id() == 0) {
     uint32_t rows = 0;
-    for (int k = 0; k < 64; ++k)
-      if ((bm >> k) & 1ull) rows |= 1u << ((b + k) >> 4);
-    atomicOr(&table->rowmask, rows);
-    if (bm != ~0ull) atomicOr(&table->mixed, 1u);
+    for (int k = 0; k < kWave; k += kSplit)
+      if ((bm >> k) & 1ull) rows |= 1u << ((b + k / kSplit) >> 4);
+    if (rows) atomicOr(&table->rowmask, rows);
+    if (other) atomicOr(&table->mixed, 1u);
   }
 }
 
 // ---- selector state per device -------------------------------------------------------------------
+
+// Thresholds of the plain heuristic, as log2 out-degrees: pivots with log2 d+(u) >= bitmap_from
+// go to the bitmap (the pivot's set serves >= 2^bitmap_from arcs); otherwise both lists small
+// (both logs <= small_max) -> merge; one list 2^skew times longer than the other -> cached
+// binary search; else merge. The defaults (6, 5, 3) are the library's plain heuristic; the last
+// three cases are the reference dispatcher's (include/ops/Intersection.cuh:182-187).
+struct HeuristicThresholds {
+  int32_t bitmap_from = 6, small_max = 5, skew = 3;
+};
 
 struct Model {
   float fixed_lx[4] = {0.f, 0.f, 0.f, 0.f};  // fixed route costs: points (log2(1 + nnz), ns)
@@ -485,6 +522,7 @@ struct DeviceState {
 
 std::mutex g_mu;
 std::map<int, DeviceState> *g_state = new std::map<int, DeviceState>();
+HeuristicThresholds g_thresholds;  // of every device's heuristic table (guarded by g_mu)
 
 void set_rowmask(RouteTable &t) {
   t.rowmask = 0;
@@ -496,14 +534,14 @@ void set_rowmask(RouteTable &t) {
   t.pad[0] = t.pad[1] = 0;
 }
 
-void heuristic_table_host(RouteTable &t) {
+void heuristic_table_host(RouteTable &t, const HeuristicThresholds &h = HeuristicThresholds()) {
   for (int b = 0; b < kRouteBins; ++b) {
     const int i = b >> 4, j = b & 15;  // log2 d+(u), log2 d+(v)
     uint8_t r;
-    if (i >= 6) r = kRouteBitmap;                  // the pivot's set serves >= 64 arcs
-    else if (i <= 5 && j <= 5) r = kRouteMerge;    // both lists small
-    else if (j - i >= 3) r = kRouteCached;         // one small, one large (skewed)
-    else r = kRouteMerge;                          // both large and even
+    if (i >= h.bitmap_from) r = kRouteBitmap;
+    else if (i <= h.small_max && j <= h.small_max) r = kRouteMerge;
+    else if (j - i >= h.skew) r = kRouteCached;
+    else r = kRouteMerge;
     t.route[b] = r;
     t.cost[b] = 1.f;
   }
@@ -514,7 +552,7 @@ int state_for(int dev, DeviceState **out) {
   DeviceState &s = (*g_state)[dev];
   if (!s.heuristic) {
     RouteTable h;
-    heuristic_table_host(h);
+    heuristic_table_host(h, g_thresholds);
     GARY_HIP(hipMalloc(&s.heuristic, sizeof(RouteTable)));
     GARY_HIP(hipMalloc(&s.sync_table, sizeof(RouteTable)));
     GARY_HIP(hipMalloc(&s.sums, sizeof(BinSums) * kRouteBins * kSumCopies));
@@ -560,11 +598,13 @@ int compute_bin_sums(const gary_device_graph &g, BinSums *sums, hipStream_t st) 
       (uint32_t *)sums, kRouteBins * kSumCopies * 3);
   GARY_HIP(hipGetLastError());
   if (g.csr.nnz == 0) return GARY_OK;
-  // a bounded sample whatever the graph size: at most ~32K runs of 64 arcs for the histogram
-  // and ~8K arcs for the overlap (every 8th run and every 2048th arc on smaller graphs)
+  // a bounded sample whatever the graph size: at most ~16K runs of 64 arcs for the histogram
+  // and ~4K arcs for the overlap (every 16th run and every 4096th arc on smaller graphs), so the
+  // overlap samples take one round of the 4,096 waves (32K runs and 8K samples: 26 us on
+  // R-MAT 20)
   const int64_t nruns = (g.csr.nnz + kWave - 1) / kWave;
-  const int32_t hist_every = (int32_t)std::max<int64_t>(8, (nruns + 32767) / 32768);
-  const int32_t ovl_every = (int32_t)std::max<int64_t>(2048, (g.csr.nnz + 8191) / 8192);
+  const int32_t hist_every = (int32_t)std::max<int64_t>(16, (nruns + 16383) / 16384);
+  const int32_t ovl_every = (int32_t)std::max<int64_t>(4096, (g.csr.nnz + 4095) / 4096);
   bin_features_kernel<<<1024, 256, 0, st>>>(g, hist_every, ovl_every, sums);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
@@ -709,7 +749,7 @@ int launch_selected(const gary_device_graph &g, bool learned, const PivotCfg &pc
   if (!learned || !s->model.trained)
     return launch_routed_ws(g, w, s->heuristic, pc, 0.f, cnt, st);
   GARY_TRY(compute_bin_sums(g, w.sums, st));
-  mlp_table_kernel<<<1, kRouteBins, 0, st>>>(s->model.P, w.sums,
+  mlp_table_kernel<<<1, kRouteBins * kSplit, 0, st>>>(s->model.P, w.sums,
                                              log2f(1.f + (float)g.csr.nnz), (float)g.csr.nnz,
                                              s->model.allowed, fixed_of(s->model),
                                              s->model.margin, w.table);
@@ -755,10 +795,41 @@ int gary_tc_count_routed(const gary_device_graph *g, const uint8_t *h_routes,
 
 int gary_selector_heuristic_table(uint8_t *h_routes) {
   GARY_API_BEGIN
+  using namespace gary;
   GARY_CHECK_ARG(h_routes, "gary_selector_heuristic_table: null pointer");
-  gary::RouteTable t;
-  gary::heuristic_table_host(t);
-  std::memcpy(h_routes, t.route, gary::kRouteBins);
+  RouteTable t;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    heuristic_table_host(t, g_thresholds);
+  }
+  std::memcpy(h_routes, t.route, kRouteBins);
+  return GARY_OK;
+  GARY_API_END
+}
+
+int gary_selector_set_heuristic(int32_t bitmap_from, int32_t small_max, int32_t skew) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(bitmap_from >= 0 && bitmap_from <= 16 && small_max >= 0 && small_max <= 16 &&
+                     skew >= 1 && skew <= 16,
+                 "gary_selector_set_heuristic: thresholds %d %d %d out of range", bitmap_from,
+                 small_max, skew);
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_thresholds.bitmap_from = bitmap_from;
+  g_thresholds.small_max = small_max;
+  g_thresholds.skew = skew;
+  RouteTable t;
+  heuristic_table_host(t, g_thresholds);
+  if (g_state->empty()) return GARY_OK;  // device tables are built from g_thresholds later
+  int cur = 0;
+  GARY_TRY(current_device(&cur));
+  for (auto &kv : *g_state) {
+    if (!kv.second.heuristic) continue;
+    GARY_HIP(hipSetDevice(kv.first));
+    GARY_HIP(hipDeviceSynchronize());  // no count in flight reads the old table
+    GARY_HIP(hipMemcpy(kv.second.heuristic, &t, sizeof t, hipMemcpyHostToDevice));
+  }
+  GARY_HIP(hipSetDevice(cur));
   return GARY_OK;
   GARY_API_END
 }
@@ -1013,7 +1084,7 @@ int gary_selector_learned_table(const gary_device_graph *g, uint8_t *h_routes, f
   GARY_TRY(state_for(dev, &s));
   if (!s->model.trained) return set_error(GARY_ERR_STATE, "the selector has no trained model");
   GARY_TRY(compute_bin_sums(*g, s->sums, nullptr));
-  mlp_table_kernel<<<1, kRouteBins>>>(s->model.P, s->sums, log2f(1.f + (float)g->csr.nnz),
+  mlp_table_kernel<<<1, kRouteBins * kSplit>>>(s->model.P, s->sums, log2f(1.f + (float)g->csr.nnz),
                                       (float)g->csr.nnz, s->model.allowed, fixed_of(s->model),
                                       s->model.margin, s->sync_table);
   GARY_HIP(hipGetLastError());

@@ -169,11 +169,16 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
   block_atomic_add(count, c, &s_acc);
 }
 
-// List-driven variant: one group per arc of `arcs[0 .. *n_arcs)` (arc = index into the CSR).
-template <int G, int METHOD>
+// List-driven variant of the routed path: one group of G lanes per arc, the arcs of the cached
+// list (two-phase cached binary search, cache depth n_cache) first, then those of the merge list
+// (arc = index into the CSR). One launch serves both lists; a group changes method only where
+// the lists meet.
+template <int G>
 __global__ void __launch_bounds__(256) tc_edge_list_kernel(gary_device_graph g, int32_t n_cache,
-                                                           const uint32_t *__restrict__ arcs,
-                                                           const int32_t *__restrict__ n_arcs,
+                                                           const uint32_t *__restrict__ cached,
+                                                           const int32_t *__restrict__ n_cached,
+                                                           const uint32_t *__restrict__ merge,
+                                                           const int32_t *__restrict__ n_merge,
                                                            unsigned long long *count) {
   extern __shared__ int32_t smem[];
   const int glane = lane_id() & (G - 1);
@@ -181,16 +186,19 @@ __global__ void __launch_bounds__(256) tc_edge_list_kernel(gary_device_graph g, 
   const int gib = threadIdx.x / G;
   const int64_t ngroups = (int64_t)gridDim.x * gpb;
   int32_t *cache = smem + (size_t)gib * n_cache;
-  const int64_t E = *n_arcs;
+  const int64_t nc = *n_cached, E = nc + *n_merge;
   const uint32_t m = (uint32_t)g.csr.m;
   uint32_t c = 0;
   for (int64_t k = (int64_t)blockIdx.x * gpb + gib; k < E; k += ngroups) {
-    const uint32_t e = arcs[k];
+    const uint32_t e = k < nc ? cached[k] : merge[k - nc];
     const int32_t src = g.coo.rowIdx[e], dst = g.csr.colIdx[e];
     if ((uint32_t)src >= m || (uint32_t)dst >= m) continue;
     const int32_t *A = g.csr.colIdx + g.csr.indPtr[src];
     const int32_t *B = g.csr.colIdx + g.csr.indPtr[dst];
-    c += isect_group<G, METHOD>(A, g.csr.deg[src], B, g.csr.deg[dst], cache, n_cache, glane);
+    if (k < nc)
+      c += isect_group<G, kIsectCached>(A, g.csr.deg[src], B, g.csr.deg[dst], cache, n_cache, glane);
+    else
+      c += isect_group<G, kIsectMerge>(A, g.csr.deg[src], B, g.csr.deg[dst], nullptr, 0, glane);
   }
   __shared__ unsigned long long s_acc;
   block_atomic_add(count, c, &s_acc);
@@ -1175,10 +1183,10 @@ int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long 
 int launch_edge_lists(const gary_device_graph &g, const uint32_t *cached, const int32_t *n_cached,
                       const uint32_t *merge, const int32_t *n_merge, unsigned long long *cnt,
                       hipStream_t st) {
-  tc_edge_list_kernel<16, kIsectCached><<<4096, 256, (256 / 16) * 32 * sizeof(int32_t), st>>>(
-      g, 32, cached, n_cached, cnt);
-  GARY_HIP(hipGetLastError());
-  tc_edge_list_kernel<16, kIsectMerge><<<4096, 256, 0, st>>>(g, 0, merge, n_merge, cnt);
+  // 2048 workgroups (eight per CU): enough groups in flight for long lists, and a short launch
+  // when the table sends no arc to the edge routes
+  tc_edge_list_kernel<16><<<2048, 256, (256 / 16) * 32 * sizeof(int32_t), st>>>(
+      g, 32, cached, n_cached, merge, n_merge, cnt);
   GARY_HIP(hipGetLastError());
   return GARY_OK;
 }

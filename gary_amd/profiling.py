@@ -99,17 +99,28 @@ def read_traffic(md_path: str | Path) -> int | None:
     return int(m.group(1)) if m else None
 
 
+def read_commit(md_path: str | Path) -> str | None:
+    """The `source commit:` line of a summary (the tree the profiled command ran)."""
+    p = Path(md_path)
+    if not p.exists():
+        return None
+    m = re.search(r"^source commit:\s*(\S+)", p.read_text(), re.M)
+    return m.group(1) if m else None
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
     k = sub.add_parser("kernels")
     k.add_argument("stats_csv")
     k.add_argument("--cmd", required=True)
+    k.add_argument("--commit", default=None, help="commit of the profiled tree")
     k.add_argument("--title", default="rocprofv3 kernel trace")
     c = sub.add_parser("counters")
     c.add_argument("csvs", nargs="+")
     c.add_argument("--kernel", required=True)
     c.add_argument("--cmd", required=True)
+    c.add_argument("--commit", default=None, help="commit of the profiled tree")
     c.add_argument("--title", default="PMC counters per dispatch")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv")
@@ -117,8 +128,11 @@ def main(argv=None) -> int:
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("--cmd", required=True)
+    t.add_argument("--commit", default=None, help="commit of the profiled tree")
     t.add_argument("--title", default="HBM traffic per launch")
     a = ap.parse_args(argv)
+    if a.commit:
+        a.cmd = f"{a.cmd}\n\nsource commit: {a.commit}"
     if a.what == "kernels":
         print(f"# {a.title}\n\ncommand:\n\n    {a.cmd}\n\n{kernel_table(a.stats_csv)}")
     elif a.what == "counters":

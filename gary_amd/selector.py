@@ -31,6 +31,7 @@ _P, _F = C.c_void_p, C.c_float
 _lib.bind("gary_tc_count_routed", C.c_int, [C.POINTER(DeviceGraph), _P, _P, _F,
                                             C.POINTER(LaunchCfg), _P, _P])
 _lib.bind("gary_selector_heuristic_table", C.c_int, [_P])
+_lib.bind("gary_selector_set_heuristic", C.c_int, [C.c_int32, C.c_int32, C.c_int32])
 _lib.bind("gary_selector_bin_features", C.c_int, [C.POINTER(DeviceGraph), _P, _P])
 _lib.bind("gary_mlp_forward", C.c_int, [_P, _P, C.c_int32, _P])
 _lib.bind("gary_mlp_loss_grad", C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P])
@@ -51,6 +52,15 @@ def heuristic_table() -> np.ndarray:
     t = np.zeros(NBINS, np.uint8)
     check(lib.gary_selector_heuristic_table(ptr(t)))
     return t
+
+
+HEURISTIC_DEFAULT = {"bitmap_from": 6, "small_max": 5, "skew": 3}
+
+
+def set_heuristic(bitmap_from: int = 6, small_max: int = 5, skew: int = 3) -> None:
+    """Thresholds of the heuristic table (log2 out-degrees): log2 d+(u) >= bitmap_from -> bitmap;
+    both logs <= small_max -> merge; log2 d+(v) - log2 d+(u) >= skew -> cached; else merge."""
+    check(lib.gary_selector_set_heuristic(int(bitmap_from), int(small_max), int(skew)))
 
 
 def dispatch_table() -> np.ndarray:

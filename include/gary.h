@@ -319,8 +319,14 @@ enum {
 int gary_tc_count_routed(const gary_device_graph *g, const uint8_t *h_routes,
                          const float *h_costs, float cost_big, const gary_launch_cfg *cfg,
                          uint64_t *d_count, void *stream);
-/* The plain degree-threshold table used by GARY_TC_HEURISTIC. */
+/* The plain degree-threshold table used by GARY_TC_HEURISTIC (with the current thresholds). */
 int gary_selector_heuristic_table(uint8_t *h_routes);
+/* Thresholds of that table (one setting for the process, every device), as log2 out-degrees of
+ * the arc (u,v): log2 d+(u) >= bitmap_from -> bitmap; else both logs <= small_max -> merge;
+ * log2 d+(v) - log2 d+(u) >= skew -> cached binary search; else merge. Library defaults:
+ * (6, 5, 3); the autotuner searches them per workload. Waits for each device that already holds
+ * a table (no count in flight reads it while it changes). */
+int gary_selector_set_heuristic(int32_t bitmap_from, int32_t small_max, int32_t skew);
 
 /* MLP selector: 6 bin features -> 32 -> 32 (ReLU) -> 3 predicted log2(ns per arc) for
  * {bitmap, cached binary search, merge}. Parameters: W1[32][6] b1[32] W2[32][32] b2[32]

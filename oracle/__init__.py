@@ -38,6 +38,7 @@ def _load() -> C.CDLL:
         "or_csr_from_edges": (C.c_int, [I32, I64, P, P, I32, P, P, I64, P]),
         "or_tri_count": (U64, [I32, P, P, I32]),
         "or_tri_count_range": (U64, [I32, I32, P, P, I32]),
+        "or_tri_count_batch": (U64, [I32, P, P, P, I32, P]),
         "or_omp_threads": (I32, [I32]),
         "or_hom4": (C.c_int, [I32, P, P, I32, P]),
         "or_hom4_ranked": (C.c_int, [I32, P, P, I32, P]),
@@ -133,6 +134,17 @@ def tri_count_range(indPtr, colIdx, vbegin: int, vend: int, threads: int = 0) ->
     indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
     return int(lib.or_tri_count_range(vbegin, vend, indPtr.ctypes.data, colIdx.ctypes.data,
                                       threads))
+
+
+def tri_count_batch(vbase, indPtr, colIdx, threads: int = 0) -> np.ndarray:
+    """Per-graph triangle counts of a disjoint union (graph g on [vbase[g], vbase[g+1])),
+    parallel over graphs, each graph sequential."""
+    vbase = _a(vbase, np.int32)
+    indPtr, colIdx = _a(indPtr, np.int64), _a(colIdx, np.int32)
+    out = np.zeros(max(vbase.size - 1, 1), np.uint64)
+    lib.or_tri_count_batch(vbase.size - 1, vbase.ctypes.data, indPtr.ctypes.data,
+                           colIdx.ctypes.data, threads, out.ctypes.data)
+    return out[:vbase.size - 1]
 
 
 def omp_threads(threads: int = 0) -> int:

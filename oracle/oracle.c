@@ -250,6 +250,32 @@ uint64_t or_tri_count_range(int32_t vbegin, int32_t vend, const int64_t *indPtr,
   return count;
 }
 
+uint64_t or_tri_count_batch(int32_t ngraphs, const int32_t *vbase, const int64_t *indPtr,
+                            const int32_t *colIdx, int32_t threads, uint64_t *counts) {
+  /* TriQueryTest.cu:17-37 per graph; the loop over graphs is the parallel one */
+  uint64_t total = 0;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+  for (int32_t g = 0; g < ngraphs; ++g) {
+    uint64_t c = 0;
+    for (int32_t v = vbase[g]; v < vbase[g + 1]; ++v) {
+      const int32_t *vAdj = colIdx + indPtr[v];
+      const int64_t dv = indPtr[v + 1] - indPtr[v];
+      for (int64_t i = 0; i < dv; ++i) {
+        const int32_t u = vAdj[i];
+        c += or_intersect_count(colIdx + indPtr[u], indPtr[u + 1] - indPtr[u], vAdj, dv);
+      }
+    }
+    if (counts) counts[g] = c;
+    total += c;
+  }
+  return total;
+}
+
 /* ---- 4-vertex homomorphism counts ------------------------------------------------------------ */
 
 typedef unsigned __int128 u128;

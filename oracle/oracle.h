@@ -66,6 +66,12 @@ uint64_t or_tri_count(int32_t m, const int64_t *indPtr, const int32_t *colIdx, i
 uint64_t or_tri_count_range(int32_t vbegin, int32_t vend, const int64_t *indPtr,
                             const int32_t *colIdx, int32_t threads);
 
+/* cpuTriCount (test/TriQueryTest.cu:17-37) over a batch of graphs stored as one disjoint union
+ * (graph g owns vertices [vbase[g], vbase[g+1])): counts[g] = triangles of graph g, graphs
+ * taken in parallel (OpenMP dynamic), each counted sequentially. Returns the sum. */
+uint64_t or_tri_count_batch(int32_t ngraphs, const int32_t *vbase, const int64_t *indPtr,
+                            const int32_t *colIdx, int32_t threads, uint64_t *counts);
+
 /* Number of OpenMP threads a parallel region of `threads` (<= 0 = default) really uses. */
 int32_t or_omp_threads(int32_t threads);
 

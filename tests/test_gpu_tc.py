@@ -235,6 +235,52 @@ def test_learned_table_is_the_models(rmat20):
         selector.load()
 
 
+def _decide(costs, arcs, allowed, margin, fixed):
+    """The fused inference's decision restated: the route subset R (within `allowed`) with the
+    least sum of fixed costs plus sum over bins of arcs * min_{r in R} cost, leaving the
+    all-bitmap routing only for a predicted gain above `margin`; each bin its cheapest route in
+    R. Returns (routes, margin of the subset choice)."""
+    tot = {}
+    for R in range(1, 8):
+        if R & allowed != R:
+            continue
+        sel = [o for o in range(3) if R >> o & 1]
+        tot[R] = float((arcs * costs[:, sel].min(axis=1)).sum()) + sum(max(0.0, fixed[o]) for o in sel)
+    bestR = min(tot, key=tot.get)
+    if allowed & 1 and bestR != 1 and tot[bestR] > (1 - margin) * tot[1]:
+        bestR = 1
+    sel = [o for o in range(3) if bestR >> o & 1]
+    routes = np.array([sel[int(np.argmin(costs[b, sel]))] for b in range(costs.shape[0])])
+    others = sorted(tot.values())
+    gap = (others[1] - others[0]) / others[0] if len(others) > 1 else 1.0
+    return routes, gap
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_fused_inference_matches_forward_kernel(er_ctx, seed):
+    """The fused table kernel (features -> MLP split over threads -> subset choice) decides as
+    the standalone forward kernel's outputs on the same features do, for random models that
+    may leave the bitmap (no fixed costs). Bins whose two cheapest routes are within 1% are not
+    compared (the sampled features are float sums whose order varies between runs)."""
+    g, ctx, want = er_ctx
+    rng = np.random.default_rng(seed)
+    try:
+        p = (rng.standard_normal(selector.NPARAMS) * 0.5).astype(np.float32)
+        selector.set_params(p, True, 0.0, allowed=7, margin=0.0)
+        selector.set_fixed_costs([1.0], [[0.0, 0.0, 0.0]])
+        X, arcs = selector.bin_features(ctx)
+        costs = np.exp2(np.minimum(selector.mlp_forward(p, X), 30.0)).astype(np.float64)
+        want_routes, gap = _decide(costs, arcs.astype(np.float64), 7, 0.0, [0.0, 0.0, 0.0])
+        routes, _ = selector.learned_table(ctx)
+        srt = np.sort(costs, axis=1)
+        clear = (srt[:, 1] - srt[:, 0]) > 0.01 * srt[:, 0]
+        if gap > 0.01:
+            assert np.array_equal(routes[clear], want_routes[clear])
+        assert gq.triangleCountHom(ctx, gq.AUTO_LEARNED) == want
+    finally:
+        selector.load()
+
+
 def test_hub_fallback_star():
     """A pivot with out-degree 5000 (id orientation keeps src > dst) exceeds every LDS set and
     goes through the edge-by-edge fallback; 2,500 triangles (center, 2i, 2i+1)."""

@@ -290,6 +290,9 @@ def test_tri_count_range_partitions_and_unions(golden_graphs):
         parts = [oracle.tri_count_range(ip, ci, a, b) for a, b in zip(edges, edges[1:])]
         assert sum(parts) == total
     assert oracle.tri_count_range(ip, ci, 7, 7) == 0
+    # the batch form (parallel over graphs, each graph sequential) gives every graph's count
+    per = oracle.tri_count_batch(np.array(bases, np.int32), ip, ci)
+    assert per.tolist() == [r["triangles"] for r in golden_graphs]
 
 
 # ---- k-cliques (no reference counterpart; pinned by networkx, closed forms, brute force) -------

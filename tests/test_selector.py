@@ -46,6 +46,25 @@ def test_heuristic_table_shape():
     assert t[1 * 16 + 9] == selector.CACHED     # skewed
 
 
+def test_heuristic_thresholds():
+    """gary_selector_set_heuristic moves the table's cases (no GPU needed for the table)."""
+    try:
+        selector.set_heuristic(0, 5, 3)
+        assert np.all(selector.heuristic_table() == selector.BITMAP)
+        selector.set_heuristic(16, 16, 3)
+        assert np.all(selector.heuristic_table() == selector.MERGE)
+        selector.set_heuristic(16, 2, 2)
+        t = selector.heuristic_table()
+        assert t[1 * 16 + 1] == selector.MERGE and t[1 * 16 + 3] == selector.CACHED
+        assert t[5 * 16 + 4] == selector.MERGE and t[2 * 16 + 9] == selector.CACHED
+    finally:
+        selector.set_heuristic(**selector.HEURISTIC_DEFAULT)
+    t = selector.heuristic_table()
+    assert t[6 * 16 + 3] == selector.BITMAP and t[1 * 16 + 9] == selector.CACHED
+    with pytest.raises(Exception):
+        selector.set_heuristic(3, 5, 0)
+
+
 @pytest.mark.gpu
 def test_device_mlp_matches_fp64_reference():
     p, X, T, M = _rand_problem(3, n=200)
