@@ -659,21 +659,46 @@ int routed_ws(const gary_device_graph &g, hipStream_t st, RoutedWs &w) {
   return GARY_OK;
 }
 
+// What the host knows of a route table: which routes it uses. A table computed on the device
+// (the learned one) is unknown, and every launch is made.
+struct TableShape {
+  bool any_bitmap = true;  // some bin goes to the pivot kernels
+  bool any_edge = true;    // some bin goes to the edge-centric kernels
+  bool mixed = true;       // some bin goes elsewhere than the bitmap (per-arc route bytes needed)
+};
+
+TableShape shape_of(const RouteTable &t) {
+  TableShape sh;
+  sh.any_bitmap = sh.any_edge = false;
+  for (int b = 0; b < kRouteBins; ++b) {
+    sh.any_bitmap |= t.route[b] == kRouteBitmap;
+    sh.any_edge |= t.route[b] == kRouteCached || t.route[b] == kRouteMerge;
+  }
+  sh.mixed = t.mixed != 0;
+  return sh;
+}
+
 // Counts with every arc routed by `table` (device pointer, read by the kernels of this call).
+// Launches the table cannot need are left out (shape from the host copy of the table).
 int launch_routed_ws(const gary_device_graph &g, const RoutedWs &w, const RouteTable *table,
-                     const PivotCfg &pc, float cost_big, unsigned long long *cnt, hipStream_t st) {
+                     const PivotCfg &pc, float cost_big, unsigned long long *cnt, hipStream_t st,
+                     const TableShape &sh = TableShape()) {
   const int32_t m = g.csr.m;
-  fill_zero_kernel_sel<<<1, 64, 0, st>>>((uint32_t *)w.counts, 2);
-  const bool by_cost = cost_big > 0.f;
-  if (by_cost)
-    fill_zero_kernel_sel<<<(int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0, st>>>(
-        (uint32_t *)w.pcost, m);
-  route_kernel<<<(int)((g.csr.nnz + kRouteChunk - 1) / kRouteChunk), 256, 0, st>>>(
-      g, table, w.route, by_cost ? w.pcost : nullptr, w.cached, w.merge, w.counts);
-  GARY_HIP(hipGetLastError());
-  GARY_TRY(launch_pivot_core(g, pc, w.lists, w.stride, w.lists + 3 * w.stride, nullptr, nullptr,
-                             0, cnt, st, true, w.route, by_cost ? w.pcost : nullptr, cost_big,
-                             table));
+  const bool by_cost = cost_big > 0.f && sh.any_bitmap;
+  if (sh.mixed || by_cost) {
+    fill_zero_kernel_sel<<<1, 64, 0, st>>>((uint32_t *)w.counts, 2);
+    if (by_cost)
+      fill_zero_kernel_sel<<<(int)std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256, 0,
+                             st>>>((uint32_t *)w.pcost, m);
+    route_kernel<<<(int)((g.csr.nnz + kRouteChunk - 1) / kRouteChunk), 256, 0, st>>>(
+        g, table, w.route, by_cost ? w.pcost : nullptr, w.cached, w.merge, w.counts);
+    GARY_HIP(hipGetLastError());
+  }
+  if (sh.any_bitmap)
+    GARY_TRY(launch_pivot_core(g, pc, w.lists, w.stride, w.lists + 3 * w.stride, nullptr, nullptr,
+                               0, cnt, st, true, w.route, by_cost ? w.pcost : nullptr, cost_big,
+                               table));
+  if (!sh.any_edge) return GARY_OK;
   return launch_edge_lists(g, w.cached, w.counts, w.merge, w.counts + 1, cnt, st);
 }
 
@@ -746,8 +771,11 @@ int launch_selected(const gary_device_graph &g, bool learned, const PivotCfg &pc
   GARY_TRY(state_for(dev, &s));
   RoutedWs w;
   GARY_TRY(routed_ws(g, st, w));
-  if (!learned || !s->model.trained)
-    return launch_routed_ws(g, w, s->heuristic, pc, 0.f, cnt, st);
+  if (!learned || !s->model.trained) {
+    RouteTable h;
+    heuristic_table_host(h, g_thresholds);
+    return launch_routed_ws(g, w, s->heuristic, pc, 0.f, cnt, st, shape_of(h));
+  }
   GARY_TRY(compute_bin_sums(g, w.sums, st));
   mlp_table_kernel<<<1, kRouteBins * kSplit, 0, st>>>(s->model.P, w.sums,
                                              log2f(1.f + (float)g.csr.nnz), (float)g.csr.nnz,
@@ -789,7 +817,8 @@ int gary_tc_count_routed(const gary_device_graph *g, const uint8_t *h_routes,
   pc.bin_hub = bh;
   pc.grid = grid;
   if (cfg && cfg->block) pc.waves = cfg->block == 128 ? 2 : cfg->block == 512 ? 8 : 4;
-  return launch_routed_ws(*g, w, w.table, pc, cost_big, (unsigned long long *)d_count, st);
+  return launch_routed_ws(*g, w, w.table, pc, cost_big, (unsigned long long *)d_count, st,
+                          shape_of(t));
   GARY_API_END
 }
 

@@ -2,8 +2,9 @@
 
     python -m gary_amd.train_selector [--out gary_amd/selector_model.json] [--epochs 4000]
 
-Training graphs (default): Erdős–Rényi G(4096, 65536) and R-MAT scales 18 and 20 (edgefactor
-16), degree-oriented. Edge routes (cached binary search, merge) are timed bin by bin against the
+Training graphs (default): Erdős–Rényi G(4096, 65536), the union of the first 100 graphs of the
+batched workload (small ER and R-MAT graphs, where the per-bin best route is mixed) and R-MAT
+scales 18 and 20 (edgefactor 16), degree-oriented. Edge routes (cached binary search, merge) are timed bin by bin against the
 all-skip table; the bitmap route bin by bin as the saving of skipping the bin from the all-bitmap
 table, its remainder being its fixed cost (selector.time_bins). The per-arc costs train the MLP on
 the device; the fixed costs are kept per training graph and interpolated in log2(nnz).
@@ -36,6 +37,12 @@ def prepare(name: str):
         n, s, d = graph.chung_lu_4m()
     elif name.startswith("rmat"):
         n, s, d = graph.rmat(int(name[4:]))
+    elif name.startswith("batch"):  # the first N graphs of the batched workload, as one union
+        from . import batch
+        g, _ = batch.build_union(batch.batch_specs(int(name[5:])))
+        ctx = DeviceGraphCtx(g)
+        ctx.populateDevice()
+        return g, ctx
     else:
         raise ValueError(name)
     g = graph.oriented(n, s, d)
@@ -71,7 +78,7 @@ def evaluate(nm, g, ctx, ns, arcs, margin, cost_big) -> dict:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--graphs", default="er4096,rmat18,rmat20")
+    ap.add_argument("--graphs", default="er4096,batch100,rmat18,rmat20")
     ap.add_argument("--eval", default="c3", help="held-out graphs (timed, not trained on)")
     ap.add_argument("--out", default=str(selector.MODEL_PATH))
     ap.add_argument("--epochs", type=int, default=4000)

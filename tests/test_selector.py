@@ -128,7 +128,7 @@ def test_bin_features_track_exact_histogram():
     assert np.all(arcs[exact == 0] == 0)  # a sample only lands in a bin that has arcs
     # the overlap feature against its definition: every ovl_every-th arc (u, v), up to 64
     # evenly spaced keys of the shorter of N+(u), N+(v), the fraction found in the longer
-    ovl_every = max(2048, -(-g.nnz // 8192))
+    ovl_every = max(4096, -(-g.nnz // 4096))
     row = np.repeat(np.arange(g.m), g.deg)
     sums = np.zeros(256)
     cnt = np.zeros(256)
