@@ -55,8 +55,14 @@ def test_shipped_file_is_well_formed():
     doc = json.loads(autotune.TUNED_PATH.read_text())
     for key, rec in doc["entries"].items():
         wl, st = key.split("/")
-        assert st in gq.STRATEGIES
-        assert set(rec["cfg"]) <= set(autotune.FIELDS)
+        if st == "heuristic_table":  # route thresholds of the heuristic, not launch knobs
+            assert set(rec["thresholds"]) == set(autotune.HEURISTIC_SPACE)
+            for k, v in rec["thresholds"].items():
+                assert v in autotune.HEURISTIC_SPACE[k], key
+            assert autotune.heuristic_for(wl) == rec["thresholds"]
+        else:
+            assert st in gq.STRATEGIES
+            assert set(rec["cfg"]) <= set(autotune.FIELDS)
         assert rec["ms"] <= rec["default_ms"] * 1.05, key
 
 
