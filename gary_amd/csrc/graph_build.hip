@@ -670,6 +670,12 @@ int build_csr(int32_t n, int64_t nt, const int32_t *src, const int32_t *dst, int
 
 struct gary_ctx;
 namespace gary {
+// Exclusive scan of a[0..len) in place on `s` (values and total below 2^32); `part` holds
+// exclusive_scan_parts(len) values.
+uint32_t exclusive_scan_parts(uint32_t len) { return tiles_of(len) + 1; }
+int exclusive_scan_u32(uint32_t *a, uint32_t len, uint32_t *part, hipStream_t s) {
+  return scan_u32(a, len, part, s);
+}
 // The Chung-Lu cumulative weight table (host_graph.cpp).
 void chung_lu_cum(int32_t n, int64_t ntuples, double gamma, double max_w,
                   std::vector<uint64_t> &cum);

@@ -25,6 +25,7 @@
 
 #include "common.h"
 #include "device_util.h"
+#include "tc_internal.h"
 #include "workspace.h"
 
 namespace gary {
@@ -917,6 +918,17 @@ __global__ void __launch_bounds__(kHomBlock)
 // cursor: its first four elements are read together, and only a run longer than that is ended
 // by a binary search. The runs of 64 neighbours v are flattened over a wave (as in the triangle
 // kernel's row streaming) so that every lane counts one endpoint per step.
+//
+// Run index. Each neighbour's run in a window is read from a per-call index of the CSR: for
+// every row v, one 64-bit entry per window of kC4Win ids that N(v) touches (window id << 40 |
+// position of its first element of N(v) there) and a last entry at the row's end. A visit with
+// a run reads two consecutive entries for the run's bounds and keeps the next entry's window id
+// as the neighbour's next window, so no run end is searched for in N(v). (Searching each run's
+// end in N(v) cost log2 d(v) dependent reads, in as many cache lines, per run: 1.0e9 runs of 36
+// endpoints on average on R-MAT 22.)
+constexpr int kRunPosBits = 40;
+constexpr uint64_t kRunPosMask = (1ull << kRunPosBits) - 1;
+constexpr int32_t kRunEndWin = (1 << 23) - 1;  // window id of a row's last entry
 constexpr int kC4Words = 32768; // 128 KiB of LDS counters: one workgroup per CU
 constexpr int kC4Win = 65536;   // ids per window: 16-bit counters, two per word
 constexpr int kC4Group = 16;    // windows per work item (each item starts its cursors with a
@@ -926,11 +938,59 @@ constexpr int kC4Waves = kC4Block / kWave;
 constexpr int kC4Touch = 2048;  // endpoints listed per window; a window with more is swept whole
 constexpr int kC4Steps = 4;     // 64-endpoint steps whose loads are issued together
 
+// Entries per row of the run index: the windows N(v) touches, plus the row's last entry. One wave
+// per row; cnt[m] = 0 (the scan's total slot).
+__global__ void __launch_bounds__(256) run_count_kernel(gary_device_graph g, int32_t win_shift,
+                                                        uint32_t *__restrict__ cnt) {
+  const int lane = lane_id();
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  for (int64_t v = global_wave_id(); v <= g.csr.m; v += num_global_waves()) {
+    if (v == g.csr.m) {
+      if (lane == 0) cnt[v] = 0;
+      continue;
+    }
+    const int64_t p0 = g.csr.indPtr[v];
+    const int32_t d = g.csr.deg[v];
+    uint32_t c = 0;
+    for (int32_t e0 = 0; e0 < d; e0 += kWave) {
+      const int32_t e = e0 + lane;
+      const bool first = e < d && (e == 0 || (col[p0 + e] >> win_shift) != (col[p0 + e - 1] >> win_shift));
+      c += (uint32_t)__popcll(__ballot(first));
+    }
+    if (lane == 0) cnt[v] = c + 1;
+  }
+}
+
+// The entries themselves, at ptr[v] (the scanned counts), in window order.
+__global__ void __launch_bounds__(256) run_write_kernel(gary_device_graph g, int32_t win_shift,
+                                                        const uint32_t *__restrict__ ptr,
+                                                        uint64_t *__restrict__ runs) {
+  const int lane = lane_id();
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  for (int64_t v = global_wave_id(); v < g.csr.m; v += num_global_waves()) {
+    const int64_t p0 = g.csr.indPtr[v];
+    const int32_t d = g.csr.deg[v];
+    uint64_t at = ptr[v];
+    for (int32_t e0 = 0; e0 < d; e0 += kWave) {
+      const int32_t e = e0 + lane;
+      const int32_t w = e < d ? col[p0 + e] : 0;
+      const bool first = e < d && (e == 0 || (w >> win_shift) != (col[p0 + e - 1] >> win_shift));
+      const uint64_t mask = __ballot(first);
+      if (first)
+        runs[at + lanes_below(mask)] =
+            ((uint64_t)(w >> win_shift) << kRunPosBits) | (uint64_t)(p0 + e);
+      at += (uint64_t)__popcll(mask);
+    }
+    if (lane == 0) runs[at] = ((uint64_t)kRunEndWin << kRunPosBits) | (uint64_t)(p0 + d);
+  }
+}
+
 __global__ void __launch_bounds__(kC4Block)
     c4_window_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
                      const int32_t *__restrict__ heavy, const int32_t *__restrict__ nheavy_p,
-                     int32_t jmax, int64_t *__restrict__ cursors,
-                     int32_t *__restrict__ next_ep, int64_t stride,
+                     int32_t jmax, const uint32_t *__restrict__ run_ptr,
+                     const uint64_t *__restrict__ runs, int64_t *__restrict__ cursors,
+                     int32_t *__restrict__ next_win, int64_t stride,
                      int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
   __shared__ uint32_t cnt[kC4Words];
   __shared__ int32_t touch[kC4Touch];  // window offsets whose counter left zero
@@ -940,8 +1000,8 @@ __global__ void __launch_bounds__(kC4Block)
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int64_t nheavy = *nheavy_p;
   const int64_t nitems = nheavy * jmax;
-  int64_t *cur = cursors + (int64_t)blockIdx.x * 2 * stride;
-  int32_t *nx = next_ep + (int64_t)blockIdx.x * stride;
+  int64_t *cur = cursors + (int64_t)blockIdx.x * stride;     // run-index entry of the next run
+  int32_t *nx = next_win + (int64_t)blockIdx.x * stride;     // window id of that run
   uint64_t acc = 0;
   for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
   if (threadIdx.x == 0) s_ntouch = 0;
@@ -960,51 +1020,75 @@ __global__ void __launch_bounds__(kC4Block)
     const int32_t *Nu = col + g.csr.indPtr[u];
     for (int32_t i = threadIdx.x; i < nl; i += kC4Block) {  // cursors at the first window
       const int32_t v = Nu[i];
-      const int64_t pv = g.csr.indPtr[v];
-      const int32_t dv = g.csr.deg[v];
-      const int64_t c = pv + lower_bound(col + pv, dv, k0 * kC4Win);
-      cur[2 * i] = c;
-      cur[2 * i + 1] = pv + dv;
-      nx[i] = c < pv + dv ? col[c] : INT32_MAX;
+      int64_t lo = run_ptr[v], hi = (int64_t)run_ptr[v + 1] - 1;  // the last entry ends the row
+      while (lo < hi) {  // first entry of a window >= k0
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int32_t)(runs[mid] >> kRunPosBits) < k0) lo = mid + 1;
+        else hi = mid;
+      }
+      cur[i] = lo;
+      nx[i] = (int32_t)(runs[lo] >> kRunPosBits);
     }
     __syncthreads();
     // neighbours per wave: 64, or for fewer than kC4Block neighbours an even share, so every
     // wave of the (one per CU) workgroup has loads in flight
     const int32_t per = nl < kC4Block ? (nl + kC4Waves - 1) / kC4Waves : kWave;
-    const bool packed = nl < 65536;  // counts fit 16 bits
-    const int32_t span = packed ? kC4Win : kC4Words;
-    for (int32_t k = k0 * (kC4Win / span); k < k1 * (kC4Win / span); ++k) {
-      const int32_t wlo = k * span;
-      if (wlo >= u) break;
-      const int32_t whi = min(wlo + span, u);
+    // Passes over the index windows k0 .. k1 - 1 (kC4Win ids each). A count c_u(w) is at most
+    // min(nl, d(w)), and ids are ranked by degree, so d(w) <= deg[whi - 1] in a pass over
+    // [wlo, whi). The counter width of a pass is the narrowest that cannot overflow:
+    //   8 bits, 4 per word: 2 * kC4Win ids, two index windows (their runs joined);
+    //   16 bits, 2 per word: one index window;
+    //   32 bits: half an index window (the run is read in both halves; the cursor moves after
+    //   the second).
+    int32_t k = k0, half = 0;
+    while (k < k1 && k * kC4Win < u) {
+      const int32_t wbase = k * kC4Win;
+      int32_t bits, wlo, whi, nidx = k + 1 < k1 ? 2 : 1;
+      const int32_t whi8 = min(wbase + nidx * kC4Win, u), whi16 = min(wbase + kC4Win, u);
+      if (nl < 256 || g.csr.deg[whi8 - 1] < 256) {
+        bits = 8;
+        wlo = wbase;
+        whi = whi8;
+      } else if (nl < 65536 || g.csr.deg[whi16 - 1] < 65536) {
+        bits = 16;
+        nidx = 1;
+        wlo = wbase;
+        whi = whi16;
+      } else {
+        nidx = 1;
+        bits = 32;
+        wlo = wbase + half * (kC4Win / 2);
+        whi = min(wlo + kC4Win / 2, u);
+      }
+      // the cursors move at the end of the index windows of this pass
+      const bool advance = bits != 32 || whi == min(wbase + kC4Win, u);
+      const int lgc = bits == 8 ? 2 : bits == 16 ? 1 : 0;  // log2 counters per word
+      const uint32_t cmask = bits == 32 ? 0xffffffffu : (1u << bits) - 1;
       for (int32_t i0 = wib * per; i0 < nl; i0 += kC4Waves * per) {
         const int32_t i = i0 + lane;
         int64_t p = 0;
         int32_t len = 0;
-        // endpoints w in [wlo, whi): a prefix of N(v) from the cursor. nx[i] caches the next
-        // endpoint, so a neighbour with none in this window costs one 4-byte load.
-        if (lane < per && i < nl && nx[i] < whi) {
-          p = cur[2 * i];
-          const int64_t vend = cur[2 * i + 1];
-          const int32_t e1 = p + 1 < vend ? col[p + 1] : INT32_MAX;
-          const int32_t e2 = p + 2 < vend ? col[p + 2] : INT32_MAX;
-          const int32_t e3 = p + 3 < vend ? col[p + 3] : INT32_MAX;
-          int32_t after;  // the endpoint following the run
-          if (e1 >= whi) {
-            len = 1;
-            after = e1;
-          } else if (e2 >= whi) {
-            len = 2;
-            after = e2;
-          } else if (e3 >= whi) {
-            len = 3;
-            after = e3;
-          } else {
-            len = p + 4 < vend ? 4 + lower_bound(col + p + 4, (int32_t)(vend - p - 4), whi) : 4;
-            after = p + len < vend ? col[p + len] : INT32_MAX;
+        // the run of N(v) in index windows [k, k + nidx): nx[i] caches the window of the
+        // neighbour's next run, so a neighbour with none here costs one 4-byte load; two
+        // consecutive entries bound a run, and a third joins the next window's run
+        if (lane < per && i < nl && nx[i] < k + nidx) {
+          const int64_t jc = cur[i];
+          const uint64_t a = runs[jc];
+          uint64_t b = runs[jc + 1];
+          int64_t nxt = jc + 1;
+          if (nidx == 2 && (int32_t)(a >> kRunPosBits) == k && (int32_t)(b >> kRunPosBits) == k + 1) {
+            b = runs[jc + 2];  // entry jc + 1 is not the row's last, so jc + 2 exists
+            nxt = jc + 2;
           }
-          cur[2 * i] = p + len;
-          nx[i] = after;
+          p = (int64_t)(a & kRunPosMask);
+          len = (int32_t)((int64_t)(b & kRunPosMask) - p);
+          // the pass that reaches u: the run stops at u itself (u is in N(v), the first
+          // element >= u)
+          if (whi == u) len = lower_bound(col + p, len, u);
+          if (advance) {
+            cur[i] = nxt;
+            nx[i] = (int32_t)(b >> kRunPosBits);
+          }
         }
         const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)len);
         const int32_t start = incl - len;
@@ -1031,14 +1115,9 @@ __global__ void __launch_bounds__(kC4Block)
           for (int r = 0; r < kC4Steps; ++r) x[r] = idx[r] >= 0 ? col[idx[r]] - wlo : -1;
 #pragma unroll
           for (int r = 0; r < kC4Steps; ++r) {
-            if (x[r] < 0) continue;
-            bool first;
-            if (packed) {
-              const uint32_t sh = (x[r] & 1) * 16;
-              first = ((atomicAdd(&cnt[x[r] >> 1], 1u << sh) >> sh) & 0xffffu) == 0u;
-            } else {
-              first = atomicAdd(&cnt[x[r]], 1u) == 0u;
-            }
+            if (x[r] < 0 || x[r] >= whi - wlo) continue;  // outside [wlo, whi): not this pass
+            const uint32_t sh = (uint32_t)(x[r] & ((1 << lgc) - 1)) * (uint32_t)bits;
+            const bool first = ((atomicAdd(&cnt[x[r] >> lgc], 1u << sh) >> sh) & cmask) == 0u;
             if (first) {
               const int32_t t = atomicAdd(&s_ntouch, 1);
               if (t < kC4Touch) touch[t] = x[r];
@@ -1051,28 +1130,31 @@ __global__ void __launch_bounds__(kC4Block)
       if (nt <= kC4Touch) {  // read and clear the listed counters only
         for (int32_t t = threadIdx.x; t < nt; t += kC4Block) {
           const int32_t x = touch[t];
-          uint64_t c;
-          if (packed) {  // the word's other half may be cleared by another thread meanwhile
-            const uint32_t sh = (x & 1) * 16;
-            c = (atomicAnd(&cnt[x >> 1], ~(0xffffu << sh)) >> sh) & 0xffffu;
-          } else {
-            c = cnt[x];
-            cnt[x] = 0u;
-          }
+          // the word's other counters may be cleared by other threads meanwhile
+          const uint32_t sh = (uint32_t)(x & ((1 << lgc) - 1)) * (uint32_t)bits;
+          const uint64_t c = (atomicAnd(&cnt[x >> lgc], ~(cmask << sh)) >> sh) & cmask;
           acc += c * (c - 1) / 2;
         }
       } else {
-        const int32_t words = packed ? (whi - wlo + 1) / 2 : whi - wlo;
+        const int32_t words = ((whi - wlo) + (1 << lgc) - 1) >> lgc;
         for (int32_t x = threadIdx.x; x < words; x += kC4Block) {
           const uint32_t v = cnt[x];
           cnt[x] = 0u;
-          const uint64_t c0 = packed ? (v & 0xffffu) : v, c1 = packed ? (v >> 16) : 0u;
-          acc += c0 * (c0 - (c0 > 0)) / 2 + c1 * (c1 - (c1 > 0)) / 2;
+          for (int q = 0; q < (1 << lgc); ++q) {
+            const uint64_t c = (v >> (q * bits)) & cmask;
+            acc += c * (c - (c > 0)) / 2;
+          }
         }
       }
       __syncthreads();
       if (threadIdx.x == 0) s_ntouch = 0;
       __syncthreads();
+      if (bits == 32 && !advance) {
+        half = 1;
+      } else {
+        half = 0;
+        k += nidx;
+      }
     }
   }
   const uint64_t sacc = wave_sum_u64(acc);
@@ -1196,16 +1278,35 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
     GARY_HIP(hipStreamSynchronize(st));
     if (hv[0] > 0) {
       const int64_t stride = std::max<int32_t>(hv[1], 1);
-      void *cws = nullptr;  // per workgroup: cursor and end (int64) and next endpoint (int32)
-      GARY_TRY(workspace_get(dev, (void *)8, 3 * sizeof(int64_t) * (size_t)stride * grid_win, &cws));
       const int32_t windows = (m + kC4Win - 1) / kC4Win;
       const int32_t jmax = (windows + kC4Group - 1) / kC4Group;
       GARY_CHECK_ARG((int64_t)hv[0] * jmax < INT32_MAX, "gary_hom4_count: too many window items");
-      c4_window_kernel<<<grid_win, kC4Block, 0, st>>>(*g, lo_len, heavy, bad + 4, jmax,
-                                                      (int64_t *)cws,
-                                                      (int32_t *)((int64_t *)cws + 2 * stride * grid_win),
-                                                      stride, bad + 6,
-                                                      sums + 4);
+      // run index: at most one entry per element plus one per row
+      GARY_CHECK_ARG((uint64_t)g->csr.nnz + (uint64_t)m + 1 < (1ull << 32) &&
+                         (uint64_t)g->csr.nnz < (1ull << kRunPosBits) && windows < kRunEndWin,
+                     "gary_hom4_count: 4-cycles need nnz + m below 2^32");
+      const uint32_t nptr = (uint32_t)m + 1;
+      const size_t b_ptr = al(sizeof(uint32_t) * nptr);
+      const size_t b_part = al(sizeof(uint32_t) * exclusive_scan_parts(nptr));
+      const size_t b_runs = al(sizeof(uint64_t) * ((size_t)g->csr.nnz + (size_t)m));
+      const size_t b_cur = al(sizeof(int64_t) * (size_t)stride * grid_win);
+      const size_t b_nx = al(sizeof(int32_t) * (size_t)stride * grid_win);
+      void *cws = nullptr;
+      GARY_TRY(workspace_get(dev, (void *)8, b_ptr + b_part + b_runs + b_cur + b_nx, &cws));
+      char *q = (char *)cws;
+      uint32_t *run_ptr = (uint32_t *)q; q += b_ptr;
+      uint32_t *part = (uint32_t *)q; q += b_part;
+      uint64_t *runs = (uint64_t *)q; q += b_runs;
+      int64_t *cur = (int64_t *)q; q += b_cur;
+      int32_t *nxw = (int32_t *)q;
+      const int rgrid = (int)std::min<int64_t>(((int64_t)m + 4) / 4, 8192);  // 4 waves per block
+      run_count_kernel<<<rgrid, 256, 0, st>>>(*g, 16, run_ptr);
+      GARY_HIP(hipGetLastError());
+      GARY_TRY(exclusive_scan_u32(run_ptr, nptr, part, st));
+      run_write_kernel<<<rgrid, 256, 0, st>>>(*g, 16, run_ptr, runs);
+      GARY_HIP(hipGetLastError());
+      c4_window_kernel<<<grid_win, kC4Block, 0, st>>>(*g, lo_len, heavy, bad + 4, jmax, run_ptr,
+                                                      runs, cur, nxw, stride, bad + 6, sums + 4);
       GARY_HIP(hipGetLastError());
     }
   }

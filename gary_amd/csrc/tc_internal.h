@@ -68,6 +68,11 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                       const uint8_t *route = nullptr, const float *pcost = nullptr,
                       float cost_big = 0.f, const RouteTable *table = nullptr);
 
+// Exclusive scan of a[0..len) in place on `s` (values and total below 2^32); `part` holds
+// exclusive_scan_parts(len) values (graph_build.hip).
+uint32_t exclusive_scan_parts(uint32_t len);
+int exclusive_scan_u32(uint32_t *a, uint32_t len, uint32_t *part, hipStream_t s);
+
 // Triangle listing on the pivot path: every triangle (u, v, w) of a degree-oriented CSR
 // written to out[3 * cap], claims counted in *total (zeroed by the caller).
 int launch_tri_list(const gary_device_graph &g, unsigned long long *total, int32_t *out,

@@ -129,3 +129,18 @@ def test_hom4_light_vertex_paths():
     assert want["c4"] > 0
     assert gq.hom4Count(ctx) == want
     assert gq.hom4Count(ctx, gq.HOM_C4)["c4"] == want["c4"]
+
+
+def test_hom4_window_halves_complete_bipartite():
+    """K_{3,70000}: each hub has 70,000 lower neighbours, more than 16-bit counters hold, so its
+    windows are counted in two halves with 32-bit counters. #C4 = C(3,2) * C(70000,2) (closed
+    form), no triangles, and the oracle's wedge counts agree."""
+    a, b = 3, 70_000
+    hubs = np.arange(a)
+    leaves = a + np.arange(b)
+    hh, ll = np.meshgrid(hubs, leaves, indexing="ij")
+    g, ctx = _ranked_ctx(a + b, hh.ravel().astype(np.int32), ll.ravel().astype(np.int32))
+    got = gq.hom4Count(ctx)
+    assert got["c4"] == (a * (a - 1) // 2) * (b * (b - 1) // 2)
+    assert got["tri"] == 0 and got["k4"] == 0 and got["hom_diamond"] == 0
+    assert got == oracle.hom4_ranked(g.indPtr, g.colIdx)
