@@ -398,14 +398,7 @@ __device__ __forceinline__ float fixed_cost(const FixedCosts &fc, float lx, int 
   return fc.f[fc.n - 1][o];
 }
 
-// Fused inference, kSplit threads per bin (kRouteBins * kSplit threads, one workgroup): every
-// thread of a bin computes the first layer, thread q the hidden units [q*kHid/kSplit,
-// (q+1)*kHid/kSplit) of the second and its share of the output sums, which the bin's threads add
-// with shuffles. (One thread per bin, each doing all 1,312 multiply-adds in turn: 17 us.)
-constexpr int kSplit = 4;
-static_assert(kHid % kSplit == 0 && kRouteBins * kSplit <= 1024, "inference split");
-
-__global__ void __launch_bounds__(kRouteBins * kSplit)
+__global__ void __launch_bounds__(kRouteBins)
     mlp_table_kernel(const float *__restrict__ P, const BinSums *__restrict__ sums, float log_nnz,
                      float nnz, uint32_t allowed, FixedCosts fixed, float margin,
                      RouteTable *__restrict__ table) {
@@ -418,35 +411,18 @@ __global__ void __launch_bounds__(kRouteBins * kSplit)
     table->mixed = 0;
   }
   __syncthreads();
-  const int b = threadIdx.x / kSplit, q = threadIdx.x % kSplit;
-  float x[kIn], h1[kHid], y[kOut], c[kOut];
+  const int b = threadIdx.x;
+  float x[kIn], h1[kHid], h2[kHid], y[kOut], c[kOut];
   const BinSums s = read_sums(sums, b);
   bin_feature_vector(b, s, log_nnz, x);
-  for (int k = 0; k < kHid; ++k) {
-    float a = sP[oB1 + k];
-    for (int i = 0; i < kIn; ++i) a += sP[oW1 + k * kIn + i] * x[i];
-    h1[k] = a > 0.f ? a : 0.f;
-  }
-  for (int o = 0; o < kOut; ++o) y[o] = 0.f;
-  for (int k = q * (kHid / kSplit); k < (q + 1) * (kHid / kSplit); ++k) {
-    float a = sP[oB2 + k];
-    for (int i = 0; i < kHid; ++i) a += sP[oW2 + k * kHid + i] * h1[i];
-    const float h2 = a > 0.f ? a : 0.f;
-    for (int o = 0; o < kOut; ++o) y[o] += sP[oW3 + o * kHid + k] * h2;
-  }
-  for (int o = 0; o < kOut; ++o) {
-#pragma unroll
-    for (int d = 1; d < kSplit; d <<= 1) y[o] += __shfl_xor(y[o], d, kWave);
-    y[o] += sP[oB3 + o];
-    c[o] = exp2f(fminf(y[o], 30.f));
-  }
-  // route subsets: the bin's first thread contributes its arcs at the subset's best cost
+  mlp_forward(sP, x, h1, h2, y);
+  for (int o = 0; o < kOut; ++o) c[o] = exp2f(fminf(y[o], 30.f));
   for (uint32_t R = 1; R < 8; ++R) {
     if ((R & allowed) != R) continue;
     float best = INFINITY;
     for (int o = 0; o < kOut; ++o)
       if ((R >> o) & 1u) best = fminf(best, c[o]);
-    float contrib = q == 0 ? s.arcs * best : 0.f;
+    float contrib = s.arcs * best;
     contrib = wave_sum_f32(contrib);
     if (lane_id() == 0) atomicAdd(&s_tot[R], contrib);
   }
@@ -474,19 +450,15 @@ __global__ void __launch_bounds__(kRouteBins * kSplit)
       by = c[o];
       best = o;
     }
-  if (q == 0) {
-    table->route[b] = (uint8_t)best;  // output o == route code o (bitmap, cached, merge)
-    table->cost[b] = by;
-  }
-  // bins 16w .. 16w+15 of this wave (kSplit = 4 lanes each): rows with a bitmap bin
-  const uint64_t bm = __ballot(q == 0 && best == 0);
-  const uint64_t other = __ballot(q == 0 && best != 0);
+  table->route[b] = (uint8_t)best;  // output o == route code o (bitmap, cached, merge)
+  table->cost[b] = by;
+  const uint64_t bm = __ballot(best == 0);  // bins 64w .. 64w+63 of this wave
   if (lane_id() == 0) {
     uint32_t rows = 0;
-    for (int k = 0; k < kWave; k += kSplit)
-      if ((bm >> k) & 1ull) rows |= 1u << ((b + k / kSplit) >> 4);
-    if (rows) atomicOr(&table->rowmask, rows);
-    if (other) atomicOr(&table->mixed, 1u);
+    for (int k = 0; k < 64; ++k)
+      if ((bm >> k) & 1ull) rows |= 1u << ((b + k) >> 4);
+    atomicOr(&table->rowmask, rows);
+    if (bm != ~0ull) atomicOr(&table->mixed, 1u);
   }
 }
 
@@ -777,7 +749,7 @@ int launch_selected(const gary_device_graph &g, bool learned, const PivotCfg &pc
     return launch_routed_ws(g, w, s->heuristic, pc, 0.f, cnt, st, shape_of(h));
   }
   GARY_TRY(compute_bin_sums(g, w.sums, st));
-  mlp_table_kernel<<<1, kRouteBins * kSplit, 0, st>>>(s->model.P, w.sums,
+  mlp_table_kernel<<<1, kRouteBins, 0, st>>>(s->model.P, w.sums,
                                              log2f(1.f + (float)g.csr.nnz), (float)g.csr.nnz,
                                              s->model.allowed, fixed_of(s->model),
                                              s->model.margin, w.table);
@@ -1113,7 +1085,7 @@ int gary_selector_learned_table(const gary_device_graph *g, uint8_t *h_routes, f
   GARY_TRY(state_for(dev, &s));
   if (!s->model.trained) return set_error(GARY_ERR_STATE, "the selector has no trained model");
   GARY_TRY(compute_bin_sums(*g, s->sums, nullptr));
-  mlp_table_kernel<<<1, kRouteBins * kSplit>>>(s->model.P, s->sums, log2f(1.f + (float)g->csr.nnz),
+  mlp_table_kernel<<<1, kRouteBins>>>(s->model.P, s->sums, log2f(1.f + (float)g->csr.nnz),
                                       (float)g->csr.nnz, s->model.allowed, fixed_of(s->model),
                                       s->model.margin, s->sync_table);
   GARY_HIP(hipGetLastError());
