@@ -328,7 +328,7 @@ class Bench:
         return leg
 
     def hom4(self) -> dict:
-        gq, graph = self.gq, self.graph
+        gq, graph, np = self.gq, self.graph, self.np
         t0 = time.time()
         n, s, d = graph.rmat(22, 16, seed=graph.SEEDS["rmat22"])
         g = graph.csr_from_edges(n, s, d, graph.SYMMETRIC_RANKED)
@@ -338,23 +338,32 @@ class Bench:
         leg = {"workload": "rmat22-ef16 (a=.57,b=.19,c=.19) seed 0x6A7E0005, symmetrized, dedup, "
                            "no loops, ids ranked by (degree, id), int32 CSR",
                "n": n, "undirected_edges": g.nnz // 2}
+        from gary_amd import c4_selector
+        model = c4_selector.load(required=False)  # AUTO routes the 4-cycle vertices by it
+        leg["c4_routes"] = "learned (gary_amd/c4_model.json)" if model else "fixed degree split"
         vals = None
         for nm, pat in (("k4_diamond", gq.HOM_K4 | gq.HOM_DIAMOND), ("c4", gq.HOM_C4),
                         ("all", gq.HOM_ALL)):
             gq.hom4Count(ctx, pat)  # warm-up
-            k = 3
-            t1 = time.perf_counter()
-            for _ in range(k):
+            ts = []
+            for _ in range(5):
+                t1 = time.perf_counter()
                 vals = gq.hom4Count(ctx, pat)
-            leg[f"{nm}_ms"] = (time.perf_counter() - t1) * 1e3 / k
-            log(f"rmat22 hom4 {nm}: {leg[f'{nm}_ms']:.1f} ms")
+                ts.append((time.perf_counter() - t1) * 1e3)
+            leg[f"{nm}_ms"] = float(np.median(ts))
+            log(f"rmat22 hom4 {nm}: {leg[f'{nm}_ms']:.1f} ms (median of 5)")
         leg["counts"] = vals
+        if model:
+            leg["c4_learned_vs_fixed"] = {"rmat22": self.c4_routing(ctx)}
+            del ctx
+            n20, s20, d20 = graph.rmat(20, 16, seed=graph.SEEDS["rmat22"])
+            g20 = graph.csr_from_edges(n20, s20, d20, graph.SYMMETRIC_RANKED)
+            leg["c4_learned_vs_fixed"]["rmat20"] = self.c4_routing(self.DeviceGraphCtx(g20))
         leg["edges_per_s"] = (g.nnz // 2) / (leg["all_ms"] * 1e-3)
         # per-pattern algorithmic bytes (m undirected edges, d full degree, d+ upper degree):
         #   diamond 4*sum d^2 + 32m; 4-cycles 4*sum d^2 + 16n; 4-cliques at least 4*W + 32m
         #   (W = sum over arcs (u,v) of d+(u) + d+(v); the full K4 formula adds the common
         #   out-neighbour terms, which need per-arc triangle counts)
-        np = self.np
         d = np.diff(g.indPtr).astype(np.int64)
         row = np.repeat(np.arange(n, dtype=np.int64), d)
         upper = g.colIdx > row
@@ -372,6 +381,25 @@ class Bench:
         if not self.a.no_cpu:
             leg["cpu_sample"] = self.hom4_cpu_sample()
         return leg
+
+    def c4_routing(self, ctx) -> dict:
+        """Learned vs fixed 4-cycle vertex routing on one graph: both tables' counts (all six
+        values must agree) and the medians of 7 alternating C4 calls of each."""
+        from gary_amd import c4_selector
+        gq = self.gq
+        a = gq.hom4Count(ctx, gq.HOM_ALL, c4=gq.C4_FIXED)
+        b = gq.hom4Count(ctx, gq.HOM_ALL, c4=gq.C4_LEARNED)
+        if a != b:
+            raise SystemExit(f"learned 4-cycle routing changed the counts: {a} != {b}")
+        tf, tl = c4_selector.compare(ctx, gq.HOM_C4, 7)
+        _, nv = c4_selector.features(ctx)
+        lt = c4_selector.table(ctx, gq.C4_LEARNED)
+        same = bool(self.np.all((lt == c4_selector.fixed_table()) | (nv == 0)))
+        moved = int(nv[(lt != c4_selector.fixed_table())].sum())
+        log(f"c4 routing: fixed {tf:.1f} ms, learned {tl:.1f} ms (x{tf / tl:.2f}), "
+            f"{moved} vertices off the fixed split")
+        return {"c4_ms_fixed": tf, "c4_ms_learned": tl, "learned_vs_fixed": tf / tl,
+                "counts_equal": True, "same_routes": same, "vertices_moved": moved}
 
     def hom4_cpu_sample(self) -> dict:
         """The CPU hom4 path (the oracle's oriented-wedge counts, OpenMP) on R-MAT 20 beside the

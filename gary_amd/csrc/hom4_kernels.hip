@@ -22,9 +22,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
 
 #include "common.h"
 #include "device_util.h"
+#include "mlp.h"
 #include "tc_internal.h"
 #include "workspace.h"
 
@@ -1011,7 +1019,7 @@ __global__ void __launch_bounds__(kC4Block)
     const int64_t it = s_item;
     __syncthreads();  // everyone holds the item before s_item is written again
     if (it >= nitems) break;
-    const int32_t u = heavy[nheavy - 1 - it % nheavy];  // the list roughly ascends by rank
+    const int32_t u = heavy[nheavy - 1 - it % nheavy];  // the list ascends by id (rank)
     const int32_t j = (int32_t)(it / nheavy);
     const int32_t K = (u + kC4Win - 1) / kC4Win;
     const int32_t k0 = j * kC4Group, k1 = min(K, k0 + kC4Group);
@@ -1161,16 +1169,351 @@ __global__ void __launch_bounds__(kC4Block)
   if (lane == 0) add_u128(c4_acc, sacc);
 }
 
+
+// ---- 4-cycle vertex routing ----------------------------------------------------------------------
+//
+// Each vertex u with two or more lower neighbours is binned by (min(15, log2 nl), min(15,
+// log2 S / 2)), nl = |N(u) ∩ [0, u)| and S = Σ_{v ∈ N(u), v < u} d(v): S bounds the wedges
+// u-v-w (v, w < u) from above (estimated from 64 lower neighbours when nl > 64). A table of kRouteBins
+// route bytes sends each bin to the light kernels (LDS hash tables of one wave, then of one
+// workgroup, then the dense counters, as a vertex's wedges overflow each), to the LDS endpoint
+// windows, or to the dense counters directly. The fixed split is the degree threshold (nl >=
+// kC4HeavyLo to the windows); a learned table takes, per bin, the route with the least
+// predicted ns per vertex from the MLP, and leaves the fixed route only for a predicted gain
+// above a margin. The light kernels hold at most kC4HeavyLo - 1 lower neighbours, so a vertex
+// with more that a table sends there goes to the windows.
+constexpr uint8_t kC4Light = GARY_C4_LIGHT, kC4Window = GARY_C4_WINDOW, kC4Dense = GARY_C4_DENSE,
+                  kC4Skip = GARY_C4_SKIP;
+static_assert(GARY_C4_BINS == kRouteBins, "one route byte per bin");
+
+// A learned table sends a bin to the dense counters only below this S bin (S < 2^24 per vertex:
+// one device-memory atomic per wedge), the range its timings cover.
+constexpr int kC4DenseMaxSBin = 11;
+
+struct C4BinSums {  // per bin: vertices, and Σ of round(256 * log2 x) for x = nl, S, K
+  unsigned long long n, lnl, ls, lk;
+};
+
+__device__ __forceinline__ int c4_bin(int32_t nl, unsigned long long S) {
+  const int a = min(15, 31 - __clz(max(nl, 1)));
+  const int b = min(15, (63 - __clzll((long long)max(S, 1ull))) >> 1);
+  return a * 16 + b;
+}
+
+__host__ __device__ __forceinline__ uint8_t c4_fixed_route(int bin) {
+  return (1 << (bin >> 4)) >= kC4HeavyLo ? kC4Window : kC4Light;  // nl >= 2^(bin >> 4)
+}
+
+__device__ __forceinline__ unsigned long long log2_fix8(unsigned long long x) {
+  return (unsigned long long)(log2f((float)x) * 256.f + 0.5f);
+}
+
+// MLP inputs of one bin: means of log2 nl, log2 S and log2 K (K = ceil(u / kC4Win), the
+// endpoint windows of u), log2 m, log2(1 + vertices in the bin), mean log2(S / nl).
+__device__ __forceinline__ void c4_feature_vector(const C4BinSums &s, float log_m, float *x) {
+  const float n = (float)s.n;
+  const float lnl = (float)s.lnl / (256.f * n), ls = (float)s.ls / (256.f * n);
+  const float lk = (float)s.lk / (256.f * n);
+  x[0] = lnl / 8.f;
+  x[1] = ls / 16.f;
+  x[2] = lk / 8.f;
+  x[3] = log_m / 16.f;
+  x[4] = log2f(1.f + n) / 16.f;
+  x[5] = (ls - lnl) / 16.f;
+}
+
+// cls[u] = bin of u (0 for nl < 2, which has no 4-cycle at its top) and the per-bin sums. S is
+// exact for nl <= kWave, summed by the vertex's own lane; above, it is estimated as
+// floor(nl * Σ_{i < 64} d(N[floor(i * nl / 64)]) / 64) from 64 evenly spaced lower neighbours,
+// one wave step per vertex (a bin spans a factor of 4 in S; summing every lower arc of the hubs
+// cost 3 ms on R-MAT 22).
+__global__ void __launch_bounds__(256)
+    c4_class_kernel(gary_device_graph g, const int32_t *__restrict__ lo_len,
+                    uint8_t *__restrict__ cls, C4BinSums *__restrict__ sums) {
+  __shared__ unsigned long long s_sum[kRouteBins * 4];
+  for (int i = threadIdx.x; i < kRouteBins * 4; i += blockDim.x) s_sum[i] = 0ull;
+  __syncthreads();
+  const int lane = lane_id();
+  const int32_t m = g.csr.m;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  const int32_t *__restrict__ deg = g.csr.deg;
+  for (int64_t base = global_wave_id() * kWave; base < m; base += num_global_waves() * kWave) {
+    const int64_t u = base + lane;
+    const int32_t nl = u < m ? lo_len[u] : 0;
+    const int64_t p = u < m ? g.csr.indPtr[u] : 0;
+    unsigned long long S = 0;
+    if (nl <= kWave)
+      for (int32_t k = 0; k < nl; ++k) S += (unsigned)deg[col[p + k]];
+    uint64_t big = __ballot(nl > kWave);
+    while (big) {  // wave-uniform: one sampled step per vertex
+      const int j = __builtin_ctzll(big);
+      big &= big - 1;
+      const int64_t pj = __shfl(p, j, kWave);
+      const int32_t nj = __shfl(nl, j, kWave);
+      const int32_t k = (int32_t)(((int64_t)lane * nj) / kWave);
+      const uint64_t part = wave_sum_u64((unsigned)deg[col[pj + k]]);
+      if (lane == j) S = (unsigned long long)(((unsigned __int128)part * (unsigned)nj) / kWave);
+    }
+    if (u < m) {
+      if (nl >= 2) {
+        const int b = c4_bin(nl, S);
+        cls[u] = (uint8_t)b;
+        const unsigned long long K = ((unsigned long long)u + kC4Win - 1) / kC4Win;
+        atomicAdd(&s_sum[b * 4 + 0], 1ull);
+        atomicAdd(&s_sum[b * 4 + 1], log2_fix8((unsigned long long)nl));
+        atomicAdd(&s_sum[b * 4 + 2], log2_fix8(S));
+        atomicAdd(&s_sum[b * 4 + 3], log2_fix8(max(K, 1ull)));
+      } else {
+        cls[u] = 0;
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long *out = &sums[0].n;
+  for (int i = threadIdx.x; i < kRouteBins * 4; i += blockDim.x)
+    if (s_sum[i]) atomicAdd(&out[i], s_sum[i]);
+}
+
+// Features of every bin for the host (training rows): X[kRouteBins][kIn], vertices per bin.
+__global__ void __launch_bounds__(kRouteBins)
+    c4_features_kernel(const C4BinSums *__restrict__ sums, float log_m, float *__restrict__ X,
+                       float *__restrict__ nv) {
+  const int b = threadIdx.x;
+  const C4BinSums s = sums[b];
+  nv[b] = (float)s.n;
+  float x[kIn] = {};
+  if (s.n) c4_feature_vector(s, log_m, x);
+  for (int i = 0; i < kIn; ++i) X[b * kIn + i] = x[i];
+}
+
+// Learned table: output o of the MLP is log2(predicted ns per vertex) of route o (light,
+// window, dense). Empty bins keep the fixed route.
+__global__ void __launch_bounds__(kRouteBins)
+    c4_mlp_table_kernel(const float *__restrict__ P, const C4BinSums *__restrict__ sums,
+                        float log_m, float margin, uint8_t *__restrict__ table) {
+  __shared__ float sP[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) sP[i] = P[i];
+  __syncthreads();
+  const int b = threadIdx.x;
+  const C4BinSums s = sums[b];
+  const uint8_t fixed = c4_fixed_route(b);
+  if (s.n == 0) {
+    table[b] = fixed;
+    return;
+  }
+  float x[kIn], h1[kHid], h2[kHid], y[kOut];
+  c4_feature_vector(s, log_m, x);
+  mlp_forward(sP, x, h1, h2, y);
+  const bool light_ok = (1 << (b >> 4)) < kC4HeavyLo;
+  const bool dense_ok = (b & 15) <= kC4DenseMaxSBin;
+  int best = fixed;
+  float by = y[fixed] + log2f(fmaxf(1.f - margin, 1e-6f));  // an alternative must beat this
+  for (int o = 0; o < kOut; ++o) {
+    if (o == fixed || (o == kC4Light && !light_ok) || (o == kC4Dense && !dense_ok)) continue;
+    if (y[o] < by) {
+      by = y[o];
+      best = o;
+    }
+  }
+  table[b] = (uint8_t)best;
+}
+
+// The light and dense vertex lists from the table (per-wave runs ascend, as in
+// range_list_kernel); the window list is built by window_flag_kernel / window_list_kernel.
+__global__ void __launch_bounds__(256)
+    c4_route_kernel(const int32_t *__restrict__ lo_len, const uint8_t *__restrict__ cls,
+                    int32_t m, const uint8_t *__restrict__ table, int32_t *__restrict__ light,
+                    int32_t *__restrict__ nlight, int32_t *__restrict__ dense,
+                    int32_t *__restrict__ ndense) {
+  __shared__ uint8_t s_table[kRouteBins];
+  for (int i = threadIdx.x; i < kRouteBins; i += blockDim.x) s_table[i] = table[i];
+  __syncthreads();
+  const int lane = lane_id();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < m;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = base + lane;
+    const int32_t nl = u < m ? lo_len[u] : 0;
+    const int r = nl >= 2 ? s_table[cls[u]] : kC4Skip;
+    const bool to_light = r == kC4Light && nl < kC4HeavyLo, to_dense = r == kC4Dense;
+    const uint64_t ml = __ballot(to_light), md = __ballot(to_dense);
+    int32_t pl = 0, pd = 0;
+    if (lane == 0) {
+      if (ml) pl = atomicAdd(nlight, __popcll(ml));
+      if (md) pd = atomicAdd(ndense, __popcll(md));
+    }
+    pl = __shfl(pl, 0, kWave);
+    pd = __shfl(pd, 0, kWave);
+    if (to_light) light[pl + lanes_below(ml)] = (int32_t)u;
+    if (to_dense) dense[pd + lanes_below(md)] = (int32_t)u;
+  }
+}
+
+// The window list in id order. f[u] = 1 when u takes the windows (table == nullptr: nl >=
+// kC4HeavyLo; else its bin's route, light with nl >= kC4HeavyLo counting as windows), f[m] = 0,
+// and *max_nl = the largest nl among them; an exclusive scan of f then gives each its slot
+// (window_list_kernel). The list order sets which work items run together: left to the order
+// of atomics, the R-MAT 22 count took 175 to 236 ms from call to call, in id order 173 ms.
+__global__ void __launch_bounds__(256)
+    window_flag_kernel(const int32_t *__restrict__ lo_len, const uint8_t *__restrict__ cls,
+                       const uint8_t *__restrict__ table, int32_t m, uint32_t *__restrict__ f,
+                       int32_t *__restrict__ max_nl) {
+  int32_t mx = 0;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u <= m;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    bool w = false;
+    if (u < m) {
+      const int32_t nl = lo_len[u];
+      if (!table) {
+        w = nl >= kC4HeavyLo;
+      } else if (nl >= 2) {
+        const int r = table[cls[u]];
+        w = r == kC4Window || (r == kC4Light && nl >= kC4HeavyLo);
+      }
+      if (w) mx = max(mx, nl);
+    }
+    f[u] = w ? 1u : 0u;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, kWave));
+  if (lane_id() == 0 && mx > 0) atomicMax(max_nl, mx);
+}
+
+__global__ void __launch_bounds__(256)
+    window_list_kernel(const uint32_t *__restrict__ off, int32_t m, int32_t *__restrict__ list,
+                       int32_t *__restrict__ count) {
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < m;
+       u += (int64_t)gridDim.x * blockDim.x)
+    if (off[u + 1] != off[u]) list[off[u]] = (int32_t)u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = (int32_t)off[m];
+}
+
+// ---- 4-cycle routing state (host) ------------------------------------------------------------------
+
+struct C4Model {
+  float *P = nullptr;  // device, kParams floats
+  bool trained = false;
+  float margin = 0.f;
+};
+
+std::mutex g_c4_mu;
+std::map<int, C4Model> *g_c4_models = new std::map<int, C4Model>();
+
+int c4_model_of(int dev, C4Model *out) {
+  std::lock_guard<std::mutex> lock(g_c4_mu);
+  *out = (*g_c4_models)[dev];
+  return GARY_OK;
+}
+
+int c4_set_model(int dev, const float *h_params, float margin) {
+  std::lock_guard<std::mutex> lock(g_c4_mu);
+  C4Model &md = (*g_c4_models)[dev];
+  if (!h_params) {
+    md.trained = false;
+    return GARY_OK;
+  }
+  if (!md.P) GARY_HIP(hipMalloc(&md.P, sizeof(float) * kParams));
+  GARY_HIP(hipMemcpy(md.P, h_params, sizeof(float) * kParams, hipMemcpyHostToDevice));
+  md.trained = true;
+  md.margin = margin;
+  return GARY_OK;
+}
+
+// Zeroes the bin sums, then bins every vertex (cls) and sums the bins' features.
+int c4_classify(const gary_device_graph &g, const int32_t *lo_len, uint8_t *cls, C4BinSums *csum,
+                hipStream_t st) {
+  GARY_HIP(hipMemsetAsync(csum, 0, sizeof(C4BinSums) * kRouteBins, st));
+  const int grid = (int)std::min<int64_t>(((int64_t)g.csr.m + 255) / 256, 1024);
+  c4_class_kernel<<<std::max(grid, 1), 256, 0, st>>>(g, lo_len, cls, csum);
+  GARY_HIP(hipGetLastError());
+  return GARY_OK;
+}
+
+// Features (X, nv) or the table of a routing mode (table) for gary_hom4_c4_features /
+// gary_hom4_c4_table: checks the input as gary_hom4_count does, bins the vertices, and copies
+// the result back (synchronous on the null stream).
+int c4_inspect(const gary_device_graph &g, int c4_routes, float *h_x, float *h_nv,
+               uint8_t *h_table) {
+  const int32_t m = g.csr.m;
+  if (h_table)
+    for (int b = 0; b < kRouteBins; ++b) h_table[b] = c4_fixed_route(b);
+  if (h_x) std::fill(h_x, h_x + (size_t)kRouteBins * kIn, 0.f);
+  if (h_nv) std::fill(h_nv, h_nv + kRouteBins, 0.f);
+  if (m == 0 || g.csr.nnz == 0) return GARY_OK;
+  GARY_CHECK_ARG(g.csr.indPtr && g.csr.colIdx && g.csr.deg, "gary_hom4_c4: null arrays");
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  C4Model model;
+  GARY_TRY(c4_model_of(dev, &model));
+  if (h_table && !h_x) {
+    if (c4_routes == GARY_C4_ROUTES_FIXED ||
+        (c4_routes == GARY_C4_ROUTES_AUTO && !model.trained))
+      return GARY_OK;
+    if (!model.trained)
+      return set_error(GARY_ERR_STATE, "gary_hom4_c4_table: no 4-cycle routing model on this device");
+  }
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_us = al(sizeof(int64_t) * (size_t)m), b_len = al(sizeof(int32_t) * (size_t)m);
+  const size_t b_misc = 256, b_cls = al((size_t)m);
+  const size_t b_csum = al(sizeof(C4BinSums) * kRouteBins);
+  const size_t b_x = al(sizeof(float) * kRouteBins * (kIn + 1)), b_tab = al(kRouteBins);
+  void *ws = nullptr;
+  GARY_TRY(workspace_get(dev, (void *)12,
+                         b_us + 2 * b_len + b_misc + b_cls + b_csum + b_x + b_tab, &ws));
+  char *p = (char *)ws;
+  int64_t *up_start = (int64_t *)p; p += b_us;
+  int32_t *up_len = (int32_t *)p; p += b_len;
+  int32_t *lo_len = (int32_t *)p; p += b_len;
+  int32_t *bad = (int32_t *)p; p += b_misc;
+  uint8_t *cls = (uint8_t *)p; p += b_cls;
+  C4BinSums *csum = (C4BinSums *)p; p += b_csum;
+  float *X = (float *)p; p += b_x;
+  uint8_t *table = (uint8_t *)p;
+  GARY_HIP(hipMemsetAsync(bad, 0, b_misc, nullptr));
+  hom_split_kernel<<<std::min<int64_t>(((int64_t)m + 255) / 256, 4096), 256>>>(
+      g, up_start, up_len, lo_len, bad, (unsigned long long *)((char *)bad + 192));
+  GARY_HIP(hipGetLastError());
+  GARY_TRY(c4_classify(g, lo_len, cls, csum, nullptr));
+  if (h_x) {
+    c4_features_kernel<<<1, kRouteBins>>>(csum, log2f((float)m), X, X + kRouteBins * kIn);
+  } else {
+    c4_mlp_table_kernel<<<1, kRouteBins>>>(model.P, csum, log2f((float)m), model.margin, table);
+  }
+  GARY_HIP(hipGetLastError());
+  int32_t flag = 0;
+  unsigned long long asym = 0;
+  GARY_HIP(hipMemcpy(&flag, bad, sizeof flag, hipMemcpyDeviceToHost));
+  GARY_HIP(hipMemcpy(&asym, (char *)bad + 192, sizeof asym, hipMemcpyDeviceToHost));
+  if (flag || asym)
+    return set_error(GARY_ERR_INVALID,
+                     "gary_hom4_c4: the graph must be symmetric, loop-free and ranked "
+                     "(ids in non-decreasing degree order: GARY_CSR_SYMMETRIC_RANKED)");
+  if (h_x) {
+    GARY_HIP(hipMemcpy(h_x, X, sizeof(float) * kRouteBins * kIn, hipMemcpyDeviceToHost));
+    GARY_HIP(hipMemcpy(h_nv, X + kRouteBins * kIn, sizeof(float) * kRouteBins,
+                       hipMemcpyDeviceToHost));
+  } else {
+    GARY_HIP(hipMemcpy(h_table, table, kRouteBins, hipMemcpyDeviceToHost));
+  }
+  return GARY_OK;
+}
+
 }  // namespace
 }  // namespace gary
 
 extern "C" {
 
-int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *out,
-                    void *stream) {
+int gary_hom4_count_routed(const gary_device_graph *g, int patterns, int c4_routes,
+                           const uint8_t *h_table, gary_hom4_counts *out, void *stream) {
   GARY_API_BEGIN
   using namespace gary;
   GARY_CHECK_ARG(g && out, "gary_hom4_count: null pointer");
+  GARY_CHECK_ARG(c4_routes >= GARY_C4_ROUTES_AUTO && c4_routes <= GARY_C4_ROUTES_TABLE,
+                 "gary_hom4_count: bad 4-cycle routing mode %d", c4_routes);
+  if (c4_routes == GARY_C4_ROUTES_TABLE) {
+    GARY_CHECK_ARG(h_table, "gary_hom4_count: GARY_C4_ROUTES_TABLE without a table");
+    for (int b = 0; b < kRouteBins; ++b)
+      GARY_CHECK_ARG(h_table[b] <= kC4Skip, "gary_hom4_count: bad route %d in bin %d", h_table[b], b);
+  }
   GARY_CHECK_ARG((patterns & ~GARY_HOM_ALL) == 0, "gary_hom4_count: bad pattern mask %d", patterns);
   GARY_CHECK_ARG(g->csr.m >= 0 && g->csr.nnz >= 0, "gary_hom4_count: bad sizes");
   *out = gary_hom4_counts{};
@@ -1206,10 +1549,28 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   const size_t b_touched = need_c4 ? al(sizeof(int32_t) * (size_t)m * grid_c4) : 0;
   // heavy and light vertex lists, and the light vertices each light kernel passes on
   const size_t b_heavy = need_c4 ? al(sizeof(int32_t) * 4 * (size_t)m) : 0;
+  // 4-cycle routing by table: the model (LEARNED, or AUTO with a model), bins, sums, table
+  C4Model model;
+  if (need_c4 && (c4_routes == GARY_C4_ROUTES_AUTO || c4_routes == GARY_C4_ROUTES_LEARNED)) {
+    GARY_TRY(c4_model_of(dev, &model));
+    if (c4_routes == GARY_C4_ROUTES_LEARNED && !model.trained)
+      return set_error(GARY_ERR_STATE, "gary_hom4_count: no 4-cycle routing model on this device");
+  }
+  const bool by_table =
+      need_c4 && (c4_routes == GARY_C4_ROUTES_TABLE ||
+                  (c4_routes != GARY_C4_ROUTES_FIXED && model.trained));
+  const size_t b_cls = by_table ? al((size_t)m) : 0;
+  const size_t b_csum = by_table ? al(sizeof(C4BinSums) * kRouteBins) : 0;
+  const size_t b_tab = by_table ? al(kRouteBins) : 0;
+  // window-list flags / slots (m + 1) and their scan's partial sums
+  const uint32_t nflag = (uint32_t)m + 1;
+  const size_t b_flag = need_c4 ? al(sizeof(uint32_t) * nflag) : 0;
+  const size_t b_fpart = need_c4 ? al(sizeof(uint32_t) * exclusive_scan_parts(nflag)) : 0;
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)3,
                          b_up_start + b_up_len + b_lo_len + b_t + b_misc + b_rows + b_piv +
-                             b_cnt + b_touched + b_heavy, &ws));
+                             b_cnt + b_touched + b_heavy + b_cls + b_csum + b_tab + b_flag +
+                             b_fpart, &ws));
   char *p = (char *)ws;
   int64_t *up_start = (int64_t *)p; p += b_up_start;
   int32_t *up_len = (int32_t *)p; p += b_up_len;
@@ -1225,6 +1586,12 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   int32_t *light = heavy + m;
   int32_t *light_over = light + m;
   int32_t *light_over2 = light_over + m;
+  p += b_heavy;
+  uint8_t *cls = (uint8_t *)p; p += b_cls;
+  C4BinSums *csum = (C4BinSums *)p; p += b_csum;
+  uint8_t *d_table = (uint8_t *)p; p += b_tab;
+  uint32_t *wflag = (uint32_t *)p; p += b_flag;
+  uint32_t *wpart = (uint32_t *)p; p += b_fpart;
   // [0] bad, [1] unsupported, [2..3] cursors, [4] #heavy, [5] max lower degree of a heavy
   // vertex, [6] window-item cursor, [7] #light, [8] #small pivots, [9] #big pivots, [10] cursor,
   // [11] #light vertices with too many wedges for the wave path, [12] #those with too many for
@@ -1254,16 +1621,36 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
   hom_sums_kernel<<<1024, 256, 0, st>>>(*g, up_start, up_len, tcount, sums, sums + 2);
   GARY_HIP(hipGetLastError());
   if (need_c4) {
-    range_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, kC4HeavyLo, INT32_MAX, heavy, bad + 4,
-                                             bad + 5);
+    const bool use_block = m - 1 <= kBlockMaxId;  // packed keys hold every id
+    if (!by_table) {
+      range_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, 2, kC4HeavyLo, light, bad + 7, nullptr);
+      GARY_HIP(hipGetLastError());
+    } else {
+      if (c4_routes == GARY_C4_ROUTES_TABLE)  // pageable copy: h_table outlives this call's sync
+        GARY_HIP(hipMemcpyAsync(d_table, h_table, kRouteBins, hipMemcpyHostToDevice, st));
+      GARY_TRY(c4_classify(*g, lo_len, cls, csum, st));
+      if (c4_routes != GARY_C4_ROUTES_TABLE) {
+        c4_mlp_table_kernel<<<1, kRouteBins, 0, st>>>(model.P, csum, log2f((float)m),
+                                                      model.margin, d_table);
+        GARY_HIP(hipGetLastError());
+      }
+      // dense vertices join the list that the last light kernel passes to c4_kernel
+      c4_route_kernel<<<lgrid, 256, 0, st>>>(lo_len, cls, m, d_table, light, bad + 7,
+                                             use_block ? light_over2 : light_over,
+                                             use_block ? bad + 12 : bad + 11);
+      GARY_HIP(hipGetLastError());
+    }
+    window_flag_kernel<<<lgrid, 256, 0, st>>>(lo_len, by_table ? cls : nullptr,
+                                              by_table ? d_table : nullptr, m, wflag, bad + 5);
     GARY_HIP(hipGetLastError());
-    range_list_kernel<<<lgrid, 256, 0, st>>>(lo_len, m, 2, kC4HeavyLo, light, bad + 7, nullptr);
+    GARY_TRY(exclusive_scan_u32(wflag, nflag, wpart, st));
+    window_list_kernel<<<lgrid, 256, 0, st>>>(wflag, m, heavy, bad + 4);
     GARY_HIP(hipGetLastError());
     c4_light_wave_kernel<<<std::max(1, ncu * 7), kLightWaves * kWave, 0, st>>>(
         *g, lo_len, light, bad + 7, light_over, bad + 11, sums + 4);  // 21 KiB of LDS: 7 per CU
     GARY_HIP(hipGetLastError());
     const int32_t *rest = light_over, *nrest = bad + 11;
-    if (m - 1 <= kBlockMaxId) {  // packed keys hold every id
+    if (use_block) {
       c4_light_block_kernel<<<std::max(1, ncu), kBlockThreads, 0, st>>>(
           *g, lo_len, light_over, bad + 11, light_over2, bad + 12, bad + 13, sums + 4);
       GARY_HIP(hipGetLastError());
@@ -1349,6 +1736,80 @@ int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *
     out->hom_c4 = (uint64_t)hc4;
     out->c4 = (uint64_t)c4;
   }
+  return GARY_OK;
+  GARY_API_END
+}
+
+int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *out,
+                    void *stream) {
+  return gary_hom4_count_routed(g, patterns, GARY_C4_ROUTES_AUTO, nullptr, out, stream);
+}
+
+int gary_hom4_c4_features(const gary_device_graph *g, float *h_x, float *h_vertices) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(g && h_x && h_vertices, "gary_hom4_c4_features: null pointer");
+  GARY_CHECK_ARG(g->csr.m >= 0 && g->csr.nnz >= 0, "gary_hom4_c4_features: bad sizes");
+  return c4_inspect(*g, GARY_C4_ROUTES_AUTO, h_x, h_vertices, nullptr);
+  GARY_API_END
+}
+
+int gary_hom4_c4_table(const gary_device_graph *g, int c4_routes, uint8_t *h_table) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(g && h_table, "gary_hom4_c4_table: null pointer");
+  GARY_CHECK_ARG(g->csr.m >= 0 && g->csr.nnz >= 0, "gary_hom4_c4_table: bad sizes");
+  GARY_CHECK_ARG(c4_routes >= GARY_C4_ROUTES_AUTO && c4_routes <= GARY_C4_ROUTES_LEARNED,
+                 "gary_hom4_c4_table: mode %d is not AUTO, FIXED or LEARNED", c4_routes);
+  return c4_inspect(*g, c4_routes, nullptr, nullptr, h_table);
+  GARY_API_END
+}
+
+int gary_hom4_c4_set_model(const float *h_params, float margin) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(margin >= 0.f && margin < 1.f, "gary_hom4_c4_set_model: margin %g outside [0, 1)",
+                 (double)margin);
+  if (h_params)
+    for (int i = 0; i < kParams; ++i)
+      GARY_CHECK_ARG(std::isfinite(h_params[i]), "gary_hom4_c4_set_model: parameter %d is not finite", i);
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  return c4_set_model(dev, h_params, margin);
+  GARY_API_END
+}
+
+int gary_hom4_c4_load_model(const char *path) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(path, "gary_hom4_c4_load_model: null path");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return set_error(GARY_ERR_INVALID, "gary_hom4_c4_load_model: cannot open %s", path);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string doc = ss.str();
+  std::vector<double> params, margin;
+  if (!selector_file_numbers(doc, "params", params) || params.size() != (size_t)kParams)
+    return set_error(GARY_ERR_INVALID, "gary_hom4_c4_load_model: %s has no \"params\" array of "
+                     "%d numbers", path, kParams);
+  selector_file_numbers(doc, "margin", margin);
+  std::vector<float> p(params.begin(), params.end());
+  return gary_hom4_c4_set_model(p.data(), margin.empty() ? 0.f : (float)margin[0]);
+  GARY_API_END
+}
+
+int gary_hom4_c4_model(int *trained, float *h_params, float *margin) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(trained, "gary_hom4_c4_model: null pointer");
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  C4Model md;
+  GARY_TRY(c4_model_of(dev, &md));
+  *trained = md.trained ? 1 : 0;
+  if (md.trained && h_params)
+    GARY_HIP(hipMemcpy(h_params, md.P, sizeof(float) * kParams, hipMemcpyDeviceToHost));
+  if (md.trained && margin) *margin = md.margin;
   return GARY_OK;
   GARY_API_END
 }

@@ -32,19 +32,13 @@
 
 #include "common.h"
 #include "device_util.h"
+#include "mlp.h"
 #include "tc_internal.h"
 #include "workspace.h"
 
 namespace gary {
 
 namespace {
-
-constexpr int kIn = GARY_MLP_IN, kHid = GARY_MLP_HIDDEN, kOut = GARY_MLP_OUT;
-constexpr int kParams = GARY_MLP_PARAMS;
-// parameter layout: W1[kHid][kIn] b1[kHid] W2[kHid][kHid] b2[kHid] W3[kOut][kHid] b3[kOut]
-constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn, oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid,
-              oW3 = oB2 + kHid, oB3 = oW3 + kOut * kHid;
-static_assert(oB3 + kOut == kParams, "parameter layout");
 
 __device__ __forceinline__ int ilog2c(int32_t x) { return min(15, 31 - __clz(max(x, 1))); }
 
@@ -268,25 +262,6 @@ __global__ void bin_feature_rows_kernel(const BinSums *__restrict__ sums, float 
 
 // ---- MLP -------------------------------------------------------------------------------------------
 
-__device__ __forceinline__ void mlp_forward(const float *__restrict__ P, const float *x, float *h1,
-                                            float *h2, float *y) {
-  for (int k = 0; k < kHid; ++k) {
-    float a = P[oB1 + k];
-    for (int i = 0; i < kIn; ++i) a += P[oW1 + k * kIn + i] * x[i];
-    h1[k] = a > 0.f ? a : 0.f;
-  }
-  for (int k = 0; k < kHid; ++k) {
-    float a = P[oB2 + k];
-    for (int i = 0; i < kHid; ++i) a += P[oW2 + k * kHid + i] * h1[i];
-    h2[k] = a > 0.f ? a : 0.f;
-  }
-  for (int o = 0; o < kOut; ++o) {
-    float a = P[oB3 + o];
-    for (int i = 0; i < kHid; ++i) a += P[oW3 + o * kHid + i] * h2[i];
-    y[o] = a;
-  }
-}
-
 __global__ void mlp_forward_kernel(const float *__restrict__ P, const float *__restrict__ X,
                                    int32_t n, float *__restrict__ Y) {
   __shared__ float sP[kParams];
@@ -473,13 +448,11 @@ struct HeuristicThresholds {
   int32_t bitmap_from = 6, small_max = 5, skew = 3;
 };
 
-struct Model {
+struct Model : MlpTrainState {
   float fixed_lx[4] = {0.f, 0.f, 0.f, 0.f};  // fixed route costs: points (log2(1 + nnz), ns)
   float fixed_f[4][3] = {};
   int fixed_n = 0;
   bool trained = false;
-  float *P = nullptr, *M1 = nullptr, *M2 = nullptr, *grad = nullptr, *loss = nullptr;
-  int64_t steps = 0;
   float cost_big = 0.f;  // ns; pivots above get a workgroup (0 = degree split)
   uint32_t allowed = 7;
   float margin = 0.f;  // predicted relative gain needed to route arcs away from the bitmap
@@ -544,19 +517,7 @@ FixedCosts fixed_of(const Model &m) {
   return f;
 }
 
-int ensure_model(DeviceState &s) {
-  Model &m = s.model;
-  if (m.P) return GARY_OK;
-  GARY_HIP(hipMalloc(&m.P, sizeof(float) * kParams));
-  GARY_HIP(hipMalloc(&m.M1, sizeof(float) * kParams));
-  GARY_HIP(hipMalloc(&m.M2, sizeof(float) * kParams));
-  GARY_HIP(hipMalloc(&m.grad, sizeof(float) * kParams));
-  GARY_HIP(hipMalloc(&m.loss, sizeof(float)));
-  GARY_HIP(hipMemset(m.M1, 0, sizeof(float) * kParams));
-  GARY_HIP(hipMemset(m.M2, 0, sizeof(float) * kParams));
-  GARY_HIP(hipMemset(m.P, 0, sizeof(float) * kParams));
-  return GARY_OK;
-}
+int ensure_model(DeviceState &s) { return mlp_alloc(s.model); }
 
 __global__ void fill_zero_kernel_sel(uint32_t *__restrict__ p, int32_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -674,11 +635,19 @@ int launch_routed_ws(const gary_device_graph &g, const RoutedWs &w, const RouteT
   return launch_edge_lists(g, w.cached, w.counts, w.merge, w.counts + 1, cnt, st);
 }
 
-// The numbers of the value of top-level "key" in a selector file (a number, or an array of
-// numbers, nested arrays flattened in order). The file is the JSON object that
-// gary_amd/selector.py writes: "params", "cost_big", "allowed", "margin", "fixed_nnz",
-// "fixed_costs" come before any nested report, so the first occurrence of a key is the top-level
-// one. Returns false when the key is absent or its value holds no number.
+int check_routed(const gary_device_graph &g) {
+  GARY_CHECK_ARG(g.coo.rowIdx, "routed counting needs the COO rowIdx (call populate first)");
+  GARY_CHECK_ARG(g.csr.nnz <= (int64_t)UINT32_MAX, "routed counting: more than 2^32-1 arcs");
+  return GARY_OK;
+}
+
+}  // namespace
+
+// The numbers of the value of top-level "key" in a model file (a number, or an array of
+// numbers, nested arrays flattened in order). The files are the JSON objects that
+// gary_amd/selector.py and gary_amd/c4_selector.py write: their numeric keys ("params",
+// "margin", ...) come before any nested report, so the first occurrence of a key is the
+// top-level one. Returns false when the key is absent or its value holds no number.
 bool selector_file_numbers(const std::string &doc, const char *key, std::vector<double> &out) {
   out.clear();
   const std::string k = std::string("\"") + key + "\"";
@@ -712,13 +681,74 @@ bool selector_file_numbers(const std::string &doc, const char *key, std::vector<
   return depth == 0 && !out.empty();
 }
 
-int check_routed(const gary_device_graph &g) {
-  GARY_CHECK_ARG(g.coo.rowIdx, "routed counting needs the COO rowIdx (call populate first)");
-  GARY_CHECK_ARG(g.csr.nnz <= (int64_t)UINT32_MAX, "routed counting: more than 2^32-1 arcs");
+
+int mlp_alloc(MlpTrainState &m) {
+  if (m.P) return GARY_OK;
+  GARY_HIP(hipMalloc(&m.P, sizeof(float) * kParams));
+  GARY_HIP(hipMalloc(&m.M1, sizeof(float) * kParams));
+  GARY_HIP(hipMalloc(&m.M2, sizeof(float) * kParams));
+  GARY_HIP(hipMalloc(&m.grad, sizeof(float) * kParams));
+  GARY_HIP(hipMalloc(&m.loss, sizeof(float)));
+  GARY_HIP(hipMemset(m.M1, 0, sizeof(float) * kParams));
+  GARY_HIP(hipMemset(m.M2, 0, sizeof(float) * kParams));
+  GARY_HIP(hipMemset(m.P, 0, sizeof(float) * kParams));
   return GARY_OK;
 }
 
-}  // namespace
+int mlp_init(MlpTrainState &s, uint64_t seed) {
+  std::vector<float> p(kParams, 0.f);
+  uint64_t st = seed;
+  auto next = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int off, int cnt, int fan_in) {
+    const double lim = std::sqrt(6.0 / fan_in);
+    for (int i = 0; i < cnt; ++i) p[(size_t)off + i] = (float)((2.0 * next() - 1.0) * lim);
+  };
+  fill(oW1, kHid * kIn, kIn);
+  fill(oW2, kHid * kHid, kHid);
+  fill(oW3, kOut * kHid, kHid);
+  GARY_HIP(hipMemcpy(s.P, p.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
+  GARY_HIP(hipMemset(s.M1, 0, sizeof(float) * kParams));
+  GARY_HIP(hipMemset(s.M2, 0, sizeof(float) * kParams));
+  s.steps = 0;
+  return GARY_OK;
+}
+
+int mlp_train(MlpTrainState &s, const float *h_x, const float *h_t, const float *h_mask, int32_t n,
+              int32_t epochs, float lr, float *h_loss) {
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  void *ws = nullptr;
+  const size_t fl = (size_t)n * (kIn + 2 * kOut + kParams + 1);
+  GARY_TRY(workspace_get(dev, (void *)7, sizeof(float) * fl, &ws));
+  float *X = (float *)ws, *T = X + (size_t)n * kIn, *M = T + (size_t)n * kOut;
+  float *G = M + (size_t)n * kOut, *L = G + (size_t)n * kParams;
+  GARY_HIP(hipMemcpy(X, h_x, sizeof(float) * (size_t)n * kIn, hipMemcpyHostToDevice));
+  GARY_HIP(hipMemcpy(T, h_t, sizeof(float) * (size_t)n * kOut, hipMemcpyHostToDevice));
+  GARY_HIP(hipMemcpy(M, h_mask, sizeof(float) * (size_t)n * kOut, hipMemcpyHostToDevice));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int32_t ep = 0; ep < epochs; ++ep) {
+    ++s.steps;
+    const float c1 = 1.f - std::pow(b1, (float)s.steps), c2 = 1.f - std::pow(b2, (float)s.steps);
+    mlp_backward_kernel<<<(n + 127) / 128, 128>>>(s.P, X, T, M, n, G, L);
+    mlp_reduce_kernel<<<(kParams + 1 + 127) / 128, 128>>>(G, L, n, s.grad, s.loss);
+    adam_kernel<<<(kParams + 127) / 128, 128>>>(s.P, s.grad, s.M1, s.M2, lr, b1, b2, eps, c1, c2);
+  }
+  GARY_HIP(hipGetLastError());
+  mlp_backward_kernel<<<(n + 127) / 128, 128>>>(s.P, X, T, M, n, G, L);
+  mlp_reduce_kernel<<<(kParams + 1 + 127) / 128, 128>>>(G, L, n, s.grad, s.loss);
+  GARY_HIP(hipGetLastError());
+  float loss = 0.f;
+  GARY_HIP(hipMemcpy(&loss, s.loss, sizeof(float), hipMemcpyDeviceToHost));
+  if (h_loss) *h_loss = loss;
+  return GARY_OK;
+}
 
 int launch_routed(const gary_device_graph &g, const RouteTable *table, const PivotCfg &pc,
                   float cost_big, unsigned long long *cnt, hipStream_t st) {
@@ -1022,53 +1052,27 @@ int gary_selector_train(const float *h_x, const float *h_t, const float *h_mask,
   DeviceState *s = nullptr;
   GARY_TRY(state_for(dev, &s));
   GARY_TRY(ensure_model(*s));
-  Model &md = s->model;
-  if (seed) {  // fresh He-uniform initialisation from a splitmix64 stream, zero biases
-    std::vector<float> p(kParams, 0.f);
-    uint64_t st = seed;
-    auto next = [&]() {
-      st += 0x9E3779B97F4A7C15ull;
-      uint64_t z = st;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-      z ^= z >> 31;
-      return (double)(z >> 11) * (1.0 / 9007199254740992.0);
-    };
-    auto fill = [&](int off, int cnt, int fan_in) {
-      const double lim = std::sqrt(6.0 / fan_in);
-      for (int i = 0; i < cnt; ++i) p[(size_t)off + i] = (float)((2.0 * next() - 1.0) * lim);
-    };
-    fill(oW1, kHid * kIn, kIn);
-    fill(oW2, kHid * kHid, kHid);
-    fill(oW3, kOut * kHid, kHid);
-    GARY_HIP(hipMemcpy(md.P, p.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
-    GARY_HIP(hipMemset(md.M1, 0, sizeof(float) * kParams));
-    GARY_HIP(hipMemset(md.M2, 0, sizeof(float) * kParams));
-    md.steps = 0;
-  }
-  void *ws = nullptr;
-  const size_t fl = (size_t)n * (kIn + 2 * kOut + kParams + 1);
-  GARY_TRY(workspace_get(dev, (void *)7, sizeof(float) * fl, &ws));
-  float *X = (float *)ws, *T = X + (size_t)n * kIn, *M = T + (size_t)n * kOut;
-  float *G = M + (size_t)n * kOut, *L = G + (size_t)n * kParams;
-  GARY_HIP(hipMemcpy(X, h_x, sizeof(float) * (size_t)n * kIn, hipMemcpyHostToDevice));
-  GARY_HIP(hipMemcpy(T, h_t, sizeof(float) * (size_t)n * kOut, hipMemcpyHostToDevice));
-  GARY_HIP(hipMemcpy(M, h_mask, sizeof(float) * (size_t)n * kOut, hipMemcpyHostToDevice));
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (int32_t ep = 0; ep < epochs; ++ep) {
-    ++md.steps;
-    const float c1 = 1.f - std::pow(b1, (float)md.steps), c2 = 1.f - std::pow(b2, (float)md.steps);
-    mlp_backward_kernel<<<(n + 127) / 128, 128>>>(md.P, X, T, M, n, G, L);
-    mlp_reduce_kernel<<<(kParams + 1 + 127) / 128, 128>>>(G, L, n, md.grad, md.loss);
-    adam_kernel<<<(kParams + 127) / 128, 128>>>(md.P, md.grad, md.M1, md.M2, lr, b1, b2, eps, c1, c2);
-  }
-  GARY_HIP(hipGetLastError());
-  mlp_backward_kernel<<<(n + 127) / 128, 128>>>(md.P, X, T, M, n, G, L);
-  mlp_reduce_kernel<<<(kParams + 1 + 127) / 128, 128>>>(G, L, n, md.grad, md.loss);
-  GARY_HIP(hipGetLastError());
-  float loss = 0.f;
-  GARY_HIP(hipMemcpy(&loss, md.loss, sizeof(float), hipMemcpyDeviceToHost));
-  if (h_loss) *h_loss = loss;
+  if (seed) GARY_TRY(mlp_init(s->model, seed));  // else continue from the current parameters
+  return mlp_train(s->model, h_x, h_t, h_mask, n, epochs, lr, h_loss);
+  GARY_API_END
+}
+
+int gary_mlp_train(const float *h_x, const float *h_t, const float *h_mask, int32_t n,
+                   int32_t epochs, float lr, uint64_t seed, float *h_params, float *h_loss) {
+  GARY_API_BEGIN
+  using namespace gary;
+  GARY_CHECK_ARG(h_x && h_t && h_mask && h_params && n >= 1 && n <= (1 << 16) && epochs >= 0 &&
+                     lr > 0.f && seed != 0,
+                 "gary_mlp_train: bad args");
+  int dev = 0;
+  GARY_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lock(g_mu);
+  static std::map<int, MlpTrainState> *trainers = new std::map<int, MlpTrainState>();
+  MlpTrainState &t = (*trainers)[dev];
+  GARY_TRY(mlp_alloc(t));
+  GARY_TRY(mlp_init(t, seed));
+  GARY_TRY(mlp_train(t, h_x, h_t, h_mask, n, epochs, lr, h_loss));
+  GARY_HIP(hipMemcpy(h_params, t.P, sizeof(float) * kParams, hipMemcpyDeviceToHost));
   return GARY_OK;
   GARY_API_END
 }

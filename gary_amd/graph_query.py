@@ -86,14 +86,30 @@ class Hom4Counts(C.Structure):
 
 
 _bind("gary_hom4_count", C.c_int, [C.POINTER(DeviceGraph), C.c_int, C.POINTER(Hom4Counts), _P])
+_bind("gary_hom4_count_routed", C.c_int, [C.POINTER(DeviceGraph), C.c_int, C.c_int, _P,
+                                          C.POINTER(Hom4Counts), _P])
+
+# 4-cycle routing modes (gary_hom4_count_routed); a route table is GARY_C4_BINS route bytes
+C4_AUTO, C4_FIXED, C4_LEARNED, C4_TABLE = 0, 1, 2, 3
 
 
-def hom4Count(g, patterns: int = HOM_ALL, stream: int | None = None) -> dict:
+def hom4Count(g, patterns: int = HOM_ALL, stream: int | None = None, c4=None) -> dict:
     """4-vertex homomorphism counts of a ranked symmetric graph (graph.SYMMETRIC_RANKED):
-    dict with tri, k4, hom_k4, hom_diamond, hom_c4, c4."""
+    dict with tri, k4, hom_k4, hom_diamond, hom_c4, c4. `c4` routes the 4-cycle vertices: None
+    (AUTO: the learned model when one is set, else the fixed degree split), C4_FIXED,
+    C4_LEARNED, or a table of 256 route bytes (gary_amd.c4_selector)."""
     dg = _graph(g)
     out = Hom4Counts()
-    check(lib.gary_hom4_count(C.byref(dg), patterns, C.byref(out), stream))
+    if c4 is None:
+        check(lib.gary_hom4_count(C.byref(dg), patterns, C.byref(out), stream))
+    elif isinstance(c4, int):
+        check(lib.gary_hom4_count_routed(C.byref(dg), patterns, c4, None, C.byref(out), stream))
+    else:
+        t = np.ascontiguousarray(c4, np.uint8)
+        if t.shape != (256,):
+            raise ValueError("a 4-cycle route table has 256 entries")
+        check(lib.gary_hom4_count_routed(C.byref(dg), patterns, C4_TABLE,
+                                         t.ctypes.data_as(C.c_void_p), C.byref(out), stream))
     return {k: int(getattr(out, k)) for k, _ in Hom4Counts._fields_}
 
 

@@ -348,6 +348,10 @@ int gary_mlp_loss_grad(const float *h_params, const float *h_x, const float *h_t
 /* Full-batch Adam on the device-resident model; seed != 0 re-initialises it first. */
 int gary_selector_train(const float *h_x, const float *h_t, const float *h_mask, int32_t n,
                         int32_t epochs, float lr, uint64_t seed, float *h_loss);
+/* The same training from a fresh initialisation (seed != 0) into h_params (GARY_MLP_PARAMS
+ * floats), leaving every model of the library as it is (trains the 4-cycle routing model). */
+int gary_mlp_train(const float *h_x, const float *h_t, const float *h_mask, int32_t n,
+                   int32_t epochs, float lr, uint64_t seed, float *h_params, float *h_loss);
 /* Model parameters in/out; `trained` enables GARY_TC_AUTO_LEARNED (else it runs the heuristic
  * table); `allowed` is a bit mask of the routes the model may pick; the learned table leaves the
  * all-bitmap routing only when another route subset is predicted to save more than `margin`
@@ -409,6 +413,52 @@ typedef struct gary_hom4_counts {
  * `stream`; values that exceed 64 bits fail with GARY_ERR_OVERFLOW. */
 int gary_hom4_count(const gary_device_graph *g, int patterns, gary_hom4_counts *out,
                     void *stream);
+
+/* 4-cycle vertex routes. The 4-cycles are counted at their highest-ranked vertex u from the
+ * wedges u-v-w (v, w < u). Each vertex with nl >= 2 lower neighbours falls in bin
+ * 16 * min(15, floor(log2 nl)) + min(15, floor(log2 S) / 2), S = the sum of d(v) over its lower
+ * neighbours v (an upper bound on its wedges; above 64 lower neighbours, nl / 64 times the sum over
+ * 64 evenly spaced ones), and a table of GARY_C4_BINS route bytes sends each bin to one counting
+ * path: */
+enum {
+  GARY_C4_LIGHT = 0,  /* LDS hash table of one wave, then of one workgroup, then dense counters,
+                         as the vertex's wedges overflow each (nl < 256 only; above, the windows) */
+  GARY_C4_WINDOW = 1, /* LDS counters over windows of 65,536 endpoint ids */
+  GARY_C4_DENSE = 2,  /* one workgroup with a dense counter array in device memory */
+  GARY_C4_SKIP = 3    /* not counted (for timing the others) */
+};
+#define GARY_C4_BINS 256
+/* How a call routes its vertices: AUTO is LEARNED when a 4-cycle model is set on the device
+ * (gary_hom4_c4_set_model / gary_hom4_c4_load_model), else FIXED; FIXED is the degree split
+ * (nl >= 256 to the windows, the rest light); LEARNED takes per bin the route with the least
+ * MLP-predicted ns per vertex (fails with GARY_ERR_STATE without a model); TABLE uses the
+ * caller's table. */
+enum {
+  GARY_C4_ROUTES_AUTO = 0,
+  GARY_C4_ROUTES_FIXED = 1,
+  GARY_C4_ROUTES_LEARNED = 2,
+  GARY_C4_ROUTES_TABLE = 3
+};
+/* gary_hom4_count with the 4-cycle routes chosen by `c4_routes` (h_table: GARY_C4_BINS route
+ * bytes for GARY_C4_ROUTES_TABLE, else ignored). No reference counterpart; counts do not depend
+ * on the routes, except that GARY_C4_SKIP bins are left out of c4 and hom_c4. */
+int gary_hom4_count_routed(const gary_device_graph *g, int patterns, int c4_routes,
+                           const uint8_t *h_table, gary_hom4_counts *out, void *stream);
+/* Per-bin MLP inputs of the 4-cycle routing of g (h_x: GARY_C4_BINS x GARY_MLP_IN floats) and the
+ * vertices in each bin (h_vertices: GARY_C4_BINS floats). Synchronous. */
+int gary_hom4_c4_features(const gary_device_graph *g, float *h_x, float *h_vertices);
+/* The table a routing mode (AUTO, FIXED or LEARNED) gives g. Synchronous. */
+int gary_hom4_c4_table(const gary_device_graph *g, int c4_routes, uint8_t *h_table);
+/* Sets the 4-cycle model of the current device: GARY_MLP_PARAMS floats (the layout of the
+ * selector's MLP; outputs = log2 ns per vertex of routes LIGHT, WINDOW, DENSE) and the predicted
+ * relative gain needed to leave the fixed route (0 <= margin < 1). h_params = NULL removes it. */
+int gary_hom4_c4_set_model(const float *h_params, float margin);
+/* Reads "params" (GARY_MLP_PARAMS numbers) and optional "margin" from the JSON file that
+ * gary_amd/c4_selector.py writes, and sets them as above. */
+int gary_hom4_c4_load_model(const char *path);
+/* Whether the current device has a 4-cycle model; h_params (GARY_MLP_PARAMS floats) and margin
+ * receive it when non-null and set. */
+int gary_hom4_c4_model(int *trained, float *h_params, float *margin);
 
 /* Number of k-cliques (1 <= k <= 10) of the simple graph given as a GARY_CSR_SYMMETRIC_RANKED CSR:
  * for each vertex u, the (k-1)-cliques of the DAG induced on its higher-ranked neighbours, counted
