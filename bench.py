@@ -353,7 +353,7 @@ class Bench:
             leg[f"{nm}_ms"] = float(np.median(ts))
             log(f"rmat22 hom4 {nm}: {leg[f'{nm}_ms']:.1f} ms (median of 5)")
         leg["counts"] = vals
-        if model:
+        if model and not self.a.no_c4_compare:
             leg["c4_learned_vs_fixed"] = {"rmat22": self.c4_routing(ctx)}
             del ctx
             n20, s20, d20 = graph.rmat(20, 16, seed=graph.SEEDS["rmat22"])
@@ -688,6 +688,9 @@ def main(argv=None) -> int:
     ap.add_argument("--build-graphs", default="rmat20_degree,rmat22_ranked",
                     help="graphs of the build leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-c4-compare", action="store_true",
+                    help="hom4 leg: skip the learned vs fixed 4-cycle routing comparison (its "
+                         "R-MAT 20 calls would mix into a profile of the R-MAT 22 kernels)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--only", action="store_true",
                     help="headline: time --strategy only, and run no other leg (profiler runs)")

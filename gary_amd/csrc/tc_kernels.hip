@@ -17,8 +17,10 @@
 //    and a wave's loads are contiguous. Each N+(u) is read once per pivot rather than once per
 //    arc. Pivots are binned on the device by out-degree, heaviest first.
 //
-// Counts are per-lane 32-bit partials, reduced per wave to 64 bits and added with one atomic per
-// wave.
+// Counts: each lane sums 32-bit counts per edge (at most min(d(u), d(v)) < 2^31) or per pivot
+// call of stream_rows / pivot_fallback (bounded below 2^32 there) into a 64-bit partial, so no
+// partial wraps however many edges or pivots a persistent lane takes; partials are reduced per
+// wave and added with one atomic per workgroup.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
   const uint32_t m = (uint32_t)g.csr.m;
   int64_t e, e_end, step;
   if constexpr (PART == GARY_PARTITION_DYNAMIC) {
-    uint32_t c = 0;
+    uint64_t c = 0;
     const int leader = lane_id() & ~(G - 1);
     while (true) {
       unsigned long long base = 0;
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__(256) tc_edge_kernel(gary_device_graph g, int32
     e_end = E;
     step = ngroups;
   }
-  uint32_t c = 0;
+  uint64_t c = 0;
   for (; e < e_end; e += step) {
     if (route && route[e] != want) continue;
     const int32_t src = g.coo.rowIdx[e], dst = g.coo.colIdx[e];
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(256) tc_edge_list_kernel(gary_device_graph g, 
   int32_t *cache = smem + (size_t)gib * n_cache;
   const int64_t nc = *n_cached, E = nc + *n_merge;
   const uint32_t m = (uint32_t)g.csr.m;
-  uint32_t c = 0;
+  uint64_t c = 0;
   for (int64_t k = (int64_t)blockIdx.x * gpb + gib; k < E; k += ngroups) {
     const uint32_t e = k < nc ? cached[k] : merge[k - nc];
     const int32_t src = g.coo.rowIdx[e], dst = g.csr.colIdx[e];
@@ -411,6 +413,9 @@ __device__ __forceinline__ uint32_t probe_chunk(const int4 (&data)[CH / 4], int3
 // blocks that hold colIdx elements. Chunk indices are 32-bit (colIdx below 2^34 elements).
 // (16-byte chunks with two windows in flight: 3.38 ms per R-MAT 20 count; 32-byte chunks with
 // a reset slot array and four shuffles: 3.21 ms; this form: 3.10 ms.)
+// The returned per-lane count is below 2^32: a lane probes at most one chunk (CH elements) per
+// window, and a call has at most (Σ_v (d(v) / CH + 2)) / 64 + d / 64 + 1 windows, so the count is
+// at most nnz / 64 + 3d / 8 + 8 < 2^28 + 2^30 + 8 for colIdx below 2^34 elements and d < 2^31.
 __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
                                                 const int32_t *__restrict__ L, int32_t d,
                                                 int32_t r0, int32_t stride, const int32_t *table,
@@ -481,6 +486,7 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
 }
 
 // Edge-by-edge fallback for a pivot whose set does not fit in LDS: rows r0, r0+stride, ...
+// A lane's count per row is at most ceil(d(v) / 64), so the call's is below nnz / 64 + d < 2^32.
 __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
                                                    const int32_t *__restrict__ L, int32_t d,
                                                    int32_t r0, int32_t stride, int lane,
@@ -531,7 +537,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
   const int32_t n_small = counts[0], n_big = counts[1];
   if (n_small == 0 && n_big == 0) return;
   const uint32_t m = (uint32_t)g.csr.m;
-  uint32_t c = 0;
+  uint64_t c = 0;
   // ---- phase 1: big pivots, one workgroup each
   while (true) {
     if (threadIdx.x == 0) table[0] = atomicAdd(&cursors[1], 1);
