@@ -7,6 +7,8 @@
 // the launch goes through the C ABI. triangleCount() exposes every strategy.
 #pragma once
 
+#include <utility>
+
 #include "../DeviceGraph.h"
 
 namespace graph_query {
@@ -23,6 +25,15 @@ enum class Strategy : int {
 // Elements of index cache per workgroup (getIndexCacheSMem); multiply by sizeof(vidT).
 inline size_t getIndexCacheSMem(int blockDim, size_t cacheSize) {
   return (size_t)gary_index_cache_smem(blockDim, (int64_t)cacheSize);
+}
+
+// getIndexCacheSMem(dim3 blockDim, size_t cacheSize) (src/GraphQueryHom.cu:11-13): any block
+// shape with x, y, z members (HIP's dim3), whose threads x * y * z make the block.
+template <class Dim3, class = decltype(std::declval<const Dim3 &>().x *
+                                       std::declval<const Dim3 &>().y *
+                                       std::declval<const Dim3 &>().z)>
+inline size_t getIndexCacheSMem(const Dim3 &blockDim, size_t cacheSize) {
+  return getIndexCacheSMem((int)(blockDim.x * blockDim.y * blockDim.z), cacheSize);
 }
 
 // count += Σ over arcs (u,v) of |N+(u) ∩ N+(v)| (caller-zeroed device counter).

@@ -26,6 +26,31 @@ def test_headers_compile_standalone(tmp_path, hdr):
         assert r.returncode == 0, r.stderr
 
 
+def test_index_cache_smem_takes_a_block_shape(tmp_path):
+    """getIndexCacheSMem accepts a dim3-like block shape as the reference's does
+    (src/GraphQueryHom.cu:11-13): waves of x*y*z threads times the cache size."""
+    src = tmp_path / "t.cpp"
+    src.write_text('''#include "kernels/GraphQueryHom.h"
+#include <cstdio>
+struct Shape { unsigned x, y, z; };
+int main() {
+  const size_t a = graph_query::getIndexCacheSMem(Shape{256, 1, 1}, 32);
+  const size_t b = graph_query::getIndexCacheSMem(Shape{64, 2, 2}, 32);
+  const size_t c = graph_query::getIndexCacheSMem(256, 32);
+  std::printf("%zu %zu %zu\\n", a, b, c);
+  return (a == 128 && b == 128 && c == 128) ? 0 : 1;
+}
+''')
+    exe = tmp_path / "t"
+    lib = ROOT / "gary_amd" / "libgary.so"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src),
+                        "-o", str(exe), str(lib), f"-Wl,-rpath,{lib.parent}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 def test_facade_binary_built():
     assert BIN.exists(), "tests/cpp/tri_query_test is built by __graft_entry__.build()"
 
