@@ -399,18 +399,33 @@ __device__ __forceinline__ uint32_t probe_hash(const int32_t *t, const PivotSet 
 template <int CH, bool HASH>
 __device__ __forceinline__ uint32_t probe_chunk(const int4 (&data)[CH / 4], int32_t jlo, int32_t jhi,
                                                 const int32_t *table, const PivotSet &ps) {
-  uint32_t c = 0;
+  if constexpr (HASH) {  // a hash probe loops: only the chunk's own elements are probed
+    uint32_t c = 0;
 #pragma unroll
-  for (int h = 0; h < CH / 4; ++h) {
-    const int32_t el[4] = {data[h].x, data[h].y, data[h].z, data[h].w};
+    for (int h = 0; h < CH / 4; ++h) {
+      const int32_t el[4] = {data[h].x, data[h].y, data[h].z, data[h].w};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int pos = 4 * h + q;
-      if (jlo <= pos && pos < jhi)
-        c += HASH ? probe_hash(table, ps, el[q]) : probe_bitmap(table, ps, el[q]);
+      for (int q = 0; q < 4; ++q) {
+        const int pos = 4 * h + q;
+        if (jlo <= pos && pos < jhi) c += probe_hash(table, ps, el[q]);
+      }
     }
+    return c;
+  } else {
+    // Every slot is probed (a bitmap probe reads inside the set whatever the value) into a hit
+    // mask, and the slots of the chunk's row [jlo, jhi) are selected once: a per-slot range
+    // check cost two compares, a cleared result and an add per element.
+    uint32_t hits = 0;
+#pragma unroll
+    for (int h = 0; h < CH / 4; ++h) {
+      const int32_t el[4] = {data[h].x, data[h].y, data[h].z, data[h].w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) hits |= probe_bitmap(table, ps, el[q]) << (4 * h + q);
+    }
+    const int32_t lo = min(max(jlo, 0), CH), hi = min(max(jhi, 0), CH);
+    const uint32_t valid = hi > lo ? (((1u << (hi - lo)) - 1u) << lo) : 0u;
+    return (uint32_t)__popc(hits & valid);
   }
-  return c;
 }
 
 // The rows r0, r0 + stride, ... (< d) of pivot list L, 64 rows per batch. A batch is flattened
