@@ -1003,7 +1003,7 @@ __global__ void __launch_bounds__(kC4Block)
   __shared__ uint32_t cnt[kC4Words];
   __shared__ int32_t touch[kC4Touch];  // window offsets whose counter left zero
   __shared__ int32_t slots[kC4Waves][kWave];
-  __shared__ int32_t s_item, s_ntouch;
+  __shared__ int32_t s_item, s_ntouch, s_kskip;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int64_t nheavy = *nheavy_p;
@@ -1012,7 +1012,21 @@ __global__ void __launch_bounds__(kC4Block)
   int32_t *nx = next_win + (int64_t)blockIdx.x * stride;     // window id of that run
   uint64_t acc = 0;
   for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
-  if (threadIdx.x == 0) s_ntouch = 0;
+  if (threadIdx.x == 0) {
+    s_ntouch = 0;
+    // An endpoint w of degree <= 1 closes no 4-cycle (c_u(w) <= d(w) <= 1), and ids are ranked
+    // by degree, so the windows below the first id of degree >= 2 are skipped whole (on R-MAT 22,
+    // 35 of its 64 windows: 43% of the ids are isolated and 56% have degree below 2).
+    int32_t lo = 0, hi = g.csr.m;
+    while (lo < hi) {
+      const int32_t mid = (int32_t)(((int64_t)lo + hi) >> 1);
+      if (g.csr.deg[mid] < 2) lo = mid + 1;
+      else hi = mid;
+    }
+    s_kskip = lo / kC4Win;
+  }
+  __syncthreads();
+  const int32_t kskip = s_kskip;
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(item_cursor, 1);
     __syncthreads();
@@ -1022,8 +1036,8 @@ __global__ void __launch_bounds__(kC4Block)
     const int32_t u = heavy[nheavy - 1 - it % nheavy];  // the list ascends by id (rank)
     const int32_t j = (int32_t)(it / nheavy);
     const int32_t K = (u + kC4Win - 1) / kC4Win;
-    const int32_t k0 = j * kC4Group, k1 = min(K, k0 + kC4Group);
-    if (k0 >= K) continue;
+    const int32_t k0 = max(j * kC4Group, kskip), k1 = min(K, j * kC4Group + kC4Group);
+    if (k0 >= k1) continue;
     const int32_t nl = lo_len[u];
     const int32_t *Nu = col + g.csr.indPtr[u];
     for (int32_t i = threadIdx.x; i < nl; i += kC4Block) {  // cursors at the first window
