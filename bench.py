@@ -105,6 +105,12 @@ class Bench:
         assert total % steps == 0
         return ms, total // steps, (kms / launches if launches else None)
 
+    def forward_cfg(self, workload, strategy: str):
+        """The tuned launch configuration of `strategy` with every arc streamed forward."""
+        c = self.autotune.cfg_for(workload, strategy) if workload else None
+        f = {k: getattr(c, k) for k in self.autotune.FIELDS} if c is not None else {}
+        return self.gq.launch_cfg(**f, direction=self.gq.FORWARD)
+
     def step_median(self, ctx, strategy: int, steps: int, cfg=None) -> float:
         """Median ms of `steps` counts each timed alone (HIP events around one count, the host
         waiting between counts): the per-step figure beside the back-to-back mean."""
@@ -172,6 +178,18 @@ class Bench:
             if kms is not None:
                 kernel_ms[nm] = kms
             log(f"{nm}: {ms:.3f} ms/step" + (f", triangles={tri}" if nm == a.strategy else ""))
+        direction = None
+        if not a.only:
+            # the pivot path with every arc streamed forward (no reversed-arc list), beside the
+            # default min-direction bitmap
+            fms, ft, fkms = self.time_count(ctx, self.names["bitmap"], a.steps, 1,
+                                            self.forward_cfg(wl, "bitmap"))
+            if ft != tri:
+                raise SystemExit(f"forward bitmap counted {ft} != {tri}")
+            direction = {"forward_ms": fms, "forward_kernel_ms": fkms,
+                         "min_direction_ms": results.get("bitmap"),
+                         "min_direction_kernel_ms": kernel_ms.get("bitmap")}
+            log(f"bitmap forward: {fms:.3f} ms/step (main kernel {fkms:.3f} ms)")
         self.dispatch_ms = None if a.only else self.time_dispatch(ctx, tri)
         medians = {nm: self.step_median(ctx, self.names[nm], a.steps,
                                         self.autotune.cfg_for(wl, nm) if wl else None)
@@ -215,6 +233,7 @@ class Bench:
                          if traffic else None},
             "kernel_ms": kernel_ms,
             "strategies_ms": results,
+            "pivot_direction": direction,
             "learned_vs_plain": results["heuristic"] / results[a.strategy]
             if "heuristic" in results and a.strategy != "heuristic" else 1.0,
             "tuned_cfg": {nm: (None if c is None else
@@ -246,6 +265,12 @@ class Bench:
             leg[f"{nm}_ms"] = ms
             log(f"chung_lu_4m {nm}: {ms:.3f} ms")
         leg["triangles"] = ref
+        fms, ft, _ = self.time_count(ctx, self.names["bitmap"], max(1, min(a.steps, 10)), 2,
+                                     self.forward_cfg("c3", "bitmap"))
+        if ft != ref:
+            raise SystemExit(f"chung_lu_4m: forward bitmap counted {ft} != {ref}")
+        leg["bitmap_forward_ms"] = fms
+        log(f"chung_lu_4m bitmap forward: {fms:.3f} ms")
         leg["reference_dispatch_ms"] = self.time_dispatch(ctx, ref)
         ht_ms, ht = self.time_heuristic_tuned(ctx, "c3", ref, max(1, min(a.steps, 10)),
                                               self.autotune.cfg_for("c3", "heuristic"))

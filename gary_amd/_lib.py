@@ -48,7 +48,7 @@ class DeviceGraph(C.Structure):
 class LaunchCfg(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("group", C.c_int32),
                 ("cache_depth", C.c_int32), ("partition", C.c_int32), ("bin_small", C.c_int32),
-                ("bin_hub", C.c_int32), ("reserved", C.c_int32)]
+                ("bin_hub", C.c_int32), ("direction", C.c_int32)]
 
 
 def _load() -> C.CDLL:

@@ -560,6 +560,7 @@ struct RoutedWs {
   RouteTable *table = nullptr;
   BinSums *sums = nullptr;
   size_t stride = 0;
+  RevBufs rev;  // reversed arcs of the bitmap route (pivot path)
 };
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -576,9 +577,11 @@ int routed_ws(const gary_device_graph &g, hipStream_t st, RoutedWs &w) {
   const size_t b_counts = 256;
   const size_t b_table = align256(sizeof(RouteTable));
   const size_t b_sums = align256(sizeof(BinSums) * kRouteBins * kSumCopies);
+  const size_t b_rev = rev_bytes((int32_t)m, nnz);
   void *ws = nullptr;
   GARY_TRY(workspace_get(dev, (void *)((uintptr_t)st ^ 0x5E1Eull),
-                         b_route + b_pcost + b_lists + 2 * b_arcs + b_counts + b_table + b_sums,
+                         b_route + b_pcost + b_lists + 2 * b_arcs + b_counts + b_table + b_sums +
+                             b_rev,
                          &ws));
   char *p = (char *)ws;
   w.route = (uint8_t *)p; p += b_route;
@@ -588,7 +591,8 @@ int routed_ws(const gary_device_graph &g, hipStream_t st, RoutedWs &w) {
   w.merge = (uint32_t *)p; p += b_arcs;
   w.counts = (int32_t *)p; p += b_counts;
   w.table = (RouteTable *)p; p += b_table;
-  w.sums = (BinSums *)p;
+  w.sums = (BinSums *)p; p += b_sums;
+  w.rev = rev_carve(p, (int32_t)m, nnz);
   return GARY_OK;
 }
 
@@ -630,7 +634,7 @@ int launch_routed_ws(const gary_device_graph &g, const RoutedWs &w, const RouteT
   if (sh.any_bitmap)
     GARY_TRY(launch_pivot_core(g, pc, w.lists, w.stride, w.lists + 3 * w.stride, nullptr, nullptr,
                                0, cnt, st, true, w.route, by_cost ? w.pcost : nullptr, cost_big,
-                               table));
+                               table, pc.min_direction && w.rev.rrange ? &w.rev : nullptr));
   if (!sh.any_edge) return GARY_OK;
   return launch_edge_lists(g, w.cached, w.counts, w.merge, w.counts + 1, cnt, st);
 }

@@ -35,7 +35,41 @@ constexpr int kDefaultBinSmall = 128;
 
 struct PivotCfg {
   int bin_small = kDefaultBinSmall, bin_hub = 2048, grid = 0, waves = 4;
+  bool min_direction = true;  // GARY_PIVOT_MIN_DIRECTION (reverse rows) or GARY_PIVOT_FORWARD
 };
+
+// Reverse rows of the pivot path. An arc (u, v) is streamed from the endpoint whose out-list is
+// shorter: N+(v) through the set of N+(u) at pivot u (forward), or, when d+(u) + kRevSlack <
+// d+(v), N+(u) through the set of N+(v) at pivot v (reverse). The reversed arcs are grouped by
+// v per call into rlist (the sources u); rrange[v] = (begin, end) of v's list there.
+// rrange == nullptr streams every arc forward. The grouping is a two-level counting sort
+// (tc_kernels.hip) over 2^lg buckets and `tiles` equal ranges of arcs; it needs m <= 2^24 and
+// nnz < 2^32.
+constexpr int32_t kRevSlack = 128;
+constexpr int kRevMaxBuckets = 4096;    // LDS counters per tile
+constexpr int kRevMaxBucketIds = 4096;  // LDS counters per bucket (ids per bucket)
+
+struct RevShape {
+  int32_t lg = 6;        // log2 of the bucket count
+  int32_t ids = 1;       // ids per bucket: ceil(m / 2^lg)
+  int32_t tiles = 1;     // arc ranges of the two tile passes
+  int64_t tile_arcs = 1; // arcs per tile (the last one may hold fewer)
+};
+
+struct RevBufs {
+  uint2 *rrange = nullptr;    // [m] (begin, end) in rlist
+  int32_t *rlist = nullptr;   // [nnz]
+  uint32_t *ghist = nullptr;  // [2^lg * tiles + 1]
+  uint32_t *part = nullptr;   // scan partials of ghist
+  uint64_t *pairs = nullptr;  // [nnz] (v >> lg, u) by bucket
+  RevShape sh;
+};
+
+// Whether reverse rows apply to a graph of m vertices and nnz arcs; the scratch bytes of RevBufs
+// (0 when they do not apply) and their carving from `p`.
+bool rev_supported(int32_t m, int64_t nnz);
+size_t rev_bytes(int32_t m, int64_t nnz);
+RevBufs rev_carve(void *p, int32_t m, int64_t nnz);
 
 struct RouteTable {
   uint8_t route[kRouteBins];
@@ -66,7 +100,8 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                       const int32_t *gid_of, unsigned long long *gcount, int32_t ngraphs,
                       unsigned long long *cnt, hipStream_t st, bool timed,
                       const uint8_t *route = nullptr, const float *pcost = nullptr,
-                      float cost_big = 0.f, const RouteTable *table = nullptr);
+                      float cost_big = 0.f, const RouteTable *table = nullptr,
+                      const RevBufs *rev = nullptr);
 
 // Exclusive scan of a[0..len) in place on `s` (values and total below 2^32); `part` holds
 // exclusive_scan_parts(len) values (graph_build.hip).

@@ -441,13 +441,19 @@ __device__ __forceinline__ uint32_t probe_chunk(const int4 (&data)[CH / 4], int3
 // blocks that hold colIdx elements. Chunk indices are 32-bit (colIdx below 2^34 elements).
 // (16-byte chunks with two windows in flight: 3.38 ms per R-MAT 20 count; 32-byte chunks with
 // a reset slot array and four shuffles: 3.21 ms; this form: 3.10 ms.)
-// The returned per-lane count is below 2^32: a lane probes at most one chunk (CH elements) per
-// window, and a call has at most (Σ_v (d(v) / CH + 2)) / 64 + d / 64 + 1 windows, so the count is
-// at most nnz / 64 + 3d / 8 + 8 < 2^28 + 2^30 + 8 for colIdx below 2^34 elements and d < 2^31.
+// The rows streamed are N+(v) for the row vertices L[0..d) (the pivot's out-list: forward rows,
+// skipped when d+(v) > dmax, since such an arc is reversed and streamed at v) and then L2[0..d2)
+// (the sources of the arcs reversed onto this pivot), rows r0, r0 + stride, ... of the
+// concatenation. The returned per-lane count is below 2^32: a lane probes at most one chunk (CH
+// elements) per window, and a call has at most (Σ_v (d(v) / CH + 2)) / 64 + (d + d2) / 64 + 1
+// windows, so the count is at most nnz / 64 + 3(d + d2) / 8 + 8 < 2^28 + 2^31 + 8 for colIdx
+// below 2^34 elements and d + d2 < 2^32.
 __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
                                                 const int32_t *__restrict__ L, int32_t d,
-                                                int32_t r0, int32_t stride, const int32_t *table,
-                                                const PivotSet &ps, RowSlots &rs, int lane,
+                                                const int32_t *__restrict__ L2, int32_t d2,
+                                                int32_t dmax, int32_t r0, int32_t stride,
+                                                const int32_t *table, const PivotSet &ps,
+                                                RowSlots &rs, int lane,
                                                 const uint8_t *__restrict__ Lroute) {
   constexpr int CH = kChunk;
   constexpr int LG = 3;
@@ -455,18 +461,23 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
   const uint64_t nnz = (uint64_t)g.csr.nnz;
   const int32_t *__restrict__ col = g.csr.colIdx;
   const bool hash = ps.log_t != 0;
+  const int32_t dtot = d + d2;
   uint32_t c = 0;
   int32_t tag = 1 << 30;  // forces a slot reset before the first window
-  for (int32_t b = r0; b < d; b += stride * kWave) {
+  for (int32_t b = r0; b < dtot; b += stride * kWave) {
     const int32_t r = b + lane * stride;
     int32_t dv = 0;
     int64_t pv = 0;
-    if (r < d && (!Lroute || Lroute[r] == kRouteBitmap)) {
-      const int32_t v = L[r];
-      if ((uint32_t)v < m) {
-        dv = g.csr.deg[v];
-        pv = g.csr.indPtr[v];
-      }
+    int32_t v = -1;
+    if (r < d) {
+      if (!Lroute || Lroute[r] == kRouteBitmap) v = L[r];
+    } else if (r < dtot) {
+      v = L2[r - d];
+    }
+    if ((uint32_t)v < m) {
+      dv = g.csr.deg[v];
+      pv = g.csr.indPtr[v];
+      if (r < d && dv > dmax) dv = 0;  // reversed: counted at pivot v
     }
     const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + CH - 1) >> LG) - (pv >> LG)) : 0;
     const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
@@ -513,20 +524,24 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
   return c;
 }
 
-// Edge-by-edge fallback for a pivot whose set does not fit in LDS: rows r0, r0+stride, ...
-// A lane's count per row is at most ceil(d(v) / 64), so the call's is below nnz / 64 + d < 2^32.
+// Edge-by-edge fallback for a pivot whose set does not fit in LDS: rows r0, r0+stride, ... of
+// the same row lists as stream_rows. A lane's count per row is at most ceil(min(d, d(v)) / 64)
+// + 1, so the call's is below nnz / 64 + d + d2 < 2^32 for d + d2 < 2^31.
 __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
                                                    const int32_t *__restrict__ L, int32_t d,
-                                                   int32_t r0, int32_t stride, int lane,
-                                                   const uint8_t *__restrict__ Lroute) {
+                                                   const int32_t *__restrict__ L2, int32_t d2,
+                                                   int32_t dmax, int32_t r0, int32_t stride,
+                                                   int lane, const uint8_t *__restrict__ Lroute) {
   const uint32_t m = (uint32_t)g.csr.m;
   uint32_t c = 0;
-  for (int32_t k = r0; k < d; k += stride) {
-    if (Lroute && Lroute[k] != kRouteBitmap) continue;
-    const int32_t v = L[k];
+  for (int32_t k = r0; k < d + d2; k += stride) {
+    if (k < d && Lroute && Lroute[k] != kRouteBitmap) continue;
+    const int32_t v = k < d ? L[k] : L2[k - d];
     if ((uint32_t)v >= m) continue;
-    c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + g.csr.indPtr[v], g.csr.deg[v],
-                                        nullptr, 0, lane);
+    const int32_t dv = g.csr.deg[v];
+    if (k < d && dv > dmax) continue;  // reversed: counted at pivot v
+    c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + g.csr.indPtr[v], dv, nullptr, 0,
+                                        lane);
   }
   return c;
 }
@@ -546,6 +561,21 @@ constexpr int kSmallBatchLight = 16;
 constexpr int kLightLog = 4;
 static_assert(kLightLog >= 1 && kLightLog <= 15, "light-pivot threshold");
 
+// Reverse rows (rrange != nullptr): pivot u also streams N+(x) for the x in rlist[rrange[u].x ..
+// rrange[u].y), the sources of the arcs (x, u) with d+(x) + kRevSlack < d+(u), and
+// skips its own out-neighbours v with d+(v) > d+(u) + kRevSlack, whose arcs are streamed at v.
+__device__ __forceinline__ int32_t rev_rows(const uint2 *__restrict__ rrange,
+                                            const int32_t *__restrict__ rlist, int32_t u,
+                                            const int32_t *&L2) {
+  if (!rrange) {
+    L2 = nullptr;
+    return 0;
+  }
+  const uint2 r = rrange[u];
+  L2 = rlist + r.x;
+  return (int32_t)(r.y - r.x);
+}
+
 template <int WAVES, int TABLE_WORDS>
 __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8)))
     tc_pivot_kernel(gary_device_graph g, const int32_t *__restrict__ small,
@@ -553,7 +583,8 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     int32_t *__restrict__ cursors, unsigned long long *count,
                     const int32_t *__restrict__ gid_of, unsigned long long *gcount,
                     const uint8_t *__restrict__ route, const RouteTable *__restrict__ rtable,
-                    const int32_t *__restrict__ split) {
+                    const int32_t *__restrict__ split, const uint2 *__restrict__ rrange,
+                    const int32_t *__restrict__ rlist) {
   __shared__ int32_t table[TABLE_WORDS];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   __shared__ RowSlots s_rows[WAVES];
@@ -577,13 +608,17 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     const int32_t d = (uint32_t)u < m ? g.csr.deg[u] : 0;
     if (d >= 2) {
       const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
+      const int32_t *L2;
+      const int32_t d2 = rev_rows(rrange, rlist, u, L2);
+      const int32_t dmax = rrange ? d + kRevSlack : INT32_MAX;
       PivotSet s;
       if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
         pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
-        c += stream_rows(g, L, d, wib, WAVES, table, s, rb, lane,
-                                  route ? route + g.csr.indPtr[u] : nullptr);
+        c += stream_rows(g, L, d, L2, d2, dmax, wib, WAVES, table, s, rb, lane,
+                         route ? route + g.csr.indPtr[u] : nullptr);
       } else {
-        c += pivot_fallback(g, L, d, wib, WAVES, lane, route ? route + g.csr.indPtr[u] : nullptr);
+        c += pivot_fallback(g, L, d, L2, d2, dmax, wib, WAVES, lane,
+                            route ? route + g.csr.indPtr[u] : nullptr);
       }
       if (gid_of) {
         wave_atomic_add(&gcount[gid_of[u]], c);
@@ -622,14 +657,18 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
         }
       }
       const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
+      const int32_t *L2;
+      const int32_t d2 = rev_rows(rrange, rlist, u, L2);
+      const int32_t dmax = rrange ? d + kRevSlack : INT32_MAX;
       PivotSet s;
       if (!pivot_plan(L[0], L[d - 1], d, TABLE_WORDS / WAVES, s)) {
-        c += pivot_fallback(g, L, d, 0, 1, lane, route ? route + g.csr.indPtr[u] : nullptr);
+        c += pivot_fallback(g, L, d, L2, d2, dmax, 0, 1, lane,
+                            route ? route + g.csr.indPtr[u] : nullptr);
         continue;
       }
       pivot_build<false>(wt, s, L, d, lane, kWave);
-      c += stream_rows(g, L, d, 0, 1, wt, s, rb, lane,
-                                route ? route + g.csr.indPtr[u] : nullptr);
+      c += stream_rows(g, L, d, L2, d2, dmax, 0, 1, wt, s, rb, lane,
+                       route ? route + g.csr.indPtr[u] : nullptr);
       wave_sync();  // the set is rebuilt for the next pivot
     }
   }
@@ -661,6 +700,7 @@ __global__ void fill_zero_kernel(uint32_t *__restrict__ p, int64_t n) {
 // (weight = out-degree, or the predicted cost in cost mode), so the persistent kernels take the
 // long pivots early and the tail of the launch is made of short ones.
 constexpr int kBuckets = 16;
+constexpr int32_t kRevRowsBig = 256;  // reverse-row mode: pivots with more rows get a workgroup
 constexpr int kKeys = 3 * kBuckets;
 // counter words: [0] #small, [1] #big, [2] always 0, [3] #hub, [4..5] main cursors,
 // [6..7] hub cursors, [8, 8+kKeys) histogram, [8+kKeys, 8+2*kKeys) scatter cursors
@@ -681,18 +721,33 @@ static_assert(kSplitOff + 4 <= kPivotCounterWords, "pivot counter words");
 __device__ __forceinline__ int pivot_key(const int32_t *__restrict__ deg, int64_t v, int64_t end,
                                          int32_t t_small, int32_t t_big, uint32_t rowmask,
                                          const float *__restrict__ pcost, float cost_big,
-                                         bool ordered) {
+                                         bool ordered, const uint2 *__restrict__ rrange) {
   if (v >= end) return -1;
   const int32_t d = deg[v];
   if (d < 2) return -1;
-  if (!((rowmask >> min(15, 31 - __clz(d))) & 1u)) return -1;
   const int lg = min(15, 31 - __clz(d));
+  // rows reversed onto v (their bins were checked against the route table when listed)
+  int32_t rr = 0;
+  if (rrange) {
+    const uint2 r = rrange[v];
+    rr = (int32_t)(r.y - r.x);
+  }
+  if (rr == 0 && !((rowmask >> lg) & 1u)) return -1;
   if (d > t_big) return 2 * kBuckets + (15 - lg);
   if (pcost && cost_big > 0.f) {
     const float cst = pcost[v];
-    if (!(cst > 0.f)) return -1;
-    const int lc = min(15, max(0, (int)log2f(cst)));
-    return cst > cost_big ? kBuckets + (15 - lc) : (ordered ? 15 - lc : 15);
+    if (!(cst > 0.f) && rr == 0) return -1;
+    const int lc = min(15, max(0, (int)log2f(fmaxf(cst, 1.f))));
+    return cst > cost_big || (rrange && d + rr > kRevRowsBig) ? kBuckets + (15 - lc)
+                                                  : (ordered ? 15 - lc : 15);
+  }
+  if (rrange) {
+    // weight: rows (at most d forward and rr reversed) times the set size, each row being at
+    // most d + kRevSlack long; the pivot gets a workgroup for a large set or many rows
+    const uint64_t w = (uint64_t)(d + rr) * (uint64_t)d;
+    const int lw = min(15, (63 - __clzll(w)) >> 1);
+    const bool big = d > t_small || d + rr > kRevRowsBig;
+    return big ? kBuckets + (15 - lw) : (ordered ? 15 - lw : 15);
   }
   return d <= t_small ? (ordered ? 15 - lg : 15) : kBuckets + (15 - lg);
 }
@@ -700,7 +755,8 @@ __device__ __forceinline__ int pivot_key(const int32_t *__restrict__ deg, int64_
 __global__ void __launch_bounds__(256)
     bin_hist_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small, int32_t t_big,
                     int32_t *__restrict__ counters, const RouteTable *__restrict__ table,
-                    const float *__restrict__ pcost, float cost_big, bool ordered) {
+                    const float *__restrict__ pcost, float cost_big, bool ordered,
+                    const uint2 *__restrict__ rrange) {
   const uint32_t rowmask = table ? table->rowmask : 0xffffffffu;
   __shared__ int32_t s_hist[kKeys];
   const int lane = lane_id();
@@ -710,7 +766,8 @@ __global__ void __launch_bounds__(256)
   const int64_t beg = (int64_t)blockIdx.x * chunk, end = min((int64_t)m, beg + chunk);
   for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
     const int key =
-        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered);
+        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered,
+                  rrange);
     for_each_key(key, [&](int k, uint64_t mask, int leader) {
       if (lane == leader) atomicAdd(&s_hist[k], __popcll(mask));
     });
@@ -725,7 +782,8 @@ __global__ void __launch_bounds__(256)
     bin_scatter_kernel(const int32_t *__restrict__ deg, int32_t m, int32_t t_small,
                        int32_t t_big, int32_t *__restrict__ lists, int64_t stride,
                        int32_t *__restrict__ counters, const RouteTable *__restrict__ table,
-                       const float *__restrict__ pcost, float cost_big, bool ordered) {
+                       const float *__restrict__ pcost, float cost_big, bool ordered,
+                       const uint2 *__restrict__ rrange) {
   const uint32_t rowmask = table ? table->rowmask : 0xffffffffu;
   __shared__ int32_t s_cnt[kKeys], s_base[kKeys], s_start[kKeys], s_tot[kKeys], s_pre[kKeys];
   const int lane = lane_id();
@@ -759,7 +817,8 @@ __global__ void __launch_bounds__(256)
   const int64_t beg = (int64_t)blockIdx.x * chunk, end = min((int64_t)m, beg + chunk);
   for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
     const int key =
-        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered);
+        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered,
+                  rrange);
     for_each_key(key, [&](int k, uint64_t mask, int leader) {
       if (lane == leader) atomicAdd(&s_cnt[k], __popcll(mask));
     });
@@ -773,7 +832,8 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   for (int64_t base = beg + (threadIdx.x & ~63); base < end; base += blockDim.x) {
     const int key =
-        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered);
+        pivot_key(deg, base + lane, end, t_small, t_big, rowmask, pcost, cost_big, ordered,
+                  rrange);
     for_each_key(key, [&](int k, uint64_t mask, int leader) {
       int32_t pos0 = 0;
       if (lane == leader) pos0 = atomicAdd(&s_cnt[k], __popcll(mask));
@@ -781,6 +841,171 @@ __global__ void __launch_bounds__(256)
       if (key == k)
         lists[(k / kBuckets) * stride + s_base[k] + pos0 + lanes_below(mask)] = (int32_t)(base + lane);
     });
+  }
+}
+
+// ======================================================================================
+// Reverse rows: the arcs streamed at their target
+// ======================================================================================
+//
+// Arc (u, v) is reversed when d+(u) + kRevSlack < d+(v): its intersection is then taken by
+// streaming N+(u) through the set of N+(v) at pivot v, so each arc streams the shorter of the
+// two out-lists (on R-MAT 20: 1.97 G probes and 260 M 32-byte chunks instead of 4.43 G and
+// 567 M). Only arcs the route table sends to the bitmap are listed (all arcs when unrouted).
+//
+// The reversed arcs are grouped by target with a two-level counting sort whose counters all live
+// in LDS (device-scope atomics run at about 6.5 G/s here: one global fetch-add per arc and pass
+// took 3.8 ms for the 11.1 M reversed arcs of R-MAT 20):
+//   1. rev_hist_kernel: tile t (an equal range of the COO's arcs) counts its reversed arcs per
+//      bucket into ghist[bucket * tiles + t];
+//   2. an exclusive scan of ghist gives every (bucket, tile) its first position;
+//   3. rev_place_kernel: the tile walks its arcs again and writes (v >> lg, u) pairs at its
+//      positions (ranks from LDS fetch-adds);
+//   4. rev_group_kernel: one workgroup per bucket counts its pairs per target in LDS, scans the
+//      counts, writes rrange[v] for every v of the bucket and places each u.
+// Bucket of v: the low lg bits of v ^ (v >> lg) ^ (v >> 2 lg). Given the bucket and h = v >> lg,
+// v's low bits are bucket ^ h ^ (h >> lg), so a bucket holds at most ceil(m / 2^lg) ids, indexed
+// densely by h; and ids with few set bits (R-MAT's hubs) spread over the buckets, where buckets
+// by id range put 688 K of the 11.1 M reversed arcs of R-MAT 20 in one bucket.
+// The tile passes read each arc's source from the COO rowIdx, so the grouping needs the COO
+// (without it the pivot path streams every arc forward). Walking 64-row batches of the CSR
+// instead, with rows found by a slot write and a max-scan per 64 arcs, took 174 + 238 us for
+// the two tile passes on R-MAT 20: two dependent loads per step and one step in flight.
+
+constexpr int kRevBlock = 512;
+constexpr int kRevArcs = 4;  // arcs a thread of the tile passes loads before using them
+constexpr int kRevGroupBlock = 1024;
+constexpr int kRevUnroll = 4;  // pairs a thread of rev_group_kernel loads before using them
+
+__device__ __forceinline__ uint32_t rev_bucket(uint32_t v, int lg) {
+  return (v ^ (v >> lg) ^ (v >> (2 * lg))) & ((1u << lg) - 1u);
+}
+
+// Calls f(v, u) for every reversed arc (u, v) of tile blockIdx.x (arcs [t * tile_arcs,
+// (t + 1) * tile_arcs) of the COO), each thread loading kRevArcs arcs and their degrees before
+// testing them.
+template <class F>
+__device__ __forceinline__ void walk_reversed(const gary_device_graph &g,
+                                              const uint8_t *__restrict__ route,
+                                              const RevShape &sh, F &&f) {
+  const int32_t *__restrict__ row = g.coo.rowIdx;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  const int32_t *__restrict__ deg = g.csr.deg;
+  const uint32_t m = (uint32_t)g.csr.m;
+  const int64_t e_beg = (int64_t)blockIdx.x * sh.tile_arcs;
+  const int64_t e_end = min(g.csr.nnz, e_beg + sh.tile_arcs);
+  for (int64_t e0 = e_beg + threadIdx.x; e0 < e_end; e0 += kRevArcs * kRevBlock) {
+    int32_t u[kRevArcs], v[kRevArcs], du[kRevArcs], dv[kRevArcs];
+#pragma unroll
+    for (int q = 0; q < kRevArcs; ++q) {
+      const int64_t e = e0 + q * kRevBlock;
+      const bool in = e < e_end && (!route || route[e] == kRouteBitmap);
+      u[q] = in ? row[e] : -1;
+      v[q] = in ? col[e] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < kRevArcs; ++q) {
+      const bool ok = (uint32_t)u[q] < m && (uint32_t)v[q] < m;
+      du[q] = ok ? deg[u[q]] : 0;
+      dv[q] = ok ? deg[v[q]] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kRevArcs; ++q)
+      if (du[q] + kRevSlack < dv[q]) f(v[q], u[q]);
+  }
+}
+
+__global__ void __launch_bounds__(kRevBlock)
+    rev_hist_kernel(gary_device_graph g, const uint8_t *__restrict__ route,
+                    const RouteTable *__restrict__ rtable, RevShape sh,
+                    uint32_t *__restrict__ ghist) {
+  __shared__ uint32_t s_hist[kRevMaxBuckets];
+  if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
+  const int nb = 1 << sh.lg;
+  for (int i = threadIdx.x; i < nb; i += kRevBlock) s_hist[i] = 0;
+  __syncthreads();
+  walk_reversed(g, route, sh, [&](int32_t v, int32_t) {
+    atomicAdd(&s_hist[rev_bucket((uint32_t)v, sh.lg)], 1u);
+  });
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += kRevBlock)
+    ghist[(int64_t)i * sh.tiles + blockIdx.x] = s_hist[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) ghist[(int64_t)nb * sh.tiles] = 0u;
+}
+
+__global__ void __launch_bounds__(kRevBlock)
+    rev_place_kernel(gary_device_graph g, const uint8_t *__restrict__ route,
+                     const RouteTable *__restrict__ rtable, RevShape sh,
+                     const uint32_t *__restrict__ ghist, uint64_t *__restrict__ pairs) {
+  __shared__ uint32_t s_cur[kRevMaxBuckets];
+  if (rtable && !rtable->mixed) route = nullptr;
+  const int nb = 1 << sh.lg;
+  for (int i = threadIdx.x; i < nb; i += kRevBlock)
+    s_cur[i] = ghist[(int64_t)i * sh.tiles + blockIdx.x];
+  __syncthreads();
+  walk_reversed(g, route, sh, [&](int32_t v, int32_t u) {
+    const uint32_t pos = atomicAdd(&s_cur[rev_bucket((uint32_t)v, sh.lg)], 1u);
+    pairs[pos] = ((uint64_t)((uint32_t)v >> sh.lg) << 32) | (uint32_t)u;
+  });
+}
+
+__global__ void __launch_bounds__(kRevGroupBlock)
+    rev_group_kernel(RevShape sh, int32_t m, const uint32_t *__restrict__ ghist,
+                     const uint64_t *__restrict__ pairs, uint2 *__restrict__ rrange,
+                     int32_t *__restrict__ rlist) {
+  constexpr int kW = kRevGroupBlock / kWave;
+  __shared__ uint32_t s_cnt[kRevMaxBucketIds];
+  __shared__ uint32_t s_wave[kW];
+  const uint32_t bk = blockIdx.x;
+  const uint32_t beg = ghist[(int64_t)bk * sh.tiles];
+  const uint32_t end = ghist[(int64_t)(bk + 1) * sh.tiles];  // the last bucket reads the total
+  for (int i = threadIdx.x; i < sh.ids; i += kRevGroupBlock) s_cnt[i] = 0;
+  __syncthreads();
+  for (uint32_t i0 = beg + threadIdx.x; i0 < end; i0 += kRevUnroll * kRevGroupBlock) {
+    uint32_t h[kRevUnroll];
+#pragma unroll
+    for (int q = 0; q < kRevUnroll; ++q) {
+      const uint32_t i = i0 + q * kRevGroupBlock;
+      h[q] = i < end ? (uint32_t)(pairs[i] >> 32) : 0xffffffffu;
+    }
+#pragma unroll
+    for (int q = 0; q < kRevUnroll; ++q)
+      if (h[q] != 0xffffffffu) atomicAdd(&s_cnt[h[q]], 1u);
+  }
+  __syncthreads();
+  // exclusive scan of the sh.ids counters: each thread owns `per` consecutive ones
+  const int per = (sh.ids + kRevGroupBlock - 1) / kRevGroupBlock;
+  const int i0 = threadIdx.x * per;
+  uint32_t sum = 0;
+  for (int q = 0; q < per; ++q)
+    if (i0 + q < sh.ids) sum += s_cnt[i0 + q];
+  const uint32_t incl = wave_inclusive_scan_u32(sum);
+  if (lane_id() == kWave - 1) s_wave[threadIdx.x / kWave] = incl;
+  __syncthreads();
+  uint32_t run = beg + incl - sum;
+  for (int q = 0; q < (int)(threadIdx.x / kWave); ++q) run += s_wave[q];
+  const uint32_t lmask = (1u << sh.lg) - 1u;
+  for (int q = 0; q < per; ++q) {
+    const int hq = i0 + q;
+    if (hq >= sh.ids) break;
+    const uint32_t c = s_cnt[hq];
+    s_cnt[hq] = run;  // the list start of this id, then its fill cursor
+    const uint32_t lo = (bk ^ (uint32_t)hq ^ ((uint32_t)hq >> sh.lg)) & lmask;
+    const int64_t v = ((int64_t)hq << sh.lg) | lo;
+    if (v < m) rrange[v] = make_uint2(run, run + c);
+    run += c;
+  }
+  __syncthreads();
+  for (uint32_t i0 = beg + threadIdx.x; i0 < end; i0 += kRevUnroll * kRevGroupBlock) {
+    uint64_t pr[kRevUnroll];
+#pragma unroll
+    for (int q = 0; q < kRevUnroll; ++q) {
+      const uint32_t i = i0 + q * kRevGroupBlock;
+      pr[q] = i < end ? pairs[i] : ~0ull;
+    }
+#pragma unroll
+    for (int q = 0; q < kRevUnroll; ++q)
+      if (pr[q] != ~0ull) rlist[atomicAdd(&s_cnt[(uint32_t)(pr[q] >> 32)], 1u)] = (int32_t)(uint32_t)pr[q];
   }
 }
 
@@ -1037,7 +1262,7 @@ namespace {
 
 struct TcCfg {
   int block = 256, grid = 0, group = 64, cache_depth = 64, partition = GARY_PARTITION_INTERLEAVED;
-  int bin_small = kDefaultBinSmall, bin_hub = 2048;
+  int bin_small = kDefaultBinSmall, bin_hub = 2048, direction = GARY_PIVOT_MIN_DIRECTION;
 };
 
 constexpr int kPivotWords = 4096;  // 16 KiB: a big pivot's set, or one small-pivot set per wave
@@ -1054,6 +1279,7 @@ PivotCfg pivot_cfg(const TcCfg &c) {
   p.bin_hub = c.bin_hub;
   p.grid = c.grid;
   p.waves = c.block == 128 ? 2 : c.block == 512 ? 8 : 4;
+  p.min_direction = c.direction == GARY_PIVOT_MIN_DIRECTION;
   return p;
 }
 
@@ -1066,6 +1292,7 @@ int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
     c.partition = in->partition;
     if (in->bin_small) c.bin_small = in->bin_small;
     if (in->bin_hub) c.bin_hub = in->bin_hub;
+    c.direction = in->direction;
   }
   GARY_CHECK_ARG(c.block >= 64 && c.block <= 512 && c.block % 64 == 0,
                  "launch cfg: block=%d must be a multiple of 64 in [64, 512]", c.block);
@@ -1078,6 +1305,8 @@ int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
                      c.partition == GARY_PARTITION_DYNAMIC,
                  "launch cfg: bad partition %d", c.partition);
   GARY_CHECK_ARG(c.grid >= 0, "launch cfg: grid=%d", c.grid);
+  GARY_CHECK_ARG(c.direction == GARY_PIVOT_MIN_DIRECTION || c.direction == GARY_PIVOT_FORWARD,
+                 "launch cfg: bad pivot direction %d", c.direction);
   GARY_CHECK_ARG(c.bin_small >= 2 && c.bin_hub >= c.bin_small && c.bin_hub <= kPivotWords / 2,
                  "launch cfg: bins small=%d hub=%d out of range", c.bin_small, c.bin_hub);
   const size_t lds = (size_t)(c.block / c.group) * c.cache_depth * sizeof(int32_t);
@@ -1143,16 +1372,66 @@ int launch_edge(const gary_device_graph &g, const TcCfg &c, unsigned long long *
 
 }  // namespace
 
+bool rev_supported(int32_t m, int64_t nnz) {
+  return m >= 1 && m <= (1 << 24) && nnz >= 1 && nnz < (1ll << 32);
+}
+
+namespace {
+
+RevShape rev_shape(int32_t m, int64_t nnz) {
+  RevShape r;
+  int lg = 1;
+  while ((1ll << lg) < (int64_t)m) ++lg;
+  r.lg = std::min(12, std::max(6, lg - 9));
+  r.ids = (int32_t)(((int64_t)m + (1ll << r.lg) - 1) >> r.lg);
+  r.tiles = (int32_t)std::max<int64_t>(1, std::min<int64_t>(1024, nnz / 16384));
+  r.tile_arcs = (nnz + r.tiles - 1) / r.tiles;
+  return r;
+}
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t rev_bytes(int32_t m, int64_t nnz) {
+  if (!rev_supported(m, nnz)) return 0;
+  const RevShape sh = rev_shape(m, nnz);
+  const uint32_t hist = (uint32_t)(1u << sh.lg) * (uint32_t)sh.tiles + 1u;
+  return al256(sizeof(uint2) * (size_t)m) + al256(sizeof(uint32_t) * hist) +
+         al256(sizeof(uint32_t) * exclusive_scan_parts(hist)) +
+         al256(sizeof(uint64_t) * (size_t)nnz) + al256(sizeof(int32_t) * (size_t)nnz);
+}
+
+RevBufs rev_carve(void *p, int32_t m, int64_t nnz) {
+  RevBufs r;
+  if (!rev_supported(m, nnz)) return r;
+  r.sh = rev_shape(m, nnz);
+  const uint32_t hist = (uint32_t)(1u << r.sh.lg) * (uint32_t)r.sh.tiles + 1u;
+  char *c = (char *)p;
+  r.rrange = (uint2 *)c;
+  c += al256(sizeof(uint2) * (size_t)m);
+  r.ghist = (uint32_t *)c;
+  c += al256(sizeof(uint32_t) * hist);
+  r.part = (uint32_t *)c;
+  c += al256(sizeof(uint32_t) * exclusive_scan_parts(hist));
+  r.pairs = (uint64_t *)c;
+  c += al256(sizeof(uint64_t) * (size_t)nnz);
+  r.rlist = (int32_t *)c;
+  return r;
+}
+
 // Pivot path: bin pivots by out-degree on the device, then the hub launch (d > bin_hub, 8-wave
 // workgroups with 32 KiB sets) and the main persistent launch (big pivots per workgroup, small
 // pivots per wave). `lists` holds 3 * stride int32, `counters` 8 int32 (zeroed here by a kernel,
-// so the sequence can be captured in a graph). No host round trip, no allocation.
+// so the sequence can be captured in a graph). With `rev`, the reversed arcs are grouped by
+// target first (the counting sort above: two passes over the arcs, a scan, one pass per bucket)
+// and pivots are binned by their rows as well. No host round trip, no allocation.
 int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                       int32_t *lists, size_t stride, int32_t *counters,
                       const int32_t *gid_of, unsigned long long *gcount, int32_t ngraphs,
                       unsigned long long *cnt, hipStream_t st, bool timed,
                       const uint8_t *route, const float *pcost, float cost_big,
-                      const RouteTable *table) {
+                      const RouteTable *table, const RevBufs *rev) {
   const int32_t m = g.csr.m;
   if (m == 0 || g.csr.nnz == 0) return GARY_OK;
   GARY_CHECK_ARG(((uintptr_t)g.csr.colIdx & 15) == 0,
@@ -1175,33 +1454,55 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                      pc.bin_hub >= pc.bin_small && pc.bin_hub <= kPivotWords / 2,
                  "pivot path: bins small=%d hub=%d out of range for %d waves", pc.bin_small,
                  pc.bin_hub, pc.waves);
+  const uint2 *rrange = nullptr;
+  const int32_t *rlist = nullptr;
+  if (rev && rev->rrange) {
+    const RevShape &sh = rev->sh;
+    GARY_CHECK_ARG(g.coo.rowIdx && g.coo.nnz == g.csr.nnz,
+                   "reverse rows need the COO rowIdx (call populate first)");
+    GARY_CHECK_ARG(rev_supported(m, g.csr.nnz) && (1 << sh.lg) <= kRevMaxBuckets &&
+                       sh.ids <= kRevMaxBucketIds && (int64_t)sh.tiles * sh.tile_arcs >= g.csr.nnz,
+                   "reverse rows: m=%d nnz=%lld out of range", m, (long long)g.csr.nnz);
+    const uint32_t hist = (uint32_t)(1u << sh.lg) * (uint32_t)sh.tiles + 1u;
+    rev_hist_kernel<<<sh.tiles, kRevBlock, 0, st>>>(g, route, table, sh, rev->ghist);
+    GARY_HIP(hipGetLastError());
+    GARY_TRY(exclusive_scan_u32(rev->ghist, hist, rev->part, st));
+    rev_place_kernel<<<sh.tiles, kRevBlock, 0, st>>>(g, route, table, sh, rev->ghist,
+                                                     rev->pairs);
+    GARY_HIP(hipGetLastError());
+    rev_group_kernel<<<1 << sh.lg, kRevGroupBlock, 0, st>>>(sh, m, rev->ghist, rev->pairs,
+                                                            rev->rrange, rev->rlist);
+    GARY_HIP(hipGetLastError());
+    rrange = rev->rrange;
+    rlist = rev->rlist;
+  }
   const bool ordered = gid_of == nullptr;
   bin_hist_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, counters, table,
-                                         pcost, cost_big, ordered);
+                                         pcost, cost_big, ordered, rrange);
   GARY_HIP(hipGetLastError());
   bin_scatter_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, lists,
                                             (int64_t)stride, counters, table, pcost, cost_big,
-                                            ordered);
+                                            ordered, rrange);
   GARY_HIP(hipGetLastError());
   // hubs: small list empty (counters[2] == 0), big list = bin 2
   tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
-      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of, gcount, route, table,
-      counters + kSplitOff + 2);
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of,
+      gcount, route, table, counters + kSplitOff + 2, rrange, rlist);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_begin(st));
   const int grid = pc.grid ? pc.grid : 2048;
   if (pc.waves == 2)
     tc_pivot_kernel<2, kPivotWords><<<grid, 2 * 64, 0, st>>>(
-        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table,
-        counters + kSplitOff);
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
+        table, counters + kSplitOff, rrange, rlist);
   else if (pc.waves == 8)
     tc_pivot_kernel<8, kPivotWords><<<grid, 8 * 64, 0, st>>>(
-        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table,
-        counters + kSplitOff);
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
+        table, counters + kSplitOff, rrange, rlist);
   else
     tc_pivot_kernel<4, kPivotWords><<<grid, 4 * 64, 0, st>>>(
-        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route, table,
-        counters + kSplitOff);
+        g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
+        table, counters + kSplitOff, rrange, rlist);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_end(st));
   return GARY_OK;
@@ -1235,10 +1536,17 @@ int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long 
   GARY_TRY(current_device(&dev));
   void *ws = nullptr;
   const size_t stride = ((size_t)m + 63) & ~(size_t)63;
-  GARY_TRY(workspace_get(dev, st, sizeof(int32_t) * (3 * stride + kPivotCounterWords), &ws));
+  const size_t core = sizeof(int32_t) * (3 * stride + kPivotCounterWords);
+  // the reversed arcs are listed from the COO: without its rowIdx every arc streams forward
+  const bool rev = c.direction == GARY_PIVOT_MIN_DIRECTION && rev_supported(m, g.csr.nnz) &&
+                   g.coo.rowIdx != nullptr && g.coo.nnz == g.csr.nnz;
+  // (the reversed-arc list lives after the pivot lists in the same per-stream buffer)
+  GARY_TRY(workspace_get(dev, st, core + (rev ? rev_bytes(m, g.csr.nnz) : 0), &ws));
   int32_t *lists = (int32_t *)ws;
+  RevBufs rb;
+  if (rev) rb = rev_carve((char *)ws + core, m, g.csr.nnz);
   return launch_pivot_core(g, pivot_cfg(c), lists, stride, lists + 3 * stride, nullptr, nullptr,
-                           0, cnt, st, true);
+                           0, cnt, st, true, nullptr, nullptr, 0.f, nullptr, rev ? &rb : nullptr);
 }
 
 }  // namespace
@@ -1263,10 +1571,11 @@ int launch_tri_list(const gary_device_graph &g, unsigned long long *total, int32
   GARY_HIP(hipGetLastError());
   const int bgrid = (int)std::min<int64_t>(((int64_t)m + 1023) / 1024, 4096);
   bin_hist_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, counters,
-                                         nullptr, nullptr, 0.f, true);
+                                         nullptr, nullptr, 0.f, true, nullptr);
   GARY_HIP(hipGetLastError());
   bin_scatter_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, lists,
-                                            (int64_t)stride, counters, nullptr, nullptr, 0.f, true);
+                                            (int64_t)stride, counters, nullptr, nullptr, 0.f, true,
+                                            nullptr);
   GARY_HIP(hipGetLastError());
   // per-wave scratch chunks (the hub launch's waves reuse the main launch's first chunks; the
   // launches run in stream order). A chunk holds about 1/16384 of the output, 256 to 2,048
@@ -1281,7 +1590,8 @@ int launch_tri_list(const gary_device_graph &g, unsigned long long *total, int32
                          &scratch));
   const ListOut lo{total, out, cap, (int32_t *)scratch, chunk};
   tri_list_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
-      g, lists, lists + 2 * stride, counters + 2, counters + 6, lo, counters + kSplitOff + 2);
+      g, lists, lists + 2 * stride, counters + 2, counters + 6, lo,
+      counters + kSplitOff + 2);
   GARY_HIP(hipGetLastError());
   tri_list_pivot_kernel<4, kPivotWords><<<kListGrid, 4 * 64, 0, st>>>(
       g, lists, lists + stride, counters, counters + 4, lo, counters + kSplitOff);

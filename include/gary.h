@@ -262,6 +262,13 @@ enum {
   GARY_PARTITION_CHUNKED = 1      /* WorkPartition::CHUNKED */
 };
 
+/* Arc direction of the pivot strategies (BITMAP and the bitmap route of HEURISTIC /
+ * AUTO_LEARNED). MIN_DIRECTION streams each arc (u, v) from the endpoint with the shorter
+ * out-list: N+(u) through the set of N+(v) at pivot v when d+(u) is shorter by more than a small
+ * slack, which needs a per-call list of those arcs grouped by v (about 4 bytes per arc of
+ * scratch, nnz < 2^32). FORWARD streams every N+(v) at pivot u, with no such list. */
+enum { GARY_PIVOT_MIN_DIRECTION = 0, GARY_PIVOT_FORWARD = 1 };
+
 /* Launch knobs. Zero fields take the library default (or the autotuned value). */
 typedef struct gary_launch_cfg {
   int32_t block;       /* threads per workgroup of the edge-centric kernels (64, 128, 192, 256) */
@@ -271,7 +278,7 @@ typedef struct gary_launch_cfg {
   int32_t partition;   /* GARY_PARTITION_* for the edge-centric strategies */
   int32_t bin_small;   /* pivots with out-degree <= bin_small use the small-table wave kernel */
   int32_t bin_hub;     /* pivots with out-degree > bin_hub get a whole workgroup */
-  int32_t reserved;
+  int32_t direction;   /* GARY_PIVOT_* for the pivot strategies (0 = MIN_DIRECTION) */
 } gary_launch_cfg;
 
 /* Elements (not bytes) of LDS index cache for a workgroup of `block` threads with depth

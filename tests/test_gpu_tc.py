@@ -218,6 +218,11 @@ def test_chung_lu_4m(chung_lu_4m, strategy):
     assert gq.triangleCountHom(ctx, strategy) == want
 
 
+def test_chung_lu_4m_forward(chung_lu_4m):
+    ctx, want = chung_lu_4m
+    assert gq.triangleCountHom(ctx, gq.BITMAP, gq.launch_cfg(direction=gq.FORWARD)) == want
+
+
 def test_learned_table_is_the_models(rmat20):
     """AUTO_LEARNED reads the loaded model: its table on R-MAT 20 is the fused-inference table,
     and a model that forbids the bitmap route yields a table without it."""
@@ -279,6 +284,79 @@ def test_fused_inference_matches_forward_kernel(er_ctx, seed):
         assert gq.triangleCountHom(ctx, gq.AUTO_LEARNED) == want
     finally:
         selector.load()
+
+
+def _reverse_graph():
+    """Id orientation (u -> v iff u > v). Three pivots receive many reversed arcs (sources of
+    out-degree 2 or 3 far below theirs): A (out-degree 5,000 over ids spread across 10^6: too
+    large for any LDS set, the edge-by-edge fallback), B (3,000: the hub kernel) and C (300: a
+    workgroup pivot). Each source also points at one or two out-neighbours of its pivot, so every
+    triangle (source, pivot, w) is counted through a reversed arc."""
+    src, dst = [], []
+    A, B, C = 1_000_000, 900_000, 800_000
+    outs = {A: [200 * i for i in range(5000)], B: [150 * i + 7 for i in range(3000)],
+            C: [1000 * i + 3 for i in range(300)]}
+    for p, ws in outs.items():
+        src += [p] * len(ws)
+        dst += ws
+    for j in range(400):  # into A: two out-neighbours each
+        u = 1_000_001 + j
+        src += [u, u, u]
+        dst += [A, 200 * ((7 * j) % 5000), 200 * ((7 * j + 1) % 5000)]
+    for j in range(300):  # into B
+        u = 1_050_000 + j
+        src += [u, u]
+        dst += [B, 150 * ((3 * j) % 3000) + 7]
+    for j in range(100):  # into C
+        u = 1_060_000 + j
+        src += [u, u]
+        dst += [C, 1000 * ((11 * j) % 300) + 3]
+    return graph.csr_from_edges(1_100_000, np.array(src, np.int32), np.array(dst, np.int32),
+                                graph.ORIENT_ID)
+
+
+@pytest.mark.parametrize("direction", [gq.MIN_DIRECTION, gq.FORWARD])
+def test_reverse_rows_every_pivot_path(direction):
+    g = _reverse_graph()
+    want = oracle.tri_count(g.indPtr, g.colIdx)
+    assert want == 800 + 300 + 100
+    ctx = _ctx(g)
+    for s in (gq.BITMAP, gq.HEURISTIC, gq.AUTO_LEARNED):
+        for bins in ((0, 0), (512, 2048), (2, 2)):  # default; big pivots per wave; all per group
+            cfg = gq.launch_cfg(bin_small=bins[0], bin_hub=bins[1], direction=direction)
+            assert gq.triangleCountHom(ctx, s, cfg) == want, (s, bins)
+
+
+@pytest.mark.parametrize("direction", [gq.MIN_DIRECTION, gq.FORWARD])
+@pytest.mark.parametrize("strategy", [gq.BITMAP, gq.HEURISTIC, gq.AUTO_LEARNED])
+def test_pivot_directions(golden_ctxs, rmat16, er_ctx, direction, strategy):
+    """Both arc directions of the pivot path give the oracle's count: forward (every N+(v) at
+    pivot u) and min-direction (arcs into pivots with much longer out-lists streamed there)."""
+    cfg = gq.launch_cfg(direction=direction)
+    for rec, ctx in golden_ctxs:
+        assert gq.triangleCountHom(ctx, strategy, cfg) == rec["triangles"], rec["name"]
+    for _, ctx, want in (rmat16, er_ctx):
+        assert gq.triangleCountHom(ctx, strategy, cfg) == want
+        # per-wave pivots up to out-degree 512, which then receive reversed arcs too
+        big = gq.launch_cfg(bin_small=512, bin_hub=2048, direction=direction)
+        assert gq.triangleCountHom(ctx, strategy, big) == want
+
+
+def test_bitmap_without_coo(rmat16):
+    """Min-direction lists the reversed arcs from the COO; a graph whose COO was never populated
+    streams every arc forward and still counts exactly."""
+    g, _, want = rmat16
+    ctx = DeviceGraphCtx(g)  # no populateDevice(): rowIdx absent
+    assert gq.triangleCountHom(ctx, gq.BITMAP) == want
+    with pytest.raises(Exception):  # the routed strategies need the COO
+        gq.triangleCountHom(ctx, gq.HEURISTIC)
+
+
+def test_rmat20_full_forward(rmat20):
+    """The headline workload with every arc streamed forward (no reversed-arc list)."""
+    g, ctx, want = rmat20
+    for s in (gq.BITMAP, gq.AUTO_LEARNED):
+        assert gq.triangleCountHom(ctx, s, gq.launch_cfg(direction=gq.FORWARD)) == want
 
 
 def test_hub_fallback_star():
