@@ -554,10 +554,19 @@ __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
 // counts[0] = #small, counts[1] = #big; cursors[0..1] start at zero.
 // Per-graph mode (gid_of != nullptr, batched disjoint unions): each wave adds its partial count
 // to gcount[gid_of[u]] whenever the pivot's graph changes, instead of to *count.
-// The small list is ordered heaviest first; its first split[0] pivots (out-degree >=
-// 2^kLightLog) are claimed kSmallBatchHeavy at a time, the rest kSmallBatchLight at a time.
-constexpr int kSmallBatchHeavy = 4;
-constexpr int kSmallBatchLight = 16;
+// The small list is ordered heaviest first; its first split[0] pivots (weight bucket >=
+// kLightLog) are claimed H at a time, the rest Lc at a time (below).
+// Claim sizes (pivots per atomic on the small list's cursor), forward / with reversed rows. A
+// claim costs a fetch-add serialised with every other wave's on one address (about 10 ns each)
+// and one chain of header loads; with reversed rows the small pivots stream little (their arcs
+// to much longer out-lists are streamed at the other end), so larger claims pay: R-MAT 20
+// 2.24 -> 2.03 ms per launch at 8 / 24 against 4 / 16, while forward streaming loses balance
+// with them (2.90 -> 3.13 ms).
+constexpr int kSmallBatchHeavy = 4, kSmallBatchHeavyRev = 8;
+constexpr int kSmallBatchLight = 16, kSmallBatchLightRev = 24;
+static_assert(kSmallBatchHeavyRev <= kWave && kSmallBatchLightRev <= kWave &&
+                  kSmallBatchHeavy <= kWave && kSmallBatchLight <= kWave,
+              "one pivot header per lane");
 constexpr int kLightLog = 4;
 static_assert(kLightLog >= 1 && kLightLog <= 15, "light-pivot threshold");
 
@@ -631,44 +640,66 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
   int32_t *wt = table + wib * (TABLE_WORDS / WAVES);
   int32_t cur = -1;  // graph of the wave's partial count (per-graph mode)
   // claim k covers pivots [k*H, k*H + H) while k < ceil(n_heavy / H), then light claims of L
+  const int32_t H = rrange ? kSmallBatchHeavyRev : kSmallBatchHeavy;
+  const int32_t Lc = rrange ? kSmallBatchLightRev : kSmallBatchLight;
   const int32_t n_heavy = min(split[0], n_small);
-  const int32_t heavy_claims = (n_heavy + kSmallBatchHeavy - 1) / kSmallBatchHeavy;
+  const int32_t heavy_claims = (n_heavy + H - 1) / H;
   while (true) {
     int32_t k = 0;
     if (lane == 0) k = atomicAdd(&cursors[0], 1);
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
-    const int32_t base = k < heavy_claims
-                             ? k * kSmallBatchHeavy
-                             : n_heavy + (k - heavy_claims) * kSmallBatchLight;
+    const int32_t base = k < heavy_claims ? k * H : n_heavy + (k - heavy_claims) * Lc;
     if (base >= n_small) break;
-    const int32_t end = min(base + (k < heavy_claims ? kSmallBatchHeavy : kSmallBatchLight),
-                            k < heavy_claims ? n_heavy : n_small);
-    for (int32_t p = base; p < end; ++p) {
-      const int32_t u = small[p];
-      if ((uint32_t)u >= m) continue;
-      const int32_t d = g.csr.deg[u];
+    const int32_t end = min(base + (k < heavy_claims ? H : Lc), k < heavy_claims ? n_heavy : n_small);
+    // The claim's pivot headers, one pivot per lane: id, out-degree, row start, reversed rows,
+    // graph and first and last out-neighbour, so the claim pays one chain of dependent loads
+    // instead of one per pivot (R-MAT 20: the small pivots' plan and set builds took 0.81 ms
+    // of the launch with a chain per pivot).
+    const int32_t cnt = end - base;
+    int32_t hu = -1, hd = 0, hlo = 0, hhi = 0, hg = -1;
+    int64_t hp = 0;
+    uint2 hr = make_uint2(0u, 0u);
+    if (lane < cnt) {
+      hu = small[base + lane];
+      if ((uint32_t)hu < m) {
+        hd = g.csr.deg[hu];
+        hp = g.csr.indPtr[hu];
+        if (rrange) hr = rrange[hu];
+        if (gid_of) hg = gid_of[hu];
+      }
+      if (hd >= 2) {
+        hlo = g.csr.colIdx[hp];
+        hhi = g.csr.colIdx[hp + hd - 1];
+      }
+    }
+    for (int32_t j = 0; j < cnt; ++j) {
+      const int32_t d = __builtin_amdgcn_readlane(hd, j);
       if (d < 2) continue;
       if (gid_of) {
-        const int32_t gu = gid_of[u];
+        const int32_t gu = __builtin_amdgcn_readlane(hg, j);
         if (gu != cur) {
           if (cur >= 0) wave_atomic_add(&gcount[cur], c);
           c = 0;
           cur = gu;
         }
       }
-      const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
-      const int32_t *L2;
-      const int32_t d2 = rev_rows(rrange, rlist, u, L2);
+      const uint32_t phi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(hp >> 32), j);
+      const uint32_t plo = (uint32_t)__builtin_amdgcn_readlane((int32_t)hp, j);
+      const int64_t pu = (int64_t)(((uint64_t)phi << 32) | plo);
+      const int32_t *L = g.csr.colIdx + pu;
+      const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readlane((int32_t)hr.x, j);
+      const int32_t d2 = (int32_t)((uint32_t)__builtin_amdgcn_readlane((int32_t)hr.y, j) - rb0);
+      const int32_t *L2 = rrange ? rlist + rb0 : nullptr;
       const int32_t dmax = rrange ? d + kRevSlack : INT32_MAX;
+      const uint8_t *Lroute = route ? route + pu : nullptr;
       PivotSet s;
-      if (!pivot_plan(L[0], L[d - 1], d, TABLE_WORDS / WAVES, s)) {
-        c += pivot_fallback(g, L, d, L2, d2, dmax, 0, 1, lane,
-                            route ? route + g.csr.indPtr[u] : nullptr);
+      if (!pivot_plan(__builtin_amdgcn_readlane(hlo, j), __builtin_amdgcn_readlane(hhi, j), d,
+                      TABLE_WORDS / WAVES, s)) {
+        c += pivot_fallback(g, L, d, L2, d2, dmax, 0, 1, lane, Lroute);
         continue;
       }
       pivot_build<false>(wt, s, L, d, lane, kWave);
-      c += stream_rows(g, L, d, L2, d2, dmax, 0, 1, wt, s, rb, lane,
-                       route ? route + g.csr.indPtr[u] : nullptr);
+      c += stream_rows(g, L, d, L2, d2, dmax, 0, 1, wt, s, rb, lane, Lroute);
       wave_sync();  // the set is rebuilt for the next pivot
     }
   }
