@@ -46,14 +46,15 @@ struct PivotCfg {
 // (tc_kernels.hip) over 2^lg buckets and `tiles` equal ranges of arcs; it needs m <= 2^24 and
 // nnz < 2^32.
 constexpr int32_t kRevSlack = 128;
-constexpr int kRevMaxBuckets = 4096;    // LDS counters per tile
-constexpr int kRevMaxBucketIds = 4096;  // LDS counters per bucket (ids per bucket)
+constexpr int kRevMaxBuckets = 2048;    // LDS counters per tile
+constexpr int kRevMaxBucketIds = 8192;  // LDS counters per bucket (ids per bucket)
+constexpr int kRevChunk = 8192;         // arcs of a tile staged in LDS at a time
 
 struct RevShape {
-  int32_t lg = 6;        // log2 of the bucket count
+  int32_t lg = 9;        // log2 of the bucket count
   int32_t ids = 1;       // ids per bucket: ceil(m / 2^lg)
   int32_t tiles = 1;     // arc ranges of the two tile passes
-  int64_t tile_arcs = 1; // arcs per tile (the last one may hold fewer)
+  int64_t tile_arcs = 1; // arcs per tile (a multiple of kRevChunk; the last tile may hold fewer)
 };
 
 struct RevBufs {
@@ -62,6 +63,7 @@ struct RevBufs {
   uint32_t *ghist = nullptr;  // [2^lg * tiles + 1]
   uint32_t *part = nullptr;   // scan partials of ghist
   uint64_t *pairs = nullptr;  // [nnz] (v >> lg, u) by bucket
+  uint8_t *dsat = nullptr;    // [m] min(d+(v), 255)
   RevShape sh;
 };
 

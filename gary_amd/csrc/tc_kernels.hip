@@ -886,12 +886,12 @@ __global__ void __launch_bounds__(256)
 //
 // The reversed arcs are grouped by target with a two-level counting sort whose counters all live
 // in LDS (device-scope atomics run at about 6.5 G/s here: one global fetch-add per arc and pass
-// took 3.8 ms for the 11.1 M reversed arcs of R-MAT 20):
-//   1. rev_hist_kernel: tile t (an equal range of the COO's arcs) counts its reversed arcs per
-//      bucket into ghist[bucket * tiles + t];
+// took 3.8 ms for the 11.1 M reversed arcs of R-MAT 20 at slack 8):
+//   1. rev_hist_kernel: tile t (a range of the COO's arcs, a multiple of kRevChunk) counts its
+//      reversed arcs per bucket into ghist[bucket * tiles + t];
 //   2. an exclusive scan of ghist gives every (bucket, tile) its first position;
-//   3. rev_place_kernel: the tile walks its arcs again and writes (v >> lg, u) pairs at its
-//      positions (ranks from LDS fetch-adds);
+//   3. rev_place_kernel: the tile reads its arcs again, chunk by chunk, counting-sorts each
+//      chunk's reversed arcs by bucket in LDS and writes them as (v >> lg, u) pairs, in runs;
 //   4. rev_group_kernel: one workgroup per bucket counts its pairs per target in LDS, scans the
 //      counts, writes rrange[v] for every v of the bucket and places each u.
 // Bucket of v: the low lg bits of v ^ (v >> lg) ^ (v >> 2 lg). Given the bucket and h = v >> lg,
@@ -904,80 +904,146 @@ __global__ void __launch_bounds__(256)
 // the two tile passes on R-MAT 20: two dependent loads per step and one step in flight.
 
 constexpr int kRevBlock = 512;
-constexpr int kRevArcs = 4;  // arcs a thread of the tile passes loads before using them
+constexpr int kRevPer = kRevChunk / kRevBlock;  // arcs of a chunk per thread
 constexpr int kRevGroupBlock = 1024;
 constexpr int kRevUnroll = 4;  // pairs a thread of rev_group_kernel loads before using them
+static_assert(kRevChunk % kRevBlock == 0, "whole arcs per thread");
 
 __device__ __forceinline__ uint32_t rev_bucket(uint32_t v, int lg) {
   return (v ^ (v >> lg) ^ (v >> (2 * lg))) & ((1u << lg) - 1u);
 }
 
-// Calls f(v, u) for every reversed arc (u, v) of tile blockIdx.x (arcs [t * tile_arcs,
-// (t + 1) * tile_arcs) of the COO), each thread loading kRevArcs arcs and their degrees before
-// testing them.
-template <class F>
-__device__ __forceinline__ void walk_reversed(const gary_device_graph &g,
-                                              const uint8_t *__restrict__ route,
-                                              const RevShape &sh, F &&f) {
-  const int32_t *__restrict__ row = g.coo.rowIdx;
-  const int32_t *__restrict__ col = g.csr.colIdx;
-  const int32_t *__restrict__ deg = g.csr.deg;
+// dsat[v] = min(d+(v), 255): the reversal test reads one byte per arc (an array of m bytes stays
+// in L2) and the exact out-degree only when both out-degrees are large.
+__global__ void __launch_bounds__(256) rev_dsat_kernel(const int32_t *__restrict__ deg, int32_t m,
+                                                       uint8_t *__restrict__ dsat) {
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < m; v += (int64_t)gridDim.x * 256)
+    dsat[v] = (uint8_t)min(deg[v], 255);
+}
+
+// The arcs [e, e + kRevChunk) of the COO (clipped to e_end), kRevPer per thread: v[q] = target
+// of a reversed arc (d+(u) + kRevSlack < d+(v)), else -1; u[q] its source. The test reads the
+// byte dsat[v] and the exact out-degree only when it saturates; it is written as selects: with
+// early returns inside the unrolled loop the compiler kept v[q] for every arc.
+__device__ __forceinline__ void rev_load_chunk(const gary_device_graph &g,
+                                               const uint8_t *__restrict__ route,
+                                               const uint8_t *__restrict__ dsat, int64_t e,
+                                               int64_t e_end, int32_t (&u)[kRevPer],
+                                               int32_t (&v)[kRevPer]) {
   const uint32_t m = (uint32_t)g.csr.m;
-  const int64_t e_beg = (int64_t)blockIdx.x * sh.tile_arcs;
-  const int64_t e_end = min(g.csr.nnz, e_beg + sh.tile_arcs);
-  for (int64_t e0 = e_beg + threadIdx.x; e0 < e_end; e0 += kRevArcs * kRevBlock) {
-    int32_t u[kRevArcs], v[kRevArcs], du[kRevArcs], dv[kRevArcs];
 #pragma unroll
-    for (int q = 0; q < kRevArcs; ++q) {
-      const int64_t e = e0 + q * kRevBlock;
-      const bool in = e < e_end && (!route || route[e] == kRouteBitmap);
-      u[q] = in ? row[e] : -1;
-      v[q] = in ? col[e] : -1;
-    }
+  for (int q = 0; q < kRevPer; ++q) {
+    const int64_t a = e + q * kRevBlock + threadIdx.x;
+    const bool in = a < e_end && (!route || route[a] == kRouteBitmap);
+    u[q] = in ? g.coo.rowIdx[a] : -1;
+    v[q] = in ? g.csr.colIdx[a] : -1;
+  }
+  int32_t du[kRevPer], sv[kRevPer];
 #pragma unroll
-    for (int q = 0; q < kRevArcs; ++q) {
-      const bool ok = (uint32_t)u[q] < m && (uint32_t)v[q] < m;
-      du[q] = ok ? deg[u[q]] : 0;
-      dv[q] = ok ? deg[v[q]] : 0;
-    }
+  for (int q = 0; q < kRevPer; ++q) {
+    const bool ok = (uint32_t)u[q] < m && (uint32_t)v[q] < m;
+    du[q] = ok ? g.csr.deg[u[q]] : INT32_MAX - kRevSlack;
+    sv[q] = ok ? (int32_t)dsat[v[q]] : 0;
+  }
 #pragma unroll
-    for (int q = 0; q < kRevArcs; ++q)
-      if (du[q] + kRevSlack < dv[q]) f(v[q], u[q]);
+  for (int q = 0; q < kRevPer; ++q) {
+    const int32_t need = du[q] + kRevSlack;  // reversed iff need < d+(v)
+    const bool sat = sv[q] == 255 && need >= 255;
+    const int32_t dv = sat ? g.csr.deg[v[q]] : sv[q];
+    v[q] = need < dv ? v[q] : -1;
   }
 }
 
 __global__ void __launch_bounds__(kRevBlock)
     rev_hist_kernel(gary_device_graph g, const uint8_t *__restrict__ route,
-                    const RouteTable *__restrict__ rtable, RevShape sh,
-                    uint32_t *__restrict__ ghist) {
+                    const RouteTable *__restrict__ rtable, const uint8_t *__restrict__ dsat,
+                    RevShape sh, uint32_t *__restrict__ ghist) {
   __shared__ uint32_t s_hist[kRevMaxBuckets];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   const int nb = 1 << sh.lg;
   for (int i = threadIdx.x; i < nb; i += kRevBlock) s_hist[i] = 0;
   __syncthreads();
-  walk_reversed(g, route, sh, [&](int32_t v, int32_t) {
-    atomicAdd(&s_hist[rev_bucket((uint32_t)v, sh.lg)], 1u);
-  });
+  const int64_t e_beg = (int64_t)blockIdx.x * sh.tile_arcs;
+  const int64_t e_end = min(g.csr.nnz, e_beg + sh.tile_arcs);
+  for (int64_t e = e_beg; e < e_end; e += kRevChunk) {
+    int32_t u[kRevPer], v[kRevPer];
+    rev_load_chunk(g, route, dsat, e, e_end, u, v);
+#pragma unroll
+    for (int q = 0; q < kRevPer; ++q)
+      if (v[q] >= 0) atomicAdd(&s_hist[rev_bucket((uint32_t)v[q], sh.lg)], 1u);
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kRevBlock)
     ghist[(int64_t)i * sh.tiles + blockIdx.x] = s_hist[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) ghist[(int64_t)nb * sh.tiles] = 0u;
 }
 
+// Each chunk's reversed arcs are counting-sorted by bucket in LDS and written out in runs, so the
+// consecutive lanes of a store write consecutive positions of one bucket (written from the
+// lanes' registers, each store touched up to 64 lines: 1.26 ms for the 32 M reversed arcs of the
+// Chung-Lu 4M graph).
 __global__ void __launch_bounds__(kRevBlock)
     rev_place_kernel(gary_device_graph g, const uint8_t *__restrict__ route,
-                     const RouteTable *__restrict__ rtable, RevShape sh,
-                     const uint32_t *__restrict__ ghist, uint64_t *__restrict__ pairs) {
-  __shared__ uint32_t s_cur[kRevMaxBuckets];
+                     const RouteTable *__restrict__ rtable, const uint8_t *__restrict__ dsat,
+                     RevShape sh, const uint32_t *__restrict__ ghist,
+                     uint64_t *__restrict__ pairs) {
+  __shared__ uint2 s_stage[kRevChunk];
+  __shared__ uint32_t s_cur[kRevMaxBuckets];   // the tile's next position per bucket
+  __shared__ uint32_t s_off[kRevMaxBuckets];   // chunk: start of each bucket in s_stage
+  __shared__ uint32_t s_fill[kRevMaxBuckets];  // chunk: count, then fill cursor
+  __shared__ uint32_t s_wave[kRevBlock / kWave];
   if (rtable && !rtable->mixed) route = nullptr;
   const int nb = 1 << sh.lg;
   for (int i = threadIdx.x; i < nb; i += kRevBlock)
     s_cur[i] = ghist[(int64_t)i * sh.tiles + blockIdx.x];
-  __syncthreads();
-  walk_reversed(g, route, sh, [&](int32_t v, int32_t u) {
-    const uint32_t pos = atomicAdd(&s_cur[rev_bucket((uint32_t)v, sh.lg)], 1u);
-    pairs[pos] = ((uint64_t)((uint32_t)v >> sh.lg) << 32) | (uint32_t)u;
-  });
+  const int64_t e_beg = (int64_t)blockIdx.x * sh.tile_arcs;
+  const int64_t e_end = min(g.csr.nnz, e_beg + sh.tile_arcs);
+  for (int64_t e = e_beg; e < e_end; e += kRevChunk) {
+    for (int i = threadIdx.x; i < nb; i += kRevBlock) s_fill[i] = 0;
+    __syncthreads();
+    int32_t u[kRevPer], v[kRevPer];
+    rev_load_chunk(g, route, dsat, e, e_end, u, v);
+#pragma unroll
+    for (int q = 0; q < kRevPer; ++q)
+      if (v[q] >= 0) atomicAdd(&s_fill[rev_bucket((uint32_t)v[q], sh.lg)], 1u);
+    __syncthreads();
+    // exclusive scan of the nb bucket counts (each thread owns per consecutive ones)
+    const int per = (nb + kRevBlock - 1) / kRevBlock;
+    const int i0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (int q = 0; q < per; ++q)
+      if (i0 + q < nb) sum += s_fill[i0 + q];
+    const uint32_t incl = wave_inclusive_scan_u32(sum);
+    if (lane_id() == kWave - 1) s_wave[threadIdx.x / kWave] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum, total = 0;
+    for (int q = 0; q < kRevBlock / kWave; ++q) {
+      if (q < (int)(threadIdx.x / kWave)) run += s_wave[q];
+      total += s_wave[q];
+    }
+    for (int q = 0; q < per; ++q)
+      if (i0 + q < nb) {
+        const uint32_t c = s_fill[i0 + q];
+        s_off[i0 + q] = run;
+        s_fill[i0 + q] = run;
+        run += c;
+      }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kRevPer; ++q)
+      if (v[q] >= 0)
+        s_stage[atomicAdd(&s_fill[rev_bucket((uint32_t)v[q], sh.lg)], 1u)] =
+            make_uint2((uint32_t)v[q], (uint32_t)u[q]);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < total; j += kRevBlock) {
+      const uint2 pr = s_stage[j];
+      const uint32_t b = rev_bucket(pr.x, sh.lg);
+      pairs[s_cur[b] + (j - s_off[b])] = ((uint64_t)(pr.x >> sh.lg) << 32) | pr.y;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kRevBlock) s_cur[i] += s_fill[i] - s_off[i];
+    __syncthreads();
+  }
 }
 
 __global__ void __launch_bounds__(kRevGroupBlock)
@@ -1413,10 +1479,12 @@ RevShape rev_shape(int32_t m, int64_t nnz) {
   RevShape r;
   int lg = 1;
   while ((1ll << lg) < (int64_t)m) ++lg;
-  r.lg = std::min(12, std::max(6, lg - 9));
+  r.lg = std::min(11, std::max(9, lg - 13));
   r.ids = (int32_t)(((int64_t)m + (1ll << r.lg) - 1) >> r.lg);
-  r.tiles = (int32_t)std::max<int64_t>(1, std::min<int64_t>(1024, nnz / 16384));
-  r.tile_arcs = (nnz + r.tiles - 1) / r.tiles;
+  const int64_t chunks = (nnz + kRevChunk - 1) / kRevChunk;
+  r.tiles = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4096, chunks));
+  r.tile_arcs = (chunks + r.tiles - 1) / r.tiles * kRevChunk;
+  r.tiles = (int32_t)((nnz + r.tile_arcs - 1) / r.tile_arcs);
   return r;
 }
 
@@ -1430,7 +1498,8 @@ size_t rev_bytes(int32_t m, int64_t nnz) {
   const uint32_t hist = (uint32_t)(1u << sh.lg) * (uint32_t)sh.tiles + 1u;
   return al256(sizeof(uint2) * (size_t)m) + al256(sizeof(uint32_t) * hist) +
          al256(sizeof(uint32_t) * exclusive_scan_parts(hist)) +
-         al256(sizeof(uint64_t) * (size_t)nnz) + al256(sizeof(int32_t) * (size_t)nnz);
+         al256(sizeof(uint64_t) * (size_t)nnz) + al256(sizeof(int32_t) * (size_t)nnz) +
+         al256((size_t)m);
 }
 
 RevBufs rev_carve(void *p, int32_t m, int64_t nnz) {
@@ -1448,6 +1517,8 @@ RevBufs rev_carve(void *p, int32_t m, int64_t nnz) {
   r.pairs = (uint64_t *)c;
   c += al256(sizeof(uint64_t) * (size_t)nnz);
   r.rlist = (int32_t *)c;
+  c += al256(sizeof(int32_t) * (size_t)nnz);
+  r.dsat = (uint8_t *)c;
   return r;
 }
 
@@ -1492,13 +1563,17 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
     GARY_CHECK_ARG(g.coo.rowIdx && g.coo.nnz == g.csr.nnz,
                    "reverse rows need the COO rowIdx (call populate first)");
     GARY_CHECK_ARG(rev_supported(m, g.csr.nnz) && (1 << sh.lg) <= kRevMaxBuckets &&
-                       sh.ids <= kRevMaxBucketIds && (int64_t)sh.tiles * sh.tile_arcs >= g.csr.nnz,
+                       sh.ids <= kRevMaxBucketIds && (int64_t)sh.tiles * sh.tile_arcs >= g.csr.nnz &&
+                       sh.tile_arcs % kRevChunk == 0,
                    "reverse rows: m=%d nnz=%lld out of range", m, (long long)g.csr.nnz);
     const uint32_t hist = (uint32_t)(1u << sh.lg) * (uint32_t)sh.tiles + 1u;
-    rev_hist_kernel<<<sh.tiles, kRevBlock, 0, st>>>(g, route, table, sh, rev->ghist);
+    rev_dsat_kernel<<<(int)std::min<int64_t>(((int64_t)m + 255) / 256, 2048), 256, 0, st>>>(
+        g.csr.deg, m, rev->dsat);
+    GARY_HIP(hipGetLastError());
+    rev_hist_kernel<<<sh.tiles, kRevBlock, 0, st>>>(g, route, table, rev->dsat, sh, rev->ghist);
     GARY_HIP(hipGetLastError());
     GARY_TRY(exclusive_scan_u32(rev->ghist, hist, rev->part, st));
-    rev_place_kernel<<<sh.tiles, kRevBlock, 0, st>>>(g, route, table, sh, rev->ghist,
+    rev_place_kernel<<<sh.tiles, kRevBlock, 0, st>>>(g, route, table, rev->dsat, sh, rev->ghist,
                                                      rev->pairs);
     GARY_HIP(hipGetLastError());
     rev_group_kernel<<<1 << sh.lg, kRevGroupBlock, 0, st>>>(sh, m, rev->ghist, rev->pairs,
