@@ -41,11 +41,15 @@ struct PivotCfg {
 // Reverse rows of the pivot path. An arc (u, v) is streamed from the endpoint whose out-list is
 // shorter: N+(v) through the set of N+(u) at pivot u (forward), or, when d+(u) + kRevSlack <
 // d+(v), N+(u) through the set of N+(v) at pivot v (reverse). The reversed arcs are grouped by
-// v per call into rlist (the sources u); rrange[v] = (begin, end) of v's list there.
+// v per call (sources of out-degree kRevMaxSourceDeg or more are never reversed) into rrows, one
+// (row start, out-degree) of the source u per arc, so a pivot reads
+// its reverse rows without a dependent load of indPtr[u] and d+(u); rrange[v] = (begin, end) of
+// v's list there.
 // rrange == nullptr streams every arc forward. The grouping is a two-level counting sort
 // (tc_kernels.hip) over 2^lg buckets and `tiles` equal ranges of arcs; it needs m <= 2^24 and
 // nnz < 2^32.
 constexpr int32_t kRevSlack = 128;
+constexpr int32_t kRevMaxSourceDeg = 1 << 19;  // sources with a longer out-list stay forward
 constexpr int kRevMaxBuckets = 2048;    // LDS counters per tile
 constexpr int kRevMaxBucketIds = 8192;  // LDS counters per bucket (ids per bucket)
 constexpr int kRevChunk = 8192;         // arcs of a tile staged in LDS at a time
@@ -58,11 +62,11 @@ struct RevShape {
 };
 
 struct RevBufs {
-  uint2 *rrange = nullptr;    // [m] (begin, end) in rlist
-  int32_t *rlist = nullptr;   // [nnz]
+  uint2 *rrange = nullptr;    // [m] (begin, end) in rrows
+  uint2 *rrows = nullptr;     // [nnz] (indPtr[u], d+(u)) of each reversed arc's source
   uint32_t *ghist = nullptr;  // [2^lg * tiles + 1]
   uint32_t *part = nullptr;   // scan partials of ghist
-  uint64_t *pairs = nullptr;  // [nnz] (v >> lg, u) by bucket
+  uint64_t *pairs = nullptr;  // [nnz] (v >> lg) << 51 | d+(u) << 32 | indPtr[u], by bucket
   uint8_t *dsat = nullptr;    // [m] min(d+(v), 255)
   RevShape sh;
 };

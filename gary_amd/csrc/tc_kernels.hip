@@ -442,15 +442,15 @@ __device__ __forceinline__ uint32_t probe_chunk(const int4 (&data)[CH / 4], int3
 // (16-byte chunks with two windows in flight: 3.38 ms per R-MAT 20 count; 32-byte chunks with
 // a reset slot array and four shuffles: 3.21 ms; this form: 3.10 ms.)
 // The rows streamed are N+(v) for the row vertices L[0..d) (the pivot's out-list: forward rows,
-// skipped when d+(v) > dmax, since such an arc is reversed and streamed at v) and then L2[0..d2)
-// (the sources of the arcs reversed onto this pivot), rows r0, r0 + stride, ... of the
-// concatenation. The returned per-lane count is below 2^32: a lane probes at most one chunk (CH
-// elements) per window, and a call has at most (Σ_v (d(v) / CH + 2)) / 64 + (d + d2) / 64 + 1
-// windows, so the count is at most nnz / 64 + 3(d + d2) / 8 + 8 < 2^28 + 2^31 + 8 for colIdx
-// below 2^34 elements and d + d2 < 2^32.
+// skipped when d+(v) > dmax, since such an arc is reversed and streamed at v) and then the rows
+// R2[0..d2) given by (row start, length) (the sources of the arcs reversed onto this pivot), rows
+// r0, r0 + stride, ... of the concatenation. The returned per-lane count is below 2^32: a lane
+// probes at most one chunk (CH elements) per window, and a call has at most (Σ_v (d(v) / CH +
+// 2)) / 64 + (d + d2) / 64 + 1 windows, so the count is at most nnz / 64 + 3(d + d2) / 8 + 8 <
+// 2^28 + 2^31 + 8 for colIdx below 2^34 elements and d + d2 < 2^32.
 __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
                                                 const int32_t *__restrict__ L, int32_t d,
-                                                const int32_t *__restrict__ L2, int32_t d2,
+                                                const uint2 *__restrict__ R2, int32_t d2,
                                                 int32_t dmax, int32_t r0, int32_t stride,
                                                 const int32_t *table, const PivotSet &ps,
                                                 RowSlots &rs, int lane,
@@ -468,16 +468,19 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
     const int32_t r = b + lane * stride;
     int32_t dv = 0;
     int64_t pv = 0;
-    int32_t v = -1;
     if (r < d) {
-      if (!Lroute || Lroute[r] == kRouteBitmap) v = L[r];
+      if (!Lroute || Lroute[r] == kRouteBitmap) {
+        const int32_t v = L[r];
+        if ((uint32_t)v < m) {
+          dv = g.csr.deg[v];
+          pv = g.csr.indPtr[v];
+          if (dv > dmax) dv = 0;  // reversed: counted at pivot v
+        }
+      }
     } else if (r < dtot) {
-      v = L2[r - d];
-    }
-    if ((uint32_t)v < m) {
-      dv = g.csr.deg[v];
-      pv = g.csr.indPtr[v];
-      if (r < d && dv > dmax) dv = 0;  // reversed: counted at pivot v
+      const uint2 rw = R2[r - d];
+      pv = rw.x;
+      dv = (int32_t)rw.y;
     }
     const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + CH - 1) >> LG) - (pv >> LG)) : 0;
     const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
@@ -529,19 +532,27 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
 // + 1, so the call's is below nnz / 64 + d + d2 < 2^32 for d + d2 < 2^31.
 __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
                                                    const int32_t *__restrict__ L, int32_t d,
-                                                   const int32_t *__restrict__ L2, int32_t d2,
+                                                   const uint2 *__restrict__ R2, int32_t d2,
                                                    int32_t dmax, int32_t r0, int32_t stride,
                                                    int lane, const uint8_t *__restrict__ Lroute) {
   const uint32_t m = (uint32_t)g.csr.m;
   uint32_t c = 0;
   for (int32_t k = r0; k < d + d2; k += stride) {
-    if (k < d && Lroute && Lroute[k] != kRouteBitmap) continue;
-    const int32_t v = k < d ? L[k] : L2[k - d];
-    if ((uint32_t)v >= m) continue;
-    const int32_t dv = g.csr.deg[v];
-    if (k < d && dv > dmax) continue;  // reversed: counted at pivot v
-    c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + g.csr.indPtr[v], dv, nullptr, 0,
-                                        lane);
+    int64_t pv;
+    int32_t dv;
+    if (k < d) {
+      if (Lroute && Lroute[k] != kRouteBitmap) continue;
+      const int32_t v = L[k];
+      if ((uint32_t)v >= m) continue;
+      dv = g.csr.deg[v];
+      if (dv > dmax) continue;  // reversed: counted at pivot v
+      pv = g.csr.indPtr[v];
+    } else {
+      const uint2 rw = R2[k - d];
+      pv = rw.x;
+      dv = (int32_t)rw.y;
+    }
+    c += isect_group<64, kIsectBsearch>(L, d, g.csr.colIdx + pv, dv, nullptr, 0, lane);
   }
   return c;
 }
@@ -570,18 +581,18 @@ static_assert(kSmallBatchHeavyRev <= kWave && kSmallBatchLightRev <= kWave &&
 constexpr int kLightLog = 4;
 static_assert(kLightLog >= 1 && kLightLog <= 15, "light-pivot threshold");
 
-// Reverse rows (rrange != nullptr): pivot u also streams N+(x) for the x in rlist[rrange[u].x ..
-// rrange[u].y), the sources of the arcs (x, u) with d+(x) + kRevSlack < d+(u), and
-// skips its own out-neighbours v with d+(v) > d+(u) + kRevSlack, whose arcs are streamed at v.
+// Reverse rows (rrange != nullptr): pivot u also streams the rows rrows[rrange[u].x ..
+// rrange[u].y), the sources x of the arcs (x, u) with d+(x) + kRevSlack < d+(u), and skips its
+// own out-neighbours v with d+(v) > d+(u) + kRevSlack, whose arcs are streamed at v.
 __device__ __forceinline__ int32_t rev_rows(const uint2 *__restrict__ rrange,
-                                            const int32_t *__restrict__ rlist, int32_t u,
-                                            const int32_t *&L2) {
+                                            const uint2 *__restrict__ rrows, int32_t u,
+                                            const uint2 *&R2) {
   if (!rrange) {
-    L2 = nullptr;
+    R2 = nullptr;
     return 0;
   }
   const uint2 r = rrange[u];
-  L2 = rlist + r.x;
+  R2 = rrows + r.x;
   return (int32_t)(r.y - r.x);
 }
 
@@ -593,7 +604,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     const int32_t *__restrict__ gid_of, unsigned long long *gcount,
                     const uint8_t *__restrict__ route, const RouteTable *__restrict__ rtable,
                     const int32_t *__restrict__ split, const uint2 *__restrict__ rrange,
-                    const int32_t *__restrict__ rlist) {
+                    const uint2 *__restrict__ rrows) {
   __shared__ int32_t table[TABLE_WORDS];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   __shared__ RowSlots s_rows[WAVES];
@@ -617,9 +628,9 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     const int32_t d = (uint32_t)u < m ? g.csr.deg[u] : 0;
     if (d >= 2) {
       const int32_t *L = g.csr.colIdx + g.csr.indPtr[u];
-      const int32_t *L2;
-      const int32_t d2 = rev_rows(rrange, rlist, u, L2);
-      const int32_t dmax = rrange ? d + kRevSlack : INT32_MAX;
+      const uint2 *L2;
+      const int32_t d2 = rev_rows(rrange, rrows, u, L2);
+      const int32_t dmax = rrange && d < kRevMaxSourceDeg ? d + kRevSlack : INT32_MAX;
       PivotSet s;
       if (pivot_plan(L[0], L[d - 1], d, TABLE_WORDS, s)) {
         pivot_build<true>(table, s, L, d, threadIdx.x, WAVES * 64);
@@ -689,8 +700,8 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       const int32_t *L = g.csr.colIdx + pu;
       const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readlane((int32_t)hr.x, j);
       const int32_t d2 = (int32_t)((uint32_t)__builtin_amdgcn_readlane((int32_t)hr.y, j) - rb0);
-      const int32_t *L2 = rrange ? rlist + rb0 : nullptr;
-      const int32_t dmax = rrange ? d + kRevSlack : INT32_MAX;
+      const uint2 *L2 = rrange ? rrows + rb0 : nullptr;
+      const int32_t dmax = rrange && d < kRevMaxSourceDeg ? d + kRevSlack : INT32_MAX;
       const uint8_t *Lroute = route ? route + pu : nullptr;
       PivotSet s;
       if (!pivot_plan(__builtin_amdgcn_readlane(hlo, j), __builtin_amdgcn_readlane(hhi, j), d,
@@ -891,9 +902,11 @@ __global__ void __launch_bounds__(256)
 //      reversed arcs per bucket into ghist[bucket * tiles + t];
 //   2. an exclusive scan of ghist gives every (bucket, tile) its first position;
 //   3. rev_place_kernel: the tile reads its arcs again, chunk by chunk, counting-sorts each
-//      chunk's reversed arcs by bucket in LDS and writes them as (v >> lg, u) pairs, in runs;
+//      chunk's reversed arcs by bucket in LDS and writes them, in runs, as 64-bit words
+//      (v >> lg) << 51 | d+(u) << 32 | indPtr[u] (h < 2^13 ids per bucket, d+(u) < 2^19);
 //   4. rev_group_kernel: one workgroup per bucket counts its pairs per target in LDS, scans the
-//      counts, writes rrange[v] for every v of the bucket and places each u.
+//      counts, writes rrange[v] for every v of the bucket and places each source's row (start,
+//      length).
 // Bucket of v: the low lg bits of v ^ (v >> lg) ^ (v >> 2 lg). Given the bucket and h = v >> lg,
 // v's low bits are bucket ^ h ^ (h >> lg), so a bucket holds at most ceil(m / 2^lg) ids, indexed
 // densely by h; and ids with few set bits (R-MAT's hubs) spread over the buckets, where buckets
@@ -908,6 +921,9 @@ constexpr int kRevPer = kRevChunk / kRevBlock;  // arcs of a chunk per thread
 constexpr int kRevGroupBlock = 1024;
 constexpr int kRevUnroll = 4;  // pairs a thread of rev_group_kernel loads before using them
 static_assert(kRevChunk % kRevBlock == 0, "whole arcs per thread");
+static_assert(kRevMaxBuckets <= 65536 && kRevMaxBucketIds <= (1 << 13) &&
+                  kRevMaxSourceDeg <= (1 << 19),
+              "a pair word holds h in 13 bits and d+(u) in 19; a bucket id fits 16 bits");
 
 __device__ __forceinline__ uint32_t rev_bucket(uint32_t v, int lg) {
   return (v ^ (v >> lg) ^ (v >> (2 * lg))) & ((1u << lg) - 1u);
@@ -922,14 +938,15 @@ __global__ void __launch_bounds__(256) rev_dsat_kernel(const int32_t *__restrict
 }
 
 // The arcs [e, e + kRevChunk) of the COO (clipped to e_end), kRevPer per thread: v[q] = target
-// of a reversed arc (d+(u) + kRevSlack < d+(v)), else -1; u[q] its source. The test reads the
-// byte dsat[v] and the exact out-degree only when it saturates; it is written as selects: with
-// early returns inside the unrolled loop the compiler kept v[q] for every arc.
+// of a reversed arc (d+(u) + kRevSlack < d+(v) and d+(u) < kRevMaxSourceDeg), else -1; u[q] its
+// source, du[q] = d+(u). The test reads the byte dsat[v] and the exact out-degree only when it
+// saturates; it is written as selects: with early returns inside the unrolled loop the
+// compiler kept v[q] for every arc.
 __device__ __forceinline__ void rev_load_chunk(const gary_device_graph &g,
                                                const uint8_t *__restrict__ route,
                                                const uint8_t *__restrict__ dsat, int64_t e,
                                                int64_t e_end, int32_t (&u)[kRevPer],
-                                               int32_t (&v)[kRevPer]) {
+                                               int32_t (&v)[kRevPer], int32_t (&du)[kRevPer]) {
   const uint32_t m = (uint32_t)g.csr.m;
 #pragma unroll
   for (int q = 0; q < kRevPer; ++q) {
@@ -938,7 +955,7 @@ __device__ __forceinline__ void rev_load_chunk(const gary_device_graph &g,
     u[q] = in ? g.coo.rowIdx[a] : -1;
     v[q] = in ? g.csr.colIdx[a] : -1;
   }
-  int32_t du[kRevPer], sv[kRevPer];
+  int32_t sv[kRevPer];
 #pragma unroll
   for (int q = 0; q < kRevPer; ++q) {
     const bool ok = (uint32_t)u[q] < m && (uint32_t)v[q] < m;
@@ -950,7 +967,7 @@ __device__ __forceinline__ void rev_load_chunk(const gary_device_graph &g,
     const int32_t need = du[q] + kRevSlack;  // reversed iff need < d+(v)
     const bool sat = sv[q] == 255 && need >= 255;
     const int32_t dv = sat ? g.csr.deg[v[q]] : sv[q];
-    v[q] = need < dv ? v[q] : -1;
+    v[q] = need < dv && du[q] < kRevMaxSourceDeg ? v[q] : -1;
   }
 }
 
@@ -966,8 +983,8 @@ __global__ void __launch_bounds__(kRevBlock)
   const int64_t e_beg = (int64_t)blockIdx.x * sh.tile_arcs;
   const int64_t e_end = min(g.csr.nnz, e_beg + sh.tile_arcs);
   for (int64_t e = e_beg; e < e_end; e += kRevChunk) {
-    int32_t u[kRevPer], v[kRevPer];
-    rev_load_chunk(g, route, dsat, e, e_end, u, v);
+    int32_t u[kRevPer], v[kRevPer], du[kRevPer];
+    rev_load_chunk(g, route, dsat, e, e_end, u, v, du);
 #pragma unroll
     for (int q = 0; q < kRevPer; ++q)
       if (v[q] >= 0) atomicAdd(&s_hist[rev_bucket((uint32_t)v[q], sh.lg)], 1u);
@@ -987,7 +1004,8 @@ __global__ void __launch_bounds__(kRevBlock)
                      const RouteTable *__restrict__ rtable, const uint8_t *__restrict__ dsat,
                      RevShape sh, const uint32_t *__restrict__ ghist,
                      uint64_t *__restrict__ pairs) {
-  __shared__ uint2 s_stage[kRevChunk];
+  __shared__ uint64_t s_word[kRevChunk];  // the staged arcs' words, by bucket
+  __shared__ uint16_t s_bk[kRevChunk];    // and their buckets
   __shared__ uint32_t s_cur[kRevMaxBuckets];   // the tile's next position per bucket
   __shared__ uint32_t s_off[kRevMaxBuckets];   // chunk: start of each bucket in s_stage
   __shared__ uint32_t s_fill[kRevMaxBuckets];  // chunk: count, then fill cursor
@@ -1001,8 +1019,11 @@ __global__ void __launch_bounds__(kRevBlock)
   for (int64_t e = e_beg; e < e_end; e += kRevChunk) {
     for (int i = threadIdx.x; i < nb; i += kRevBlock) s_fill[i] = 0;
     __syncthreads();
-    int32_t u[kRevPer], v[kRevPer];
-    rev_load_chunk(g, route, dsat, e, e_end, u, v);
+    int32_t u[kRevPer], v[kRevPer], du[kRevPer];
+    rev_load_chunk(g, route, dsat, e, e_end, u, v, du);
+    uint32_t pu[kRevPer];  // row starts of the sources (consecutive arcs mostly share one)
+#pragma unroll
+    for (int q = 0; q < kRevPer; ++q) pu[q] = v[q] >= 0 ? (uint32_t)g.csr.indPtr[u[q]] : 0u;
 #pragma unroll
     for (int q = 0; q < kRevPer; ++q)
       if (v[q] >= 0) atomicAdd(&s_fill[rev_bucket((uint32_t)v[q], sh.lg)], 1u);
@@ -1031,14 +1052,16 @@ __global__ void __launch_bounds__(kRevBlock)
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kRevPer; ++q)
-      if (v[q] >= 0)
-        s_stage[atomicAdd(&s_fill[rev_bucket((uint32_t)v[q], sh.lg)], 1u)] =
-            make_uint2((uint32_t)v[q], (uint32_t)u[q]);
+      if (v[q] >= 0) {
+        const uint32_t b = rev_bucket((uint32_t)v[q], sh.lg);
+        const uint32_t at = atomicAdd(&s_fill[b], 1u);
+        s_word[at] = ((uint64_t)((uint32_t)v[q] >> sh.lg) << 51) | ((uint64_t)du[q] << 32) | pu[q];
+        s_bk[at] = (uint16_t)b;
+      }
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < total; j += kRevBlock) {
-      const uint2 pr = s_stage[j];
-      const uint32_t b = rev_bucket(pr.x, sh.lg);
-      pairs[s_cur[b] + (j - s_off[b])] = ((uint64_t)(pr.x >> sh.lg) << 32) | pr.y;
+      const uint32_t b = s_bk[j];
+      pairs[s_cur[b] + (j - s_off[b])] = s_word[j];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += kRevBlock) s_cur[i] += s_fill[i] - s_off[i];
@@ -1047,9 +1070,10 @@ __global__ void __launch_bounds__(kRevBlock)
 }
 
 __global__ void __launch_bounds__(kRevGroupBlock)
-    rev_group_kernel(RevShape sh, int32_t m, const uint32_t *__restrict__ ghist,
+    rev_group_kernel(gary_device_graph g, RevShape sh, const uint32_t *__restrict__ ghist,
                      const uint64_t *__restrict__ pairs, uint2 *__restrict__ rrange,
-                     int32_t *__restrict__ rlist) {
+                     uint2 *__restrict__ rrows) {
+  const int32_t m = g.csr.m;
   constexpr int kW = kRevGroupBlock / kWave;
   __shared__ uint32_t s_cnt[kRevMaxBucketIds];
   __shared__ uint32_t s_wave[kW];
@@ -1063,7 +1087,7 @@ __global__ void __launch_bounds__(kRevGroupBlock)
 #pragma unroll
     for (int q = 0; q < kRevUnroll; ++q) {
       const uint32_t i = i0 + q * kRevGroupBlock;
-      h[q] = i < end ? (uint32_t)(pairs[i] >> 32) : 0xffffffffu;
+      h[q] = i < end ? (uint32_t)(pairs[i] >> 51) : 0xffffffffu;
     }
 #pragma unroll
     for (int q = 0; q < kRevUnroll; ++q)
@@ -1102,7 +1126,9 @@ __global__ void __launch_bounds__(kRevGroupBlock)
     }
 #pragma unroll
     for (int q = 0; q < kRevUnroll; ++q)
-      if (pr[q] != ~0ull) rlist[atomicAdd(&s_cnt[(uint32_t)(pr[q] >> 32)], 1u)] = (int32_t)(uint32_t)pr[q];
+      if (pr[q] != ~0ull)
+        rrows[atomicAdd(&s_cnt[(uint32_t)(pr[q] >> 51)], 1u)] =
+            make_uint2((uint32_t)pr[q], (uint32_t)(pr[q] >> 32) & (kRevMaxSourceDeg - 1u));
   }
 }
 
@@ -1498,7 +1524,7 @@ size_t rev_bytes(int32_t m, int64_t nnz) {
   const uint32_t hist = (uint32_t)(1u << sh.lg) * (uint32_t)sh.tiles + 1u;
   return al256(sizeof(uint2) * (size_t)m) + al256(sizeof(uint32_t) * hist) +
          al256(sizeof(uint32_t) * exclusive_scan_parts(hist)) +
-         al256(sizeof(uint64_t) * (size_t)nnz) + al256(sizeof(int32_t) * (size_t)nnz) +
+         al256(sizeof(uint64_t) * (size_t)nnz) + al256(sizeof(uint2) * (size_t)nnz) +
          al256((size_t)m);
 }
 
@@ -1516,8 +1542,8 @@ RevBufs rev_carve(void *p, int32_t m, int64_t nnz) {
   c += al256(sizeof(uint32_t) * exclusive_scan_parts(hist));
   r.pairs = (uint64_t *)c;
   c += al256(sizeof(uint64_t) * (size_t)nnz);
-  r.rlist = (int32_t *)c;
-  c += al256(sizeof(int32_t) * (size_t)nnz);
+  r.rrows = (uint2 *)c;
+  c += al256(sizeof(uint2) * (size_t)nnz);
   r.dsat = (uint8_t *)c;
   return r;
 }
@@ -1557,7 +1583,7 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
                  "pivot path: bins small=%d hub=%d out of range for %d waves", pc.bin_small,
                  pc.bin_hub, pc.waves);
   const uint2 *rrange = nullptr;
-  const int32_t *rlist = nullptr;
+  const uint2 *rrows = nullptr;
   if (rev && rev->rrange) {
     const RevShape &sh = rev->sh;
     GARY_CHECK_ARG(g.coo.rowIdx && g.coo.nnz == g.csr.nnz,
@@ -1576,11 +1602,11 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
     rev_place_kernel<<<sh.tiles, kRevBlock, 0, st>>>(g, route, table, rev->dsat, sh, rev->ghist,
                                                      rev->pairs);
     GARY_HIP(hipGetLastError());
-    rev_group_kernel<<<1 << sh.lg, kRevGroupBlock, 0, st>>>(sh, m, rev->ghist, rev->pairs,
-                                                            rev->rrange, rev->rlist);
+    rev_group_kernel<<<1 << sh.lg, kRevGroupBlock, 0, st>>>(g, sh, rev->ghist, rev->pairs,
+                                                            rev->rrange, rev->rrows);
     GARY_HIP(hipGetLastError());
     rrange = rev->rrange;
-    rlist = rev->rlist;
+    rrows = rev->rrows;
   }
   const bool ordered = gid_of == nullptr;
   bin_hist_kernel<<<bgrid, 256, 0, st>>>(g.csr.deg, m, pc.bin_small, pc.bin_hub, counters, table,
@@ -1593,22 +1619,22 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
   // hubs: small list empty (counters[2] == 0), big list = bin 2
   tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
       g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of,
-      gcount, route, table, counters + kSplitOff + 2, rrange, rlist);
+      gcount, route, table, counters + kSplitOff + 2, rrange, rrows);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_begin(st));
   const int grid = pc.grid ? pc.grid : 2048;
   if (pc.waves == 2)
     tc_pivot_kernel<2, kPivotWords><<<grid, 2 * 64, 0, st>>>(
         g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
-        table, counters + kSplitOff, rrange, rlist);
+        table, counters + kSplitOff, rrange, rrows);
   else if (pc.waves == 8)
     tc_pivot_kernel<8, kPivotWords><<<grid, 8 * 64, 0, st>>>(
         g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
-        table, counters + kSplitOff, rrange, rlist);
+        table, counters + kSplitOff, rrange, rrows);
   else
     tc_pivot_kernel<4, kPivotWords><<<grid, 4 * 64, 0, st>>>(
         g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
-        table, counters + kSplitOff, rrange, rlist);
+        table, counters + kSplitOff, rrange, rrows);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_end(st));
   return GARY_OK;
