@@ -342,6 +342,27 @@ def test_pivot_directions(golden_ctxs, rmat16, er_ctx, direction, strategy):
         assert gq.triangleCountHom(ctx, strategy, big) == want
 
 
+def test_reverse_rows_huge_source():
+    """Id orientation. U = 2,000,000 points at V = 1,999,999 and at 2^19 + 6 low ids; V points at
+    2^19 + 200 low ids, so the arc (U, V) has d+(V) > d+(U) + 128 but its source's out-degree is
+    2^19 or more: it stays forward (the grouped word holds 19 bits of out-degree). Both hubs
+    exceed every LDS set and go through the edge-by-edge fallback."""
+    U, V = 2_000_000, 1_999_999
+    k = 1 << 19
+    out_u = np.arange(0, 2 * (k + 6), 2, dtype=np.int32)  # even ids
+    out_v = np.arange(0, 3 * (k + 200), 3, dtype=np.int32)  # multiples of 3
+    src = np.concatenate([np.full(len(out_u) + 1, U, np.int32), np.full(len(out_v), V, np.int32)])
+    dst = np.concatenate([out_u, [V], out_v]).astype(np.int32)
+    g = graph.csr_from_edges(U + 1, src, dst, graph.ORIENT_ID)
+    want = oracle.tri_count(g.indPtr, g.colIdx)
+    assert want == len(np.intersect1d(out_u, out_v))  # triangles (U, V, w), w = 0 mod 6
+    ctx = _ctx(g)
+    for direction in (gq.MIN_DIRECTION, gq.FORWARD):
+        cfg = gq.launch_cfg(direction=direction)
+        for s in (gq.BITMAP, gq.AUTO_LEARNED):
+            assert gq.triangleCountHom(ctx, s, cfg) == want, (s, direction)
+
+
 def test_bitmap_without_coo(rmat16):
     """Min-direction lists the reversed arcs from the COO; a graph whose COO was never populated
     streams every arc forward and still counts exactly."""
