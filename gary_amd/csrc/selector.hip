@@ -634,7 +634,8 @@ int launch_routed_ws(const gary_device_graph &g, const RoutedWs &w, const RouteT
   if (sh.any_bitmap)
     GARY_TRY(launch_pivot_core(g, pc, w.lists, w.stride, w.lists + 3 * w.stride, nullptr, nullptr,
                                0, cnt, st, true, w.route, by_cost ? w.pcost : nullptr, cost_big,
-                               table, pc.min_direction && w.rev.rrange ? &w.rev : nullptr));
+                               table,
+                               rev_wanted(pc.direction, g) && w.rev.rrange ? &w.rev : nullptr));
   if (!sh.any_edge) return GARY_OK;
   return launch_edge_lists(g, w.cached, w.counts, w.merge, w.counts + 1, cnt, st);
 }

@@ -35,7 +35,7 @@ constexpr int kDefaultBinSmall = 128;
 
 struct PivotCfg {
   int bin_small = kDefaultBinSmall, bin_hub = 2048, grid = 0, waves = 4;
-  bool min_direction = true;  // GARY_PIVOT_MIN_DIRECTION (reverse rows) or GARY_PIVOT_FORWARD
+  int direction = GARY_PIVOT_AUTO;  // GARY_PIVOT_*
 };
 
 // Reverse rows of the pivot path. An arc (u, v) is streamed from the endpoint whose out-list is
@@ -71,9 +71,12 @@ struct RevBufs {
   RevShape sh;
 };
 
-// Whether reverse rows apply to a graph of m vertices and nnz arcs; the scratch bytes of RevBufs
-// (0 when they do not apply) and their carving from `p`.
+// Whether reverse rows apply to a graph of m vertices and nnz arcs; whether a launch with
+// `direction` uses them on graph g (AUTO: from kRevAutoArcs arcs on); the scratch bytes of
+// RevBufs (0 when they do not apply) and their carving from `p`.
+constexpr int64_t kRevAutoArcs = 1 << 20;
 bool rev_supported(int32_t m, int64_t nnz);
+bool rev_wanted(int direction, const gary_device_graph &g);
 size_t rev_bytes(int32_t m, int64_t nnz);
 RevBufs rev_carve(void *p, int32_t m, int64_t nnz);
 

@@ -1385,7 +1385,7 @@ namespace {
 
 struct TcCfg {
   int block = 256, grid = 0, group = 64, cache_depth = 64, partition = GARY_PARTITION_INTERLEAVED;
-  int bin_small = kDefaultBinSmall, bin_hub = 2048, direction = GARY_PIVOT_MIN_DIRECTION;
+  int bin_small = kDefaultBinSmall, bin_hub = 2048, direction = GARY_PIVOT_AUTO;
 };
 
 constexpr int kPivotWords = 4096;  // 16 KiB: a big pivot's set, or one small-pivot set per wave
@@ -1402,7 +1402,7 @@ PivotCfg pivot_cfg(const TcCfg &c) {
   p.bin_hub = c.bin_hub;
   p.grid = c.grid;
   p.waves = c.block == 128 ? 2 : c.block == 512 ? 8 : 4;
-  p.min_direction = c.direction == GARY_PIVOT_MIN_DIRECTION;
+  p.direction = c.direction;
   return p;
 }
 
@@ -1428,7 +1428,8 @@ int resolve_cfg(const gary_launch_cfg *in, TcCfg &c) {
                      c.partition == GARY_PARTITION_DYNAMIC,
                  "launch cfg: bad partition %d", c.partition);
   GARY_CHECK_ARG(c.grid >= 0, "launch cfg: grid=%d", c.grid);
-  GARY_CHECK_ARG(c.direction == GARY_PIVOT_MIN_DIRECTION || c.direction == GARY_PIVOT_FORWARD,
+  GARY_CHECK_ARG(c.direction == GARY_PIVOT_AUTO || c.direction == GARY_PIVOT_MIN_DIRECTION ||
+                     c.direction == GARY_PIVOT_FORWARD,
                  "launch cfg: bad pivot direction %d", c.direction);
   GARY_CHECK_ARG(c.bin_small >= 2 && c.bin_hub >= c.bin_small && c.bin_hub <= kPivotWords / 2,
                  "launch cfg: bins small=%d hub=%d out of range", c.bin_small, c.bin_hub);
@@ -1497,6 +1498,14 @@ int launch_edge(const gary_device_graph &g, const TcCfg &c, unsigned long long *
 
 bool rev_supported(int32_t m, int64_t nnz) {
   return m >= 1 && m <= (1 << 24) && nnz >= 1 && nnz < (1ll << 32);
+}
+
+bool rev_wanted(int direction, const gary_device_graph &g) {
+  // the grouping lists the reversed arcs from the COO: without its rowIdx every arc streams
+  // forward
+  if (!rev_supported(g.csr.m, g.csr.nnz) || !g.coo.rowIdx || g.coo.nnz != g.csr.nnz) return false;
+  return direction == GARY_PIVOT_MIN_DIRECTION ||
+         (direction == GARY_PIVOT_AUTO && g.csr.nnz >= kRevAutoArcs);
 }
 
 namespace {
@@ -1669,10 +1678,7 @@ int launch_pivot(const gary_device_graph &g, const TcCfg &c, unsigned long long 
   void *ws = nullptr;
   const size_t stride = ((size_t)m + 63) & ~(size_t)63;
   const size_t core = sizeof(int32_t) * (3 * stride + kPivotCounterWords);
-  // the reversed arcs are listed from the COO: without its rowIdx every arc streams forward
-  const bool rev = c.direction == GARY_PIVOT_MIN_DIRECTION && rev_supported(m, g.csr.nnz) &&
-                   g.coo.rowIdx != nullptr && g.coo.nnz == g.csr.nnz;
-  // (the reversed-arc list lives after the pivot lists in the same per-stream buffer)
+  const bool rev = rev_wanted(c.direction, g);
   GARY_TRY(workspace_get(dev, st, core + (rev ? rev_bytes(m, g.csr.nnz) : 0), &ws));
   int32_t *lists = (int32_t *)ws;
   RevBufs rb;

@@ -28,9 +28,11 @@ CHUNKED = 1
 DYNAMIC = -1  # WorkPartition::DYNAMIC: chunks of edges claimed from an atomic cursor
 
 # Arc direction of the pivot strategies: each arc streamed from the endpoint with the shorter
-# out-list (reverse rows listed per call), or every N+(v) at pivot u.
-MIN_DIRECTION = 0
+# out-list (reverse rows grouped per call), every N+(v) at pivot u, or AUTO (the former from 2^20
+# arcs on).
+AUTO_DIRECTION = 0
 FORWARD = 1
+MIN_DIRECTION = 2
 
 _P = C.c_void_p
 _bind = _lib.bind
@@ -42,7 +44,7 @@ _bind("gary_index_cache_smem", C.c_int64, [C.c_int32, C.c_int64])
 
 
 def launch_cfg(block=0, grid=0, group=0, cache_depth=0, partition=INTERLEAVED, bin_small=0,
-               bin_hub=0, direction=MIN_DIRECTION) -> LaunchCfg:
+               bin_hub=0, direction=AUTO_DIRECTION) -> LaunchCfg:
     return LaunchCfg(block, grid, group, cache_depth, partition, bin_small, bin_hub, direction)
 
 

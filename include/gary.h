@@ -265,9 +265,12 @@ enum {
 /* Arc direction of the pivot strategies (BITMAP and the bitmap route of HEURISTIC /
  * AUTO_LEARNED). MIN_DIRECTION streams each arc (u, v) from the endpoint with the shorter
  * out-list: N+(u) through the set of N+(v) at pivot v when d+(u) is shorter by more than a small
- * slack, which needs a per-call list of those arcs grouped by v (about 4 bytes per arc of
- * scratch, nnz < 2^32). FORWARD streams every N+(v) at pivot u, with no such list. */
-enum { GARY_PIVOT_MIN_DIRECTION = 0, GARY_PIVOT_FORWARD = 1 };
+ * slack, which needs a per-call grouping of those arcs by v (16 bytes per arc and 9 per vertex
+ * of scratch; the graph's COO rowIdx; m <= 2^24 and nnz < 2^32). FORWARD streams every N+(v) at pivot u, with
+ * no grouping. AUTO (the default) is MIN_DIRECTION from 2^20 arcs on and FORWARD below, where the
+ * grouping's fixed cost of a few launches outweighs what it saves. Counts are the same in every
+ * mode. */
+enum { GARY_PIVOT_AUTO = 0, GARY_PIVOT_FORWARD = 1, GARY_PIVOT_MIN_DIRECTION = 2 };
 
 /* Launch knobs. Zero fields take the library default (or the autotuned value). */
 typedef struct gary_launch_cfg {
@@ -278,7 +281,7 @@ typedef struct gary_launch_cfg {
   int32_t partition;   /* GARY_PARTITION_* for the edge-centric strategies */
   int32_t bin_small;   /* pivots with out-degree <= bin_small use the small-table wave kernel */
   int32_t bin_hub;     /* pivots with out-degree > bin_hub get a whole workgroup */
-  int32_t direction;   /* GARY_PIVOT_* for the pivot strategies (0 = MIN_DIRECTION) */
+  int32_t direction;   /* GARY_PIVOT_* for the pivot strategies (0 = AUTO) */
 } gary_launch_cfg;
 
 /* Elements (not bytes) of LDS index cache for a workgroup of `block` threads with depth

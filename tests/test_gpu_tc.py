@@ -327,11 +327,12 @@ def test_reverse_rows_every_pivot_path(direction):
             assert gq.triangleCountHom(ctx, s, cfg) == want, (s, bins)
 
 
-@pytest.mark.parametrize("direction", [gq.MIN_DIRECTION, gq.FORWARD])
+@pytest.mark.parametrize("direction", [gq.MIN_DIRECTION, gq.FORWARD, gq.AUTO_DIRECTION])
 @pytest.mark.parametrize("strategy", [gq.BITMAP, gq.HEURISTIC, gq.AUTO_LEARNED])
 def test_pivot_directions(golden_ctxs, rmat16, er_ctx, direction, strategy):
-    """Both arc directions of the pivot path give the oracle's count: forward (every N+(v) at
-    pivot u) and min-direction (arcs into pivots with much longer out-lists streamed there)."""
+    """Every arc direction of the pivot path gives the oracle's count: forward (every N+(v) at
+    pivot u), min-direction (arcs into pivots with much longer out-lists streamed there) and
+    auto (min-direction from 2^20 arcs on: forward on these graphs)."""
     cfg = gq.launch_cfg(direction=direction)
     for rec, ctx in golden_ctxs:
         assert gq.triangleCountHom(ctx, strategy, cfg) == rec["triangles"], rec["name"]
