@@ -52,7 +52,7 @@ int scratch_ptr(int dev, int32_t **out) {
 // Largest vertex count: 3 * round_up(m, 64) + kPivotCounterWords <= kScratchWords.
 constexpr int64_t kMaxVertices =
     (int64_t)(((kScratchWords - gary::kPivotCounterWords) / 3) & ~(size_t)63);
-static_assert(kMaxVertices == 1397760, "the limit stated in gary.h");
+static_assert(kMaxVertices == 1397568, "the limit stated in gary.h");
 
 extern "C" {
 

@@ -27,7 +27,7 @@ constexpr int kRouteBins = 256;      // (min(15, log2 d+(u)), min(15, log2 d+(v)
 // those up to bin_hub get a workgroup of `waves` waves, larger ones the hub kernel; grid = 0
 // takes the default (2048 workgroups).
 // int32 words of pivot-path counters that follow the three pivot lists (3 * stride words).
-constexpr int kPivotCounterWords = 1024;
+constexpr int kPivotCounterWords = 1536;
 
 // Default out-degree split between per-wave and per-workgroup pivots (measured best on
 // R-MAT 20 with the heavy-first pivot order).

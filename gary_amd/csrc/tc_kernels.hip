@@ -562,12 +562,21 @@ __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
 //   shared set of TABLE_WORDS words and its WAVES waves split the rows.
 // Phase 2: each wave takes a few pivots of `small` at a time and works alone with a
 //   1/WAVES share of the table memory.
-// counts[0] = #small, counts[1] = #big; cursors[0..1] start at zero.
+// counts[0] = #small, counts[1] = #big; cursors[1] (the big list's) and the small list's striped
+// cursors start at zero.
 // Per-graph mode (gid_of != nullptr, batched disjoint unions): each wave adds its partial count
 // to gcount[gid_of[u]] whenever the pivot's graph changes, instead of to *count.
 // The small list is ordered heaviest first; its first split[0] pivots (weight bucket >=
 // kLightLog) are claimed H at a time, the rest Lc at a time (below).
-// Claim sizes (pivots per atomic on the small list's cursor), forward / with reversed rows. A
+// Small-pivot claims are striped over kClaimStripes cursors kStripeSpacing words apart: claim k
+// belongs to stripe k % kClaimStripes, whose cursor hands out k / kClaimStripes; a wave starts at
+// stripe blockIdx.x % kClaimStripes and moves on when its stripe is exhausted. With one cursor,
+// 8,192 waves taking short pivots queue on it: claims alone (no pivot work) took 0.46 ms of an
+// R-MAT 20 launch; striped, the launch went 1.99 -> 1.90 ms (Chung-Lu 4M 7.12 -> 6.74 ms). Checking
+// an exhausted stripe with a coherent load before its atomic cost more than it saved (2.17 ms).
+constexpr int kClaimStripes = 8;
+constexpr int kStripeSpacing = 64;  // words
+// Claim sizes (pivots per atomic on the small list's cursors), forward / with reversed rows. A
 // claim costs a fetch-add serialised with every other wave's on one address (about 10 ns each)
 // and one chain of header loads; with reversed rows the small pivots stream little (their arcs
 // to much longer out-lists are streamed at the other end), so larger claims pay: R-MAT 20
@@ -604,7 +613,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     const int32_t *__restrict__ gid_of, unsigned long long *gcount,
                     const uint8_t *__restrict__ route, const RouteTable *__restrict__ rtable,
                     const int32_t *__restrict__ split, const uint2 *__restrict__ rrange,
-                    const uint2 *__restrict__ rrows) {
+                    const uint2 *__restrict__ rrows, int32_t *__restrict__ stripes) {
   __shared__ int32_t table[TABLE_WORDS];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   __shared__ RowSlots s_rows[WAVES];
@@ -655,9 +664,19 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
   const int32_t Lc = rrange ? kSmallBatchLightRev : kSmallBatchLight;
   const int32_t n_heavy = min(split[0], n_small);
   const int32_t heavy_claims = (n_heavy + H - 1) / H;
+  const int32_t all_claims = heavy_claims + (n_small - n_heavy + Lc - 1) / Lc;
   while (true) {
-    int32_t k = 0;
-    if (lane == 0) k = atomicAdd(&cursors[0], 1);
+    int32_t k = all_claims;
+    if (lane == 0 && all_claims > 0) {
+      for (int t = 0; t < kClaimStripes; ++t) {
+        const int x = (int)((blockIdx.x + t) & (kClaimStripes - 1));
+        const int64_t kk = x + (int64_t)kClaimStripes * atomicAdd(&stripes[x * kStripeSpacing], 1);
+        if (kk < all_claims) {
+          k = (int32_t)kk;
+          break;
+        }
+      }
+    }
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
     const int32_t base = k < heavy_claims ? k * H : n_heavy + (k - heavy_claims) * Lc;
     if (base >= n_small) break;
@@ -754,7 +773,12 @@ constexpr int kCurOff = kHistOff + kBinCopies * kKeys;
 // [kSplitOff] heavy/light split of the small list; [+2] the same for the hub launch (whose
 // small list is empty: always 0)
 constexpr int kSplitOff = kCurOff + kBinCopies * kKeys;
-static_assert(kSplitOff + 4 <= kPivotCounterWords, "pivot counter words");
+// [kStripeOff, + kClaimStripes * kStripeSpacing) the small list's striped claim cursors (the hub
+// launch, whose small list is empty, makes no claim on them)
+constexpr int kStripeOff = 1024;
+static_assert(kSplitOff + 4 <= kStripeOff &&
+                  kStripeOff + kClaimStripes * kStripeSpacing <= kPivotCounterWords,
+              "pivot counter words");
 
 // ordered == false (per-graph counting of batched unions) puts every small pivot in the
 // lightest bucket: the small list then keeps vertex order, so consecutive pivots of a wave mostly
@@ -1628,22 +1652,22 @@ int launch_pivot_core(const gary_device_graph &g, const PivotCfg &pc,
   // hubs: small list empty (counters[2] == 0), big list = bin 2
   tc_pivot_kernel<kHubWaves, kHubWords><<<kHubGrid, kHubWaves * 64, 0, st>>>(
       g, lists, lists + 2 * stride, counters + 2, counters + 6, cnt, gid_of,
-      gcount, route, table, counters + kSplitOff + 2, rrange, rrows);
+      gcount, route, table, counters + kSplitOff + 2, rrange, rrows, counters + kStripeOff);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_begin(st));
   const int grid = pc.grid ? pc.grid : 2048;
   if (pc.waves == 2)
     tc_pivot_kernel<2, kPivotWords><<<grid, 2 * 64, 0, st>>>(
         g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
-        table, counters + kSplitOff, rrange, rrows);
+        table, counters + kSplitOff, rrange, rrows, counters + kStripeOff);
   else if (pc.waves == 8)
     tc_pivot_kernel<8, kPivotWords><<<grid, 8 * 64, 0, st>>>(
         g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
-        table, counters + kSplitOff, rrange, rrows);
+        table, counters + kSplitOff, rrange, rrows, counters + kStripeOff);
   else
     tc_pivot_kernel<4, kPivotWords><<<grid, 4 * 64, 0, st>>>(
         g, lists, lists + stride, counters, counters + 4, cnt, gid_of, gcount, route,
-        table, counters + kSplitOff, rrange, rrows);
+        table, counters + kSplitOff, rrange, rrows, counters + kStripeOff);
   GARY_HIP(hipGetLastError());
   if (timed) GARY_TRY(timing_end(st));
   return GARY_OK;

@@ -306,7 +306,7 @@ int gary_tc_count_legacy(const gary_device_graph *g, uint32_t *d_count, void *st
  * stream capture, tight serving loops, hosts that forbid hipMalloc): gary_tc_count with
  * strategy GARY_TC_BITMAP (cfg: block, grid, bin_small, bin_hub as there; may be NULL), whose
  * pivot lists and counters live in a fixed 16 MiB device array that each GPU holds until the
- * process ends, so the call allocates nothing. Graphs of more than 1,397,760 vertices fail with
+ * process ends, so the call allocates nothing. Graphs of more than 1,397,568 vertices fail with
  * GARY_ERR_INVALID. Every call on one GPU shares that array: calls that overlap on one GPU (two
  * streams, two threads) race. Accumulates into *d_count on `stream`, like gary_tc_count. */
 int gary_tc_count_noalloc(const gary_device_graph *g, const gary_launch_cfg *cfg,

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 _lib.bind("gary_tc_count_noalloc", C.c_int, [C.POINTER(DeviceGraph), C.POINTER(LaunchCfg),
                                              C.c_void_p, C.c_void_p])
-MAX_VERTICES = 1_397_760
+MAX_VERTICES = 1_397_568
 
 
 def _noalloc(ctx, d_count, cfg=None):
