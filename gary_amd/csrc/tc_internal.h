@@ -56,7 +56,8 @@ constexpr int kRevChunk = 8192;         // arcs of a tile staged in LDS at a tim
 
 struct RevShape {
   int32_t lg = 9;        // log2 of the bucket count
-  int32_t ids = 1;       // ids per bucket: ceil(m / 2^lg)
+  int32_t k = 9;         // bits of the hashed id: ceil(log2 m), at least lg
+  int32_t ids = 1;       // ids per bucket: 2^(k - lg)
   int32_t tiles = 1;     // arc ranges of the two tile passes
   int64_t tile_arcs = 1; // arcs per tile (a multiple of kRevChunk; the last tile may hold fewer)
 };

@@ -927,14 +927,11 @@ __global__ void __launch_bounds__(256)
 //   2. an exclusive scan of ghist gives every (bucket, tile) its first position;
 //   3. rev_place_kernel: the tile reads its arcs again, chunk by chunk, counting-sorts each
 //      chunk's reversed arcs by bucket in LDS and writes them, in runs, as 64-bit words
-//      (v >> lg) << 51 | d+(u) << 32 | indPtr[u] (h < 2^13 ids per bucket, d+(u) < 2^19);
-//   4. rev_group_kernel: one workgroup per bucket counts its pairs per target in LDS, scans the
+//      h << 51 | d+(u) << 32 | indPtr[u] (h < 2^13 ids per bucket, d+(u) < 2^19);
+//   4. rev_group_kernel: one workgroup per bucket counts its words per target in LDS, scans the
 //      counts, writes rrange[v] for every v of the bucket and places each source's row (start,
 //      length).
-// Bucket of v: the low lg bits of v ^ (v >> lg) ^ (v >> 2 lg). Given the bucket and h = v >> lg,
-// v's low bits are bucket ^ h ^ (h >> lg), so a bucket holds at most ceil(m / 2^lg) ids, indexed
-// densely by h; and ids with few set bits (R-MAT's hubs) spread over the buckets, where buckets
-// by id range put 688 K of the 11.1 M reversed arcs of R-MAT 20 in one bucket.
+// Buckets and ids within them come from a multiplicative hash of the target (rev_hash).
 // The tile passes read each arc's source from the COO rowIdx, so the grouping needs the COO
 // (without it the pivot path streams every arc forward). Walking 64-row batches of the CSR
 // instead, with rows found by a slot write and a max-scan per 64 arcs, took 174 + 238 us for
@@ -949,8 +946,21 @@ static_assert(kRevMaxBuckets <= 65536 && kRevMaxBucketIds <= (1 << 13) &&
                   kRevMaxSourceDeg <= (1 << 19),
               "a pair word holds h in 13 bits and d+(u) in 19; a bucket id fits 16 bits");
 
-__device__ __forceinline__ uint32_t rev_bucket(uint32_t v, int lg) {
-  return (v ^ (v >> lg) ^ (v >> (2 * lg))) & ((1u << lg) - 1u);
+// Hashed id of target v: x = v * kRevMul mod 2^k, a bijection on [0, 2^k); its top lg bits are
+// the bucket, its low k - lg bits the dense index h within the bucket, and v = x * kRevMulInv
+// mod 2^k. The product mixes all bits of v into the top ones, so R-MAT's hubs (ids with few set
+// bits) spread over the buckets: on R-MAT 20 the largest bucket holds 26.5 K of 7.7 M reversed
+// arcs (mean 15 K); folding the id's bits (v ^ v >> lg ^ v >> 2 lg) left 84 K in one bucket and
+// the grouping pass at 125 us, and buckets by id range 688 K of 11.1 M arcs (slack 8).
+constexpr uint32_t kRevMul = 0x9E3779B1u;
+constexpr uint32_t kRevMulInv = 0x0E8B2F51u;  // kRevMul * kRevMulInv = 1 mod 2^32
+static_assert(kRevMul * kRevMulInv == 1u, "multiplicative inverse mod 2^32");
+
+__device__ __forceinline__ uint32_t rev_hash(uint32_t v, const RevShape &sh) {
+  return (v * kRevMul) & ((sh.k == 32 ? 0u : (1u << sh.k)) - 1u);
+}
+__device__ __forceinline__ uint32_t rev_bucket(uint32_t v, const RevShape &sh) {
+  return rev_hash(v, sh) >> (sh.k - sh.lg);
 }
 
 // dsat[v] = min(d+(v), 255): the reversal test reads one byte per arc (an array of m bytes stays
@@ -1011,7 +1021,7 @@ __global__ void __launch_bounds__(kRevBlock)
     rev_load_chunk(g, route, dsat, e, e_end, u, v, du);
 #pragma unroll
     for (int q = 0; q < kRevPer; ++q)
-      if (v[q] >= 0) atomicAdd(&s_hist[rev_bucket((uint32_t)v[q], sh.lg)], 1u);
+      if (v[q] >= 0) atomicAdd(&s_hist[rev_bucket((uint32_t)v[q], sh)], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kRevBlock)
@@ -1050,7 +1060,7 @@ __global__ void __launch_bounds__(kRevBlock)
     for (int q = 0; q < kRevPer; ++q) pu[q] = v[q] >= 0 ? (uint32_t)g.csr.indPtr[u[q]] : 0u;
 #pragma unroll
     for (int q = 0; q < kRevPer; ++q)
-      if (v[q] >= 0) atomicAdd(&s_fill[rev_bucket((uint32_t)v[q], sh.lg)], 1u);
+      if (v[q] >= 0) atomicAdd(&s_fill[rev_bucket((uint32_t)v[q], sh)], 1u);
     __syncthreads();
     // exclusive scan of the nb bucket counts (each thread owns per consecutive ones)
     const int per = (nb + kRevBlock - 1) / kRevBlock;
@@ -1077,9 +1087,11 @@ __global__ void __launch_bounds__(kRevBlock)
 #pragma unroll
     for (int q = 0; q < kRevPer; ++q)
       if (v[q] >= 0) {
-        const uint32_t b = rev_bucket((uint32_t)v[q], sh.lg);
+        const uint32_t x = rev_hash((uint32_t)v[q], sh);
+        const uint32_t b = x >> (sh.k - sh.lg);
         const uint32_t at = atomicAdd(&s_fill[b], 1u);
-        s_word[at] = ((uint64_t)((uint32_t)v[q] >> sh.lg) << 51) | ((uint64_t)du[q] << 32) | pu[q];
+        s_word[at] = ((uint64_t)(x & (uint32_t)(sh.ids - 1)) << 51) | ((uint64_t)du[q] << 32) |
+                     pu[q];
         s_bk[at] = (uint16_t)b;
       }
     __syncthreads();
@@ -1129,14 +1141,14 @@ __global__ void __launch_bounds__(kRevGroupBlock)
   __syncthreads();
   uint32_t run = beg + incl - sum;
   for (int q = 0; q < (int)(threadIdx.x / kWave); ++q) run += s_wave[q];
-  const uint32_t lmask = (1u << sh.lg) - 1u;
+  const uint32_t kmask = (sh.k == 32 ? 0u : (1u << sh.k)) - 1u;
   for (int q = 0; q < per; ++q) {
     const int hq = i0 + q;
     if (hq >= sh.ids) break;
     const uint32_t c = s_cnt[hq];
     s_cnt[hq] = run;  // the list start of this id, then its fill cursor
-    const uint32_t lo = (bk ^ (uint32_t)hq ^ ((uint32_t)hq >> sh.lg)) & lmask;
-    const int64_t v = ((int64_t)hq << sh.lg) | lo;
+    const uint32_t x = (bk << (sh.k - sh.lg)) | (uint32_t)hq;
+    const int64_t v = (int64_t)((x * kRevMulInv) & kmask);
     if (v < m) rrange[v] = make_uint2(run, run + c);
     run += c;
   }
@@ -1539,7 +1551,8 @@ RevShape rev_shape(int32_t m, int64_t nnz) {
   int lg = 1;
   while ((1ll << lg) < (int64_t)m) ++lg;
   r.lg = std::min(11, std::max(9, lg - 13));
-  r.ids = (int32_t)(((int64_t)m + (1ll << r.lg) - 1) >> r.lg);
+  r.k = std::max(lg, r.lg);
+  r.ids = 1 << (r.k - r.lg);
   const int64_t chunks = (nnz + kRevChunk - 1) / kRevChunk;
   r.tiles = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4096, chunks));
   r.tile_arcs = (chunks + r.tiles - 1) / r.tiles * kRevChunk;
