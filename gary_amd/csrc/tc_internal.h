@@ -52,7 +52,7 @@ constexpr int32_t kRevSlack = 128;
 constexpr int32_t kRevMaxSourceDeg = 1 << 19;  // sources with a longer out-list stay forward
 constexpr int kRevMaxBuckets = 2048;    // LDS counters per tile
 constexpr int kRevMaxBucketIds = 8192;  // LDS counters per bucket (ids per bucket)
-constexpr int kRevChunk = 8192;         // arcs of a tile staged in LDS at a time
+constexpr int kRevChunk = 4096;         // arcs of a tile staged in LDS at a time
 
 struct RevShape {
   int32_t lg = 9;        // log2 of the bucket count
