@@ -245,20 +245,31 @@ struct PivotSet {
   uint32_t emask;  // bitmap: (2^(s-3) - 1) & ~3, the in-block byte offset bits of x >> 3
 };
 
+// Words of the bitmap at block size 2^s, and their step words(s + 1) - words(s). The step
+// strictly increases with s (the exact blocks' d * 2^(s-5) doubles while the coarse words' drop
+// ceil(span / 2^(s+4)) - ceil(span / 2^(s+5)) never grows), so the least words over s in [5, 20]
+// are at the first s whose step is >= 0 (or at 20), the same s a scan of every s picks (first
+// minimum). The search starts at the continuous optimum, 2^(2s-1) = span / d, and moves at most
+// a step or two: scanning all 16 block sizes was 0.16 ms of scalar work per R-MAT 20 launch.
+__device__ __forceinline__ int64_t plan_words(uint32_t span, int32_t d, int32_t s) {
+  return (((int64_t)span + (1ll << (s + 4)) - 1) >> (s + 4)) + 1 + ((int64_t)d << (s - 5));
+}
+__device__ __forceinline__ int64_t plan_step(uint32_t span, int32_t d, int32_t s) {
+  const int64_t drop = (((int64_t)span + (1ll << (s + 4)) - 1) >> (s + 4)) -
+                       (((int64_t)span + (1ll << (s + 5)) - 1) >> (s + 5));
+  return ((int64_t)d << (s - 5)) - drop;
+}
+
 __device__ __forceinline__ bool pivot_plan(int32_t lo, int32_t hi, int32_t d, int32_t table_words,
                                            PivotSet &ps) {
   ps.base = lo;
   ps.span = (uint32_t)(hi - lo) + 1u;
-  int32_t best_s = 5;
-  int64_t best = INT64_MAX;
-  for (int32_t s = 5; s <= 20; ++s) {
-    const int64_t cw = ((int64_t)ps.span + (1ll << (s + 4)) - 1) >> (s + 4);
-    const int64_t w = cw + 1 + ((int64_t)d << (s - 5));  // + the zero word after the coarse words
-    if (w < best) {
-      best = w;
-      best_s = s;
-    }
-  }
+  // bit lengths instead of a division: log2(span / d) to within one
+  int32_t best_s = ((32 - __clz((int)ps.span)) - (32 - __clz(max(d, 1))) + 2) >> 1;
+  best_s = min(max(best_s, 5), 20);
+  while (best_s > 5 && plan_step(ps.span, d, best_s - 1) >= 0) --best_s;
+  while (best_s < 20 && plan_step(ps.span, d, best_s) < 0) ++best_s;
+  const int64_t best = plan_words(ps.span, d, best_s);
   if (best <= table_words) {
     ps.s = best_s;
     ps.cwords = (int32_t)(((int64_t)ps.span + (1ll << (best_s + 4)) - 1) >> (best_s + 4));
@@ -335,6 +346,47 @@ __device__ __forceinline__ void pivot_build(int32_t *t, const PivotSet &ps,
     atomicOr(&exact[(r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)], 1u << (x & 31));
   }
   pivot_sync<BLOCK>();
+}
+
+// pivot_build for one wave and a list of at most kWave elements held in registers: e = L[lane],
+// -1 past the end of the list.
+__device__ __forceinline__ void pivot_build_reg(int32_t *t, const PivotSet &ps, int32_t e,
+                                                int lane) {
+  const int32_t fillv = ps.log_t ? -1 : 0;
+  for (int32_t i = lane; i < ps.words; i += kWave) t[i] = fillv;
+  wave_sync();
+  if (ps.log_t) {
+    if (e >= 0) {
+      const uint32_t mask = (1u << ps.log_t) - 1;
+      uint32_t h = hash_slot(e, ps.log_t);
+      while (atomicCAS(&t[h], -1, e) != -1) h = (h + 1) & mask;
+    }
+    wave_sync();
+    return;
+  }
+  uint32_t *coarse = (uint32_t *)t;
+  uint32_t *exact = (uint32_t *)(t + ps.cwords + 1);
+  const int32_t s = ps.s;
+  const uint32_t x = (uint32_t)(e - ps.base);
+  const bool in = e >= 0 && x < ps.span;
+  if (in) atomicOr(&coarse[x >> (s + 4)], 1u << ((x >> s) & 15));
+  wave_sync();
+  uint32_t carry = 0;  // rank prefix of the set coarse bits, into the high half of each word
+  for (int32_t c0 = 0; c0 < ps.cwords; c0 += kWave) {
+    const int32_t i = c0 + lane;
+    const uint32_t w = i < ps.cwords ? coarse[i] : 0u;
+    const uint32_t v = (uint32_t)__popc(w);
+    const uint32_t incl = wave_inclusive_scan_u32(v);
+    if (i < ps.cwords) coarse[i] = w | ((carry + incl - v) << 16);
+    carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+  }
+  wave_sync();
+  if (in) {
+    const uint32_t word = coarse[x >> (s + 4)], b = (x >> s) & 15;
+    const uint32_t r = (word >> 16) + (uint32_t)__popc(word & ((1u << b) - 1));
+    atomicOr(&exact[(r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)], 1u << (x & 31));
+  }
+  wave_sync();
 }
 
 __device__ __forceinline__ uint32_t pivot_probe(const int32_t *t, const PivotSet &ps, int32_t w) {
@@ -428,6 +480,72 @@ __device__ __forceinline__ uint32_t probe_chunk(const int4 (&data)[CH / 4], int3
   }
 }
 
+// One 64-row batch of stream_rows: lane l holds the (start, length) of its row (length 0: no
+// row). `tag` carries the slot tags from batch to batch of one pivot (1 << 30 before the first).
+__device__ __forceinline__ uint32_t stream_batch(const int32_t *__restrict__ col, uint64_t nnz,
+                                                 int64_t pv, int32_t dv, const int32_t *table,
+                                                 const PivotSet &ps, bool hash, RowSlots &rs,
+                                                 int lane, int32_t &tag) {
+  constexpr int CH = kChunk;
+  constexpr int LG = 3;
+  uint32_t c = 0;
+  const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + CH - 1) >> LG) - (pv >> LG)) : 0;
+  const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
+  const int32_t start = incl - nch;
+  const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+  // chunk of batch position x: base_r + x; its valid slots: [A - CH*x, A + dv - CH*x)
+  // (unsigned: the terms may wrap, the results for the row's own positions fit int32)
+  const uint32_t base_r = (uint32_t)(pv >> LG) - (uint32_t)start;
+  const uint32_t A = (uint32_t)(pv & (CH - 1)) + (uint32_t)CH * (uint32_t)start;
+  const uint32_t Bv = A + (uint32_t)dv;
+  for (int32_t w0 = 0; w0 < total; w0 += kWave) {
+    if (tag >= (1 << 30)) {  // stale slots could outrank new tags: clear them
+      rs.s[lane] = -1;
+      tag = 0;
+      wave_sync();
+    }
+    // slot of the window position where each row with chunks here begins (0 for the row
+    // running into the window), tagged so that slots left by earlier windows lose the max
+    if (nch > 0 && start < w0 + kWave && incl > w0) rs.s[max(0, start - w0)] = tag | lane;
+    wave_sync();
+    const int32_t row = wave_inclusive_max_i32(rs.s[lane]) & (kWave - 1);
+    wave_sync();
+    tag += kWave;
+    const int32_t x = w0 + lane;
+    const uint32_t ch = (uint32_t)__shfl((int32_t)base_r, row, kWave) + (uint32_t)x;
+    const int32_t jlo = (int32_t)((uint32_t)__shfl((int32_t)A, row, kWave) - (uint32_t)CH * (uint32_t)x);
+    int32_t jhi = (int32_t)((uint32_t)__shfl((int32_t)Bv, row, kWave) - (uint32_t)CH * (uint32_t)x);
+    int4 data[CH / 4];
+#pragma unroll
+    for (int h = 0; h < CH / 4; ++h) data[h] = make_int4(0, 0, 0, 0);
+    if (x < total) {
+      const uint64_t e0 = (uint64_t)ch << LG;
+      data[0] = *reinterpret_cast<const int4 *>(col + e0);
+#pragma unroll
+      for (int h = 1; h < CH / 4; ++h)
+        if (e0 + 4 * h < nnz) data[h] = *reinterpret_cast<const int4 *>(col + e0 + 4 * h);
+    } else {
+      jhi = 0;
+    }
+    c += hash ? probe_chunk<CH, true>(data, jlo, jhi, table, ps)
+              : probe_chunk<CH, false>(data, jlo, jhi, table, ps);
+  }
+  return c;
+}
+
+// Row header of forward row r of pivot list L: (row start, length) of N+(L[r]), length 0 when
+// the arc is routed elsewhere or reversed (d+(v) > dmax: counted at pivot v).
+__device__ __forceinline__ void forward_row(const gary_device_graph &g, int32_t v, int32_t dmax,
+                                            bool routed, int64_t &pv, int32_t &dv) {
+  pv = 0;
+  dv = 0;
+  if (routed && (uint32_t)v < (uint32_t)g.csr.m) {
+    dv = g.csr.deg[v];
+    pv = g.csr.indPtr[v];
+    if (dv > dmax) dv = 0;
+  }
+}
+
 // The rows r0, r0 + stride, ... (< d) of pivot list L, 64 rows per batch. A batch is flattened
 // into the aligned chunks of kChunk elements (32 bytes) that overlap its rows; each window of 64
 // chunks gives lane l chunk w0 + l, so a wave's loads cover one contiguous stretch of colIdx.
@@ -455,11 +573,7 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
                                                 const int32_t *table, const PivotSet &ps,
                                                 RowSlots &rs, int lane,
                                                 const uint8_t *__restrict__ Lroute) {
-  constexpr int CH = kChunk;
-  constexpr int LG = 3;
-  const uint32_t m = (uint32_t)g.csr.m;
   const uint64_t nnz = (uint64_t)g.csr.nnz;
-  const int32_t *__restrict__ col = g.csr.colIdx;
   const bool hash = ps.log_t != 0;
   const int32_t dtot = d + d2;
   uint32_t c = 0;
@@ -469,60 +583,13 @@ __device__ __forceinline__ uint32_t stream_rows(const gary_device_graph &g,
     int32_t dv = 0;
     int64_t pv = 0;
     if (r < d) {
-      if (!Lroute || Lroute[r] == kRouteBitmap) {
-        const int32_t v = L[r];
-        if ((uint32_t)v < m) {
-          dv = g.csr.deg[v];
-          pv = g.csr.indPtr[v];
-          if (dv > dmax) dv = 0;  // reversed: counted at pivot v
-        }
-      }
+      forward_row(g, L[r], dmax, !Lroute || Lroute[r] == kRouteBitmap, pv, dv);
     } else if (r < dtot) {
       const uint2 rw = R2[r - d];
       pv = rw.x;
       dv = (int32_t)rw.y;
     }
-    const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + CH - 1) >> LG) - (pv >> LG)) : 0;
-    const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
-    const int32_t start = incl - nch;
-    const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
-    // chunk of batch position x: base_r + x; its valid slots: [A - CH*x, A + dv - CH*x)
-    // (unsigned: the terms may wrap, the results for the row's own positions fit int32)
-    const uint32_t base_r = (uint32_t)(pv >> LG) - (uint32_t)start;
-    const uint32_t A = (uint32_t)(pv & (CH - 1)) + (uint32_t)CH * (uint32_t)start;
-    const uint32_t Bv = A + (uint32_t)dv;
-    for (int32_t w0 = 0; w0 < total; w0 += kWave) {
-      if (tag >= (1 << 30)) {  // stale slots could outrank new tags: clear them
-        rs.s[lane] = -1;
-        tag = 0;
-        wave_sync();
-      }
-      // slot of the window position where each row with chunks here begins (0 for the row
-      // running into the window), tagged so that slots left by earlier windows lose the max
-      if (nch > 0 && start < w0 + kWave && incl > w0) rs.s[max(0, start - w0)] = tag | lane;
-      wave_sync();
-      const int32_t row = wave_inclusive_max_i32(rs.s[lane]) & (kWave - 1);
-      wave_sync();
-      tag += kWave;
-      const int32_t x = w0 + lane;
-      const uint32_t ch = (uint32_t)__shfl((int32_t)base_r, row, kWave) + (uint32_t)x;
-      const int32_t jlo = (int32_t)((uint32_t)__shfl((int32_t)A, row, kWave) - (uint32_t)CH * (uint32_t)x);
-      int32_t jhi = (int32_t)((uint32_t)__shfl((int32_t)Bv, row, kWave) - (uint32_t)CH * (uint32_t)x);
-      int4 data[CH / 4];
-#pragma unroll
-      for (int h = 0; h < CH / 4; ++h) data[h] = make_int4(0, 0, 0, 0);
-      if (x < total) {
-        const uint64_t e0 = (uint64_t)ch << LG;
-        data[0] = *reinterpret_cast<const int4 *>(col + e0);
-#pragma unroll
-        for (int h = 1; h < CH / 4; ++h)
-          if (e0 + 4 * h < nnz) data[h] = *reinterpret_cast<const int4 *>(col + e0 + 4 * h);
-      } else {
-        jhi = 0;
-      }
-      c += hash ? probe_chunk<CH, true>(data, jlo, jhi, table, ps)
-                : probe_chunk<CH, false>(data, jlo, jhi, table, ps);
-    }
+    c += stream_batch(g.csr.colIdx, nnz, pv, dv, table, ps, hash, rs, lane, tag);
   }
   return c;
 }
@@ -625,6 +692,8 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
   const int32_t n_small = counts[0], n_big = counts[1];
   if (n_small == 0 && n_big == 0) return;
   const uint32_t m = (uint32_t)g.csr.m;
+  const int32_t *__restrict__ col = g.csr.colIdx;
+  const uint64_t nnz = (uint64_t)g.csr.nnz;
   uint64_t c = 0;
   // ---- phase 1: big pivots, one workgroup each
   while (true) {
@@ -702,7 +771,30 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
         hhi = g.csr.colIdx[hp + hd - 1];
       }
     }
+    // Pivots without reverse rows and with at most kWave out-neighbours are pipelined: the next
+    // such pivot's out-list is loaded (one element per lane) while this one is built and streamed, and the row headers of this one's out-neighbours are loaded before its set is
+    // built, so a pivot waits on two dependent global loads (row headers, then chunks) instead of
+    // four (out-list, out-list again for the rows, row headers, chunks).
+    auto readlane64 = [&](int64_t x, int32_t jj) {
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(x >> 32), jj);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)x, jj);
+      return (int64_t)(((uint64_t)hi << 32) | lo);
+    };
+    auto piped = [&](int32_t jj) {
+      const int32_t dj = __builtin_amdgcn_readlane(hd, jj);
+      const int32_t rj = __builtin_amdgcn_readlane((int32_t)hr.y, jj) -
+                         __builtin_amdgcn_readlane((int32_t)hr.x, jj);
+      return dj >= 2 && dj <= kWave && rj == 0;
+    };
+    auto prefetch = [&](int32_t jj) {
+      if (jj >= cnt || !piped(jj)) return -1;
+      const int32_t dj = __builtin_amdgcn_readlane(hd, jj);
+      return lane < dj ? col[readlane64(hp, jj) + lane] : -1;
+    };
+    int32_t nx = prefetch(0);
     for (int32_t j = 0; j < cnt; ++j) {
+      const int32_t e = nx;
+      nx = prefetch(j + 1);
       const int32_t d = __builtin_amdgcn_readlane(hd, j);
       if (d < 2) continue;
       if (gid_of) {
@@ -713,10 +805,8 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
           cur = gu;
         }
       }
-      const uint32_t phi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(hp >> 32), j);
-      const uint32_t plo = (uint32_t)__builtin_amdgcn_readlane((int32_t)hp, j);
-      const int64_t pu = (int64_t)(((uint64_t)phi << 32) | plo);
-      const int32_t *L = g.csr.colIdx + pu;
+      const int64_t pu = readlane64(hp, j);
+      const int32_t *L = col + pu;
       const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readlane((int32_t)hr.x, j);
       const int32_t d2 = (int32_t)((uint32_t)__builtin_amdgcn_readlane((int32_t)hr.y, j) - rb0);
       const uint2 *L2 = rrange ? rrows + rb0 : nullptr;
@@ -728,8 +818,17 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
         c += pivot_fallback(g, L, d, L2, d2, dmax, 0, 1, lane, Lroute);
         continue;
       }
-      pivot_build<false>(wt, s, L, d, lane, kWave);
-      c += stream_rows(g, L, d, L2, d2, dmax, 0, 1, wt, s, rb, lane, Lroute);
+      if (piped(j)) {
+        int64_t pv;
+        int32_t dv;
+        forward_row(g, e, dmax, e >= 0 && (!Lroute || Lroute[lane] == kRouteBitmap), pv, dv);
+        pivot_build_reg(wt, s, e, lane);
+        int32_t tag = 1 << 30;
+        c += stream_batch(col, nnz, pv, dv, wt, s, s.log_t != 0, rb, lane, tag);
+      } else {
+        pivot_build<false>(wt, s, L, d, lane, kWave);
+        c += stream_rows(g, L, d, L2, d2, dmax, 0, 1, wt, s, rb, lane, Lroute);
+      }
       wave_sync();  // the set is rebuilt for the next pivot
     }
   }
