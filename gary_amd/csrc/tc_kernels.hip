@@ -641,6 +641,8 @@ __device__ __forceinline__ uint32_t pivot_fallback(const gary_device_graph &g,
 // 8,192 waves taking short pivots queue on it: claims alone (no pivot work) took 0.46 ms of an
 // R-MAT 20 launch; striped, the launch went 1.99 -> 1.90 ms (Chung-Lu 4M 7.12 -> 6.74 ms). Checking
 // an exhausted stripe with a coherent load before its atomic cost more than it saved (2.17 ms).
+// A workgroup remembers the stripes its waves found exhausted (an LDS bit each), so at the end
+// of the launch a stripe's cursor takes one failed fetch-add per workgroup rather than per wave.
 constexpr int kClaimStripes = 8;
 constexpr int kStripeSpacing = 64;  // words
 // Claim sizes (pivots per atomic on the small list's cursors), forward / with reversed rows. A
@@ -684,6 +686,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
   __shared__ int32_t table[TABLE_WORDS];
   if (rtable && !rtable->mixed) route = nullptr;  // every bin routes to the bitmap
   __shared__ RowSlots s_rows[WAVES];
+  __shared__ uint32_t s_spent;  // bit x: this workgroup has found claim stripe x exhausted
   // No separate LDS word for the pivot index (it is passed through table[0] between pivots,
   // when the set is dead): every word kept out of LDS helps eight workgroups fit a CU.
   const int lane = lane_id();
@@ -695,6 +698,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
   const int32_t *__restrict__ col = g.csr.colIdx;
   const uint64_t nnz = (uint64_t)g.csr.nnz;
   uint64_t c = 0;
+  if (threadIdx.x == 0) s_spent = 0u;  // published by phase 1's first barrier
   // ---- phase 1: big pivots, one workgroup each
   while (true) {
     if (threadIdx.x == 0) table[0] = atomicAdd(&cursors[1], 1);
@@ -739,11 +743,13 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     if (lane == 0 && all_claims > 0) {
       for (int t = 0; t < kClaimStripes; ++t) {
         const int x = (int)((blockIdx.x + t) & (kClaimStripes - 1));
+        if ((__atomic_load_n(&s_spent, __ATOMIC_RELAXED) >> x) & 1u) continue;
         const int64_t kk = x + (int64_t)kClaimStripes * atomicAdd(&stripes[x * kStripeSpacing], 1);
         if (kk < all_claims) {
           k = (int32_t)kk;
           break;
         }
+        atomicOr(&s_spent, 1u << x);
       }
     }
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
