@@ -1003,7 +1003,7 @@ __global__ void __launch_bounds__(kC4Block)
   __shared__ uint32_t cnt[kC4Words];
   __shared__ int32_t touch[kC4Touch];  // window offsets whose counter left zero
   __shared__ int32_t slots[kC4Waves][kWave];
-  __shared__ int32_t s_item, s_ntouch, s_kskip;
+  __shared__ int32_t s_item, s_ntouch, s_kskip, s_batch;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int64_t nheavy = *nheavy_p;
@@ -1014,6 +1014,7 @@ __global__ void __launch_bounds__(kC4Block)
   for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
   if (threadIdx.x == 0) {
     s_ntouch = 0;
+    s_batch = 0;
     // An endpoint w of degree <= 1 closes no 4-cycle (c_u(w) <= d(w) <= 1), and ids are ranked
     // by degree, so the windows below the first id of degree >= 2 are skipped whole (on R-MAT 22,
     // 35 of its 64 windows: 43% of the ids are isolated and 56% have degree below 2).
@@ -1052,9 +1053,14 @@ __global__ void __launch_bounds__(kC4Block)
       nx[i] = (int32_t)(runs[lo] >> kRunPosBits);
     }
     __syncthreads();
-    // neighbours per wave: 64, or for fewer than kC4Block neighbours an even share, so every
-    // wave of the (one per CU) workgroup has loads in flight
-    const int32_t per = nl < kC4Block ? (nl + kC4Waves - 1) / kC4Waves : kWave;
+    // Neighbours per wave. N(u) ascends by rank, i.e. by degree, and a neighbour's runs grow with
+    // its degree, so contiguous shares would leave the last wave the longest runs: below kC4Block
+    // neighbours each wave takes every kC4Waves-th one (an even share of every degree range, so
+    // every wave of the one-per-CU workgroup has loads in flight); from kC4Block on, the waves take
+    // batches of 64 consecutive neighbours, the first kC4Waves by wave index and the rest from an
+    // LDS counter, so a wave that drew short runs takes more batches.
+    const bool few = nl < kC4Block;
+    const int32_t per = few ? (nl + kC4Waves - 1) / kC4Waves : kWave;
     // Passes over the index windows k0 .. k1 - 1 (kC4Win ids each). A count c_u(w) is at most
     // min(nl, d(w)), and ids are ranked by degree, so d(w) <= deg[whi - 1] in a pass over
     // [wlo, whi). The counter width of a pass is the narrowest that cannot overflow:
@@ -1086,8 +1092,8 @@ __global__ void __launch_bounds__(kC4Block)
       const bool advance = bits != 32 || whi == min(wbase + kC4Win, u);
       const int lgc = bits == 8 ? 2 : bits == 16 ? 1 : 0;  // log2 counters per word
       const uint32_t cmask = bits == 32 ? 0xffffffffu : (1u << bits) - 1;
-      for (int32_t i0 = wib * per; i0 < nl; i0 += kC4Waves * per) {
-        const int32_t i = i0 + lane;
+      for (int32_t bi = wib; few ? bi == wib : bi * kWave < nl;) {
+        const int32_t i = few ? wib + kC4Waves * lane : bi * kWave + lane;
         int64_t p = 0;
         int32_t len = 0;
         // the run of N(v) in index windows [k, k + nidx): nx[i] caches the window of the
@@ -1146,6 +1152,10 @@ __global__ void __launch_bounds__(kC4Block)
             }
           }
         }
+        if (few) break;
+        int32_t nb = 0;
+        if (lane == 0) nb = atomicAdd(&s_batch, 1);
+        bi = kC4Waves + __builtin_amdgcn_readfirstlane(__shfl(nb, 0, kWave));
       }
       __syncthreads();
       const int32_t nt = s_ntouch;
@@ -1169,7 +1179,10 @@ __global__ void __launch_bounds__(kC4Block)
         }
       }
       __syncthreads();
-      if (threadIdx.x == 0) s_ntouch = 0;
+      if (threadIdx.x == 0) {
+        s_ntouch = 0;
+        s_batch = 0;
+      }
       __syncthreads();
       if (bits == 32 && !advance) {
         half = 1;
