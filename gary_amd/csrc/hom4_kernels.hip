@@ -922,10 +922,11 @@ __global__ void __launch_bounds__(kHomBlock)
 // claimed window group by window group, so the workgroups read the same id range of the
 // neighbour lists together (R-MAT 22: 281 ms, against 334 ms vertex by vertex), and within a
 // group from the highest-ranked vertex (most wedges) down, so a group ends with small items. The
-// endpoints of N(v) in a window form one run of N(v) (sorted) that starts at a per-neighbour
-// cursor: its first four elements are read together, and only a run longer than that is ended
-// by a binary search. The runs of 64 neighbours v are flattened over a wave (as in the triangle
-// kernel's row streaming) so that every lane counts one endpoint per step.
+// endpoints of N(v) in a window form one run of N(v) (sorted), bounded by the run index below.
+// The runs of 64 neighbours v are flattened over a wave (as in the triangle kernel's row
+// streaming) so that every lane counts one endpoint per step. (Loading the cursor with nx[i] and
+// the three index entries together, two dependent loads per batch instead of three, made no
+// difference: 153.1 -> 153.7 ms per R-MAT 22 C4 call.)
 //
 // Run index. Each neighbour's run in a window is read from a per-call index of the CSR: for
 // every row v, one 64-bit entry per window of kC4Win ids that N(v) touches (window id << 40 |
