@@ -945,7 +945,6 @@ constexpr int kC4Group = 16;    // windows per work item (each item starts its c
 constexpr int kC4Block = 1024;
 constexpr int kC4Waves = kC4Block / kWave;
 constexpr int kC4Touch = 2048;  // endpoints listed per window; a window with more is swept whole
-constexpr int kC4Steps = 4;     // 64-endpoint steps whose loads are issued together
 
 // Entries per row of the run index: the windows N(v) touches, plus the row's last entry. One wave
 // per row; cnt[m] = 0 (the scan's total slot).
@@ -1012,6 +1011,7 @@ __global__ void __launch_bounds__(kC4Block)
   int64_t *cur = cursors + (int64_t)blockIdx.x * stride;     // run-index entry of the next run
   int32_t *nx = next_win + (int64_t)blockIdx.x * stride;     // window id of that run
   uint64_t acc = 0;
+  int32_t tag = 1 << 30;  // slot tags of this wave's chunk hand-off (first use clears the slots)
   for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
   if (threadIdx.x == 0) {
     s_ntouch = 0;
@@ -1119,37 +1119,55 @@ __global__ void __launch_bounds__(kC4Block)
             nx[i] = (int32_t)(b >> kRunPosBits);
           }
         }
-        const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)len);
-        const int32_t start = incl - len;
+        // The batch's runs are flattened into the aligned 8-element (32-byte) chunks that overlap
+        // them, 64 per window, lane l taking chunk w0 + l, as in the triangle kernel's row
+        // streaming: the lane's run comes from a tagged slot and a max-scan, its chunk and valid
+        // slots from three shuffles, so that hand-off is paid once per 8 endpoints (once per
+        // endpoint, with 4 endpoints per lane in flight: 153 ms per R-MAT 22 C4 call).
+        // Chunk indices are 32-bit (the window path needs nnz < 2^32).
+        const int32_t nch = len > 0 ? (int32_t)(((p + len + 7) >> 3) - (p >> 3)) : 0;
+        const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
+        const int32_t start = incl - nch;
         const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
-        // kC4Steps steps of 64 endpoints per trip: their owners first, then all loads, then
-        // all counter updates, so the loads are in flight together
-        for (int32_t w0 = 0; w0 < total; w0 += kC4Steps * kWave) {
-          int64_t idx[kC4Steps];
-#pragma unroll
-          for (int r = 0; r < kC4Steps; ++r) {
-            const int32_t ws = w0 + r * kWave;
+        const uint32_t base_r = (uint32_t)(p >> 3) - (uint32_t)start;
+        const uint32_t A = (uint32_t)(p & 7) + 8u * (uint32_t)start;
+        const uint32_t Bv = A + (uint32_t)len;
+        const int32_t span = whi - wlo;
+        for (int32_t w0 = 0; w0 < total; w0 += kWave) {
+          if (tag >= (1 << 30)) {  // stale slots could outrank new tags: clear them
             slots[wib][lane] = -1;
+            tag = 0;
             wave_sync();
-            if (len > 0 && start < ws + kWave && incl > ws) slots[wib][max(0, start - ws)] = lane;
-            wave_sync();
-            const int32_t row = max(0, wave_inclusive_max_i32(slots[wib][lane]));
-            const int64_t rp = __shfl(p, row, kWave);
-            const int32_t rs = __shfl(start, row, kWave);
-            idx[r] = ws + lane < total ? rp + (ws + lane - rs) : -1;
-            wave_sync();  // the slots are rewritten by the next step
           }
-          int32_t x[kC4Steps];
+          if (nch > 0 && start < w0 + kWave && incl > w0) slots[wib][max(0, start - w0)] = tag | lane;
+          wave_sync();
+          const int32_t row = wave_inclusive_max_i32(slots[wib][lane]) & (kWave - 1);
+          wave_sync();
+          tag += kWave;
+          const int32_t xw = w0 + lane;
+          const uint32_t ch = (uint32_t)__shfl((int32_t)base_r, row, kWave) + (uint32_t)xw;
+          const int32_t jlo = (int32_t)((uint32_t)__shfl((int32_t)A, row, kWave) - 8u * (uint32_t)xw);
+          int32_t jhi = (int32_t)((uint32_t)__shfl((int32_t)Bv, row, kWave) - 8u * (uint32_t)xw);
+          int4 data[2] = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+          if (xw < total) {
+            const uint64_t e0 = (uint64_t)ch << 3;
+            data[0] = *reinterpret_cast<const int4 *>(col + e0);
+            if (e0 + 4 < (uint64_t)g.csr.nnz) data[1] = *reinterpret_cast<const int4 *>(col + e0 + 4);
+          } else {
+            jhi = 0;
+          }
+          const int32_t el[8] = {data[0].x, data[0].y, data[0].z, data[0].w,
+                                 data[1].x, data[1].y, data[1].z, data[1].w};
 #pragma unroll
-          for (int r = 0; r < kC4Steps; ++r) x[r] = idx[r] >= 0 ? col[idx[r]] - wlo : -1;
-#pragma unroll
-          for (int r = 0; r < kC4Steps; ++r) {
-            if (x[r] < 0 || x[r] >= whi - wlo) continue;  // outside [wlo, whi): not this pass
-            const uint32_t sh = (uint32_t)(x[r] & ((1 << lgc) - 1)) * (uint32_t)bits;
-            const bool first = ((atomicAdd(&cnt[x[r] >> lgc], 1u << sh) >> sh) & cmask) == 0u;
+          for (int q = 0; q < 8; ++q) {
+            const int32_t x = el[q] - wlo;
+            // slots of this lane's run, endpoints in [wlo, whi) (others: not this pass)
+            if (q < jlo || q >= jhi || x < 0 || x >= span) continue;
+            const uint32_t sh = (uint32_t)(x & ((1 << lgc) - 1)) * (uint32_t)bits;
+            const bool first = ((atomicAdd(&cnt[x >> lgc], 1u << sh) >> sh) & cmask) == 0u;
             if (first) {
-              const int32_t t = atomicAdd(&s_ntouch, 1);
-              if (t < kC4Touch) touch[t] = x[r];
+              const int32_t tt = atomicAdd(&s_ntouch, 1);
+              if (tt < kC4Touch) touch[tt] = x;
             }
           }
         }
@@ -1700,6 +1718,9 @@ int gary_hom4_count_routed(const gary_device_graph *g, int patterns, int c4_rout
       GARY_CHECK_ARG((uint64_t)g->csr.nnz + (uint64_t)m + 1 < (1ull << 32) &&
                          (uint64_t)g->csr.nnz < (1ull << kRunPosBits) && windows < kRunEndWin,
                      "gary_hom4_count: 4-cycles need nnz + m below 2^32");
+      GARY_CHECK_ARG(((uintptr_t)g->csr.colIdx & 15) == 0,
+                     "gary_hom4_count: the 4-cycle windows read colIdx in 16-byte chunks: colIdx "
+                     "must be 16-byte aligned");
       const uint32_t nptr = (uint32_t)m + 1;
       const size_t b_ptr = al(sizeof(uint32_t) * nptr);
       const size_t b_part = al(sizeof(uint32_t) * exclusive_scan_parts(nptr));
