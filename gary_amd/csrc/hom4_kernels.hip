@@ -944,7 +944,7 @@ constexpr int kC4Group = 16;    // windows per work item (each item starts its c
                                 // binary search per neighbour; 4 windows: 327 ms on R-MAT 22)
 constexpr int kC4Block = 1024;
 constexpr int kC4Waves = kC4Block / kWave;
-constexpr int kC4Touch = 2048;  // endpoints listed per window; a window with more is swept whole
+constexpr int kC4Windows = 2;   // 64-chunk windows whose loads are issued together
 
 // Entries per row of the run index: the windows N(v) touches, plus the row's last entry. One wave
 // per row; cnt[m] = 0 (the scan's total slot).
@@ -1001,9 +1001,8 @@ __global__ void __launch_bounds__(kC4Block)
                      int32_t *__restrict__ next_win, int64_t stride,
                      int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
   __shared__ uint32_t cnt[kC4Words];
-  __shared__ int32_t touch[kC4Touch];  // window offsets whose counter left zero
   __shared__ int32_t slots[kC4Waves][kWave];
-  __shared__ int32_t s_item, s_ntouch, s_kskip, s_batch;
+  __shared__ int32_t s_item, s_kskip, s_batch;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int64_t nheavy = *nheavy_p;
@@ -1014,7 +1013,6 @@ __global__ void __launch_bounds__(kC4Block)
   int32_t tag = 1 << 30;  // slot tags of this wave's chunk hand-off (first use clears the slots)
   for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
   if (threadIdx.x == 0) {
-    s_ntouch = 0;
     s_batch = 0;
     // An endpoint w of degree <= 1 closes no 4-cycle (c_u(w) <= d(w) <= 1), and ids are ranked
     // by degree, so the windows below the first id of degree >= 2 are skipped whole (on R-MAT 22,
@@ -1133,41 +1131,52 @@ __global__ void __launch_bounds__(kC4Block)
         const uint32_t A = (uint32_t)(p & 7) + 8u * (uint32_t)start;
         const uint32_t Bv = A + (uint32_t)len;
         const int32_t span = whi - wlo;
-        for (int32_t w0 = 0; w0 < total; w0 += kWave) {
-          if (tag >= (1 << 30)) {  // stale slots could outrank new tags: clear them
-            slots[wib][lane] = -1;
-            tag = 0;
-            wave_sync();
-          }
-          if (nch > 0 && start < w0 + kWave && incl > w0) slots[wib][max(0, start - w0)] = tag | lane;
-          wave_sync();
-          const int32_t row = wave_inclusive_max_i32(slots[wib][lane]) & (kWave - 1);
-          wave_sync();
-          tag += kWave;
-          const int32_t xw = w0 + lane;
-          const uint32_t ch = (uint32_t)__shfl((int32_t)base_r, row, kWave) + (uint32_t)xw;
-          const int32_t jlo = (int32_t)((uint32_t)__shfl((int32_t)A, row, kWave) - 8u * (uint32_t)xw);
-          int32_t jhi = (int32_t)((uint32_t)__shfl((int32_t)Bv, row, kWave) - 8u * (uint32_t)xw);
-          int4 data[2] = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
-          if (xw < total) {
-            const uint64_t e0 = (uint64_t)ch << 3;
-            data[0] = *reinterpret_cast<const int4 *>(col + e0);
-            if (e0 + 4 < (uint64_t)g.csr.nnz) data[1] = *reinterpret_cast<const int4 *>(col + e0 + 4);
-          } else {
-            jhi = 0;
-          }
-          const int32_t el[8] = {data[0].x, data[0].y, data[0].z, data[0].w,
-                                 data[1].x, data[1].y, data[1].z, data[1].w};
+        // kC4Windows windows per trip: each window's hand-off, then its loads, so the loads of
+        // the trip's windows are in flight together before any endpoint is counted
+        for (int32_t w0 = 0; w0 < total; w0 += kC4Windows * kWave) {
+          int4 data[kC4Windows][2];
+          int32_t jlo[kC4Windows], jhi[kC4Windows];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const int32_t x = el[q] - wlo;
-            // slots of this lane's run, endpoints in [wlo, whi) (others: not this pass)
-            if (q < jlo || q >= jhi || x < 0 || x >= span) continue;
-            const uint32_t sh = (uint32_t)(x & ((1 << lgc) - 1)) * (uint32_t)bits;
-            const bool first = ((atomicAdd(&cnt[x >> lgc], 1u << sh) >> sh) & cmask) == 0u;
-            if (first) {
-              const int32_t tt = atomicAdd(&s_ntouch, 1);
-              if (tt < kC4Touch) touch[tt] = x;
+          for (int r = 0; r < kC4Windows; ++r) {
+            const int32_t ws = w0 + r * kWave;
+            data[r][0] = data[r][1] = make_int4(0, 0, 0, 0);
+            jlo[r] = 0;
+            jhi[r] = 0;
+            if (ws >= total) continue;  // wave-uniform
+            if (tag >= (1 << 30)) {  // stale slots could outrank new tags: clear them
+              slots[wib][lane] = -1;
+              tag = 0;
+              wave_sync();
+            }
+            if (nch > 0 && start < ws + kWave && incl > ws) slots[wib][max(0, start - ws)] = tag | lane;
+            wave_sync();
+            const int32_t row = wave_inclusive_max_i32(slots[wib][lane]) & (kWave - 1);
+            wave_sync();
+            tag += kWave;
+            const int32_t xw = ws + lane;
+            const uint32_t ch = (uint32_t)__shfl((int32_t)base_r, row, kWave) + (uint32_t)xw;
+            jlo[r] = (int32_t)((uint32_t)__shfl((int32_t)A, row, kWave) - 8u * (uint32_t)xw);
+            jhi[r] = (int32_t)((uint32_t)__shfl((int32_t)Bv, row, kWave) - 8u * (uint32_t)xw);
+            if (xw < total) {
+              const uint64_t e0 = (uint64_t)ch << 3;
+              data[r][0] = *reinterpret_cast<const int4 *>(col + e0);
+              if (e0 + 4 < (uint64_t)g.csr.nnz)
+                data[r][1] = *reinterpret_cast<const int4 *>(col + e0 + 4);
+            } else {
+              jhi[r] = 0;
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < kC4Windows; ++r) {
+            const int32_t el[8] = {data[r][0].x, data[r][0].y, data[r][0].z, data[r][0].w,
+                                   data[r][1].x, data[r][1].y, data[r][1].z, data[r][1].w};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const int32_t x = el[q] - wlo;
+              // slots of this lane's run, endpoints in [wlo, whi) (others: not this pass)
+              if (q < jlo[r] || q >= jhi[r] || x < 0 || x >= span) continue;
+              const uint32_t sh = (uint32_t)(x & ((1 << lgc) - 1)) * (uint32_t)bits;
+              atomicAdd(&cnt[x >> lgc], 1u << sh);
             }
           }
         }
@@ -1177,31 +1186,20 @@ __global__ void __launch_bounds__(kC4Block)
         bi = kC4Waves + __builtin_amdgcn_readfirstlane(__shfl(nb, 0, kWave));
       }
       __syncthreads();
-      const int32_t nt = s_ntouch;
-      if (nt <= kC4Touch) {  // read and clear the listed counters only
-        for (int32_t t = threadIdx.x; t < nt; t += kC4Block) {
-          const int32_t x = touch[t];
-          // the word's other counters may be cleared by other threads meanwhile
-          const uint32_t sh = (uint32_t)(x & ((1 << lgc) - 1)) * (uint32_t)bits;
-          const uint64_t c = (atomicAnd(&cnt[x >> lgc], ~(cmask << sh)) >> sh) & cmask;
-          acc += c * (c - 1) / 2;
-        }
-      } else {
-        const int32_t words = ((whi - wlo) + (1 << lgc) - 1) >> lgc;
-        for (int32_t x = threadIdx.x; x < words; x += kC4Block) {
-          const uint32_t v = cnt[x];
-          cnt[x] = 0u;
-          for (int q = 0; q < (1 << lgc); ++q) {
-            const uint64_t c = (v >> (q * bits)) & cmask;
-            acc += c * (c - (c > 0)) / 2;
-          }
+      // read and clear every counter of the pass (counting with fetch-adds that return nothing
+      // and listing first touches, up to 2,048 a pass, cost more: 91.6 -> 84.7 ms per R-MAT 22
+      // C4 call with the sweep)
+      const int32_t words = ((whi - wlo) + (1 << lgc) - 1) >> lgc;
+      for (int32_t x = threadIdx.x; x < words; x += kC4Block) {
+        const uint32_t v = cnt[x];
+        cnt[x] = 0u;
+        for (int q = 0; q < (1 << lgc); ++q) {
+          const uint64_t c = (v >> (q * bits)) & cmask;
+          acc += c * (c - (c > 0)) / 2;
         }
       }
       __syncthreads();
-      if (threadIdx.x == 0) {
-        s_ntouch = 0;
-        s_batch = 0;
-      }
+      if (threadIdx.x == 0) s_batch = 0;
       __syncthreads();
       if (bits == 32 && !advance) {
         half = 1;
