@@ -592,22 +592,63 @@ __global__ void __launch_bounds__(WAVES * kWave)
     for (int64_t i = threadIdx.x; i < row_words; i += kBlock) rows[i] = 0;
     posset_build(set, ps, L, d);  // begins and ends with a workgroup barrier
     if (threadIdx.x == 0) s_item = 0;  // row counter of the count phase (u is read by now)
-    for (int32_t i = wib; i < d; i += WAVES) {
-      const int32_t v = L[i];
-      if ((uint32_t)v >= (uint32_t)m) continue;
-      const int64_t vb = up_start[v];
-      const int32_t dv = up_len[v];
-      for (int32_t p0 = lane; p0 < dv; p0 += kStreamUnroll * kWave) {
-        int32_t w[kStreamUnroll];
-#pragma unroll
-        for (int k = 0; k < kStreamUnroll; ++k) {
-          const int32_t p = p0 + k * kWave;
-          w[k] = p < dv ? col[vb + p] : -1;  // -1 lies outside every set
+    // Stream the rows N+(v_i): wave wib takes the local rows i = b0 + wib + WAVES * l of each
+    // batch b0 (lane l's row), flattened into the aligned 8-element (32-byte) chunks that overlap
+    // them, 64 chunks per window, lane l taking chunk w0 + l: the chunk's row comes from a tagged
+    // slot (the wave's stage words, which the count phase rewrites, so every pivot starts with a
+    // clear) and a max-scan, its colIdx position and valid slots from shuffles, as in the
+    // triangle kernel's row streaming. (A wave per row with lanes striding it left most lanes
+    // idle on the short rows.)
+    int32_t *const slots = reinterpret_cast<int32_t *>(stage[wib]);
+    int32_t tag = 1 << 30;
+    for (int32_t b0 = 0; b0 < d; b0 += WAVES * kWave) {
+      const int32_t i = b0 + wib + WAVES * lane;
+      int64_t pv = 0;
+      int32_t dv = 0;
+      if (i < d) {
+        const int32_t v = L[i];
+        if ((uint32_t)v < (uint32_t)m) {
+          pv = up_start[v];
+          dv = up_len[v];
         }
+      }
+      const int32_t nch = dv > 0 ? (int32_t)(((pv + dv + 7) >> 3) - (pv >> 3)) : 0;
+      const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nch);
+      const int32_t start = incl - nch;
+      const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+      const uint32_t base_r = (uint32_t)(pv >> 3) - (uint32_t)start;
+      const uint32_t A = (uint32_t)(pv & 7) + 8u * (uint32_t)start;
+      const uint32_t Bv = A + (uint32_t)dv;
+      for (int32_t w0 = 0; w0 < total; w0 += kWave) {
+        if (tag >= (1 << 30)) {  // stale slots could outrank new tags: clear them
+          slots[lane] = -1;
+          tag = 0;
+          wave_sync();
+        }
+        if (nch > 0 && start < w0 + kWave && incl > w0) slots[max(0, start - w0)] = tag | lane;
+        wave_sync();
+        const int32_t row = wave_inclusive_max_i32(slots[lane]) & (kWave - 1);
+        wave_sync();
+        tag += kWave;
+        const int32_t xw = w0 + lane;
+        const uint32_t ch = (uint32_t)__shfl((int32_t)base_r, row, kWave) + (uint32_t)xw;
+        const int32_t jlo = (int32_t)((uint32_t)__shfl((int32_t)A, row, kWave) - 8u * (uint32_t)xw);
+        int32_t jhi = (int32_t)((uint32_t)__shfl((int32_t)Bv, row, kWave) - 8u * (uint32_t)xw);
+        const int32_t ri = b0 + wib + WAVES * row;  // local row of the chunk
+        int4 lo4 = make_int4(-1, -1, -1, -1), hi4 = make_int4(-1, -1, -1, -1);
+        if (xw < total) {
+          const uint64_t e0 = (uint64_t)ch << 3;
+          lo4 = *reinterpret_cast<const int4 *>(col + e0);
+          if (e0 + 4 < (uint64_t)g.csr.nnz) hi4 = *reinterpret_cast<const int4 *>(col + e0 + 4);
+        } else {
+          jhi = 0;
+        }
+        const int32_t el[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
 #pragma unroll
-        for (int k = 0; k < kStreamUnroll; ++k) {
-          const int32_t j = posset_find(set, ps, w[k]);
-          if (j >= 0) atomicOr(&rows[(int64_t)i * w2 + (j >> 5)], 1u << (j & 31));
+        for (int q = 0; q < 8; ++q) {
+          if (q < jlo || q >= jhi) continue;
+          const int32_t j = posset_find(set, ps, el[q]);
+          if (j >= 0) atomicOr(&rows[(int64_t)ri * w2 + (j >> 5)], 1u << (j & 31));
         }
       }
     }
@@ -1888,6 +1929,9 @@ int gary_kclique_count(const gary_device_graph *g, int k, uint64_t *count, void 
   }
   if (m == 0 || g->csr.nnz == 0) return GARY_OK;
   GARY_CHECK_ARG(g->csr.indPtr && g->csr.colIdx && g->csr.deg, "gary_kclique_count: null arrays");
+  GARY_CHECK_ARG(((uintptr_t)g->csr.colIdx & 15) == 0,
+                 "gary_kclique_count: the local-graph pass reads colIdx in 16-byte chunks: colIdx "
+                 "must be 16-byte aligned");
   hipStream_t st = (hipStream_t)stream;
   int dev = 0;
   GARY_TRY(current_device(&dev));
