@@ -1121,12 +1121,16 @@ __global__ void __launch_bounds__(kRevBlock)
   __syncthreads();
   const int64_t e_beg = (int64_t)blockIdx.x * sh.tile_arcs;
   const int64_t e_end = min(g.csr.nnz, e_beg + sh.tile_arcs);
-  for (int64_t e = e_beg; e < e_end; e += kRevChunk) {
-    int32_t u[kRevPer], v[kRevPer], du[kRevPer];
+  // two chunks per trip, their loads in flight together
+  for (int64_t e = e_beg; e < e_end; e += 2 * kRevChunk) {
+    int32_t u[kRevPer], v[kRevPer], du[kRevPer], u2[kRevPer], v2[kRevPer], du2[kRevPer];
     rev_load_chunk(g, route, dsat, e, e_end, u, v, du);
+    rev_load_chunk(g, route, dsat, e + kRevChunk, e_end, u2, v2, du2);
 #pragma unroll
-    for (int q = 0; q < kRevPer; ++q)
+    for (int q = 0; q < kRevPer; ++q) {
       if (v[q] >= 0) atomicAdd(&s_hist[rev_bucket((uint32_t)v[q], sh)], 1u);
+      if (v2[q] >= 0) atomicAdd(&s_hist[rev_bucket((uint32_t)v2[q], sh)], 1u);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kRevBlock)
