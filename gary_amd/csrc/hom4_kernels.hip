@@ -382,6 +382,8 @@ __global__ void __launch_bounds__(WAVES * kWave)
       const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nb);
       const int32_t excl = incl - nb;
       const int32_t arcs = __builtin_amdgcn_readlane(incl, kWave - 1);
+      // words of row i past its last nonzero one add nothing to popc(row_i & row_j)
+      const int32_t wend = 64 - (int32_t)__clzll((unsigned long long)__ballot(word != 0u));
       for (int32_t a0 = 0; a0 < arcs; a0 += kWave) {
         const int32_t a = a0 + lane;
         int32_t o = 0;  // last word with excl <= a
@@ -404,7 +406,7 @@ __global__ void __launch_bounds__(WAVES * kWave)
         const int32_t j = o * 32 + pos;
         const uint32_t *rj = rows + (int64_t)j * W2;
         uint32_t c = 0;
-        for (int32_t w = (j + 1) >> 5; w < W2; ++w) c += (uint32_t)__popc(ri[w] & rj[w]);
+        for (int32_t w = (j + 1) >> 5; w < wend; ++w) c += (uint32_t)__popc(ri[w] & rj[w]);
         k4 += c;
       }
     }
@@ -478,6 +480,8 @@ __device__ uint64_t kc_arcs_in(const uint32_t *rows, int32_t w2, uint32_t S, uin
   const int32_t incl = (int32_t)wave_inclusive_scan_u32((uint32_t)nb);
   const int32_t excl = incl - nb;
   const int32_t bits = __builtin_amdgcn_readlane(incl, kWave - 1);
+  // words of S past its last nonzero one add nothing to popc(S & row_y)
+  const int32_t wend = 64 - (int32_t)__clzll((unsigned long long)__ballot(S != 0u));
   uint64_t acc = 0;
   for (int32_t a0 = 0; a0 < bits; a0 += kWave) {
     const int32_t a = a0 + lane;
@@ -501,7 +505,7 @@ __device__ uint64_t kc_arcs_in(const uint32_t *rows, int32_t w2, uint32_t S, uin
     const int32_t y = o * 32 + pos;
     const uint32_t *ry = rows + (int64_t)y * w2;
     uint32_t c = 0;
-    for (int32_t w = (y + 1) >> 5; w < w2; ++w) c += (uint32_t)__popc(stage[w] & ry[w]);
+    for (int32_t w = (y + 1) >> 5; w < wend; ++w) c += (uint32_t)__popc(stage[w] & ry[w]);
     acc += c;
   }
   wave_sync();
