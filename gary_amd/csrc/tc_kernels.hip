@@ -650,9 +650,11 @@ constexpr int kStripeSpacing = 64;  // words
 // and one chain of header loads; with reversed rows the small pivots stream little (their arcs
 // to much longer out-lists are streamed at the other end), so larger claims pay: R-MAT 20
 // 2.24 -> 2.03 ms per launch at 8 / 24 against 4 / 16, while forward streaming loses balance
-// with them (2.90 -> 3.13 ms).
-constexpr int kSmallBatchHeavy = 4, kSmallBatchHeavyRev = 8;
-constexpr int kSmallBatchLight = 16, kSmallBatchLightRev = 24;
+// with them (2.90 -> 3.13 ms) and gains a little with smaller ones (2 / 12 against 4 / 16:
+// R-MAT 20 forward 2.84 -> 2.81 ms per count, the 2,000-graph batch 13.79 -> 13.57 ms; 4 / 64
+// and 8 / 64: 17.5 ms).
+constexpr int kSmallBatchHeavy = 2, kSmallBatchHeavyRev = 8;
+constexpr int kSmallBatchLight = 12, kSmallBatchLightRev = 24;
 static_assert(kSmallBatchHeavyRev <= kWave && kSmallBatchLightRev <= kWave &&
                   kSmallBatchHeavy <= kWave && kSmallBatchLight <= kWave,
               "one pivot header per lane");
