@@ -779,6 +779,21 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
         hhi = g.csr.colIdx[hp + hd - 1];
       }
     }
+    // The claim's set plans, one pivot per lane in vector code, packed as 1 | s << 1 |
+    // log_t << 6 | cwords << 10 (0: the set does not fit; the words follow from them and d)
+    // with the span beside it. Planned per pivot, the plan's scalar code ran on the CU's one
+    // scalar unit for every pivot of every wave: 1.8 ms of a 2,000-graph batch launch (20 M
+    // pivots).
+    static_assert(TABLE_WORDS / WAVES <= 4096, "coarse words fit 12 bits");
+    int32_t hplan = 0;
+    uint32_t hspan = 0;
+    if (lane < cnt && hd >= 2) {
+      PivotSet ps;
+      if (pivot_plan(hlo, hhi, hd, TABLE_WORDS / WAVES, ps)) {
+        hplan = 1 | ps.s << 1 | ps.log_t << 6 | ps.cwords << 10;
+        hspan = ps.span;
+      }
+    }
     // Pivots without reverse rows and with at most kWave out-neighbours are pipelined: the next
     // such pivot's out-list is loaded (one element per lane) while this one is built and streamed, and the row headers of this one's out-neighbours are loaded before its set is
     // built, so a pivot waits on two dependent global loads (row headers, then chunks) instead of
@@ -820,12 +835,20 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       const uint2 *L2 = rrange ? rrows + rb0 : nullptr;
       const int32_t dmax = rrange && d < kRevMaxSourceDeg ? d + kRevSlack : INT32_MAX;
       const uint8_t *Lroute = route ? route + pu : nullptr;
-      PivotSet s;
-      if (!pivot_plan(__builtin_amdgcn_readlane(hlo, j), __builtin_amdgcn_readlane(hhi, j), d,
-                      TABLE_WORDS / WAVES, s)) {
+      const int32_t pl = __builtin_amdgcn_readlane(hplan, j);
+      if (pl == 0) {
         c += pivot_fallback(g, L, d, L2, d2, dmax, 0, 1, lane, Lroute);
         continue;
       }
+      PivotSet s;
+      s.base = __builtin_amdgcn_readlane(hlo, j);
+      s.span = (uint32_t)__builtin_amdgcn_readlane((int32_t)hspan, j);
+      s.s = (pl >> 1) & 31;
+      s.log_t = (pl >> 6) & 15;
+      s.cwords = (pl >> 10) & 4095;
+      s.words = s.log_t ? 1 << s.log_t : s.cwords + 1 + (d << (s.s - 5));
+      s.ebase = 4u * (uint32_t)(s.cwords + 1);
+      s.emask = s.log_t ? 0u : ((1u << (s.s - 3)) - 1u) & ~3u;
       if (piped(j)) {
         int64_t pv;
         int32_t dv;
