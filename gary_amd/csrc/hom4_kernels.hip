@@ -535,6 +535,86 @@ __device__ uint64_t kc_dfs(const uint32_t *rows, int32_t w2, uint32_t S, uint32_
   return acc;
 }
 
+// K5 on a local graph whose rows fit 16 words (d <= 512): the wave's four 16-lane groups take
+// the set bits x of row i in turn (group g the bits g, g + 4, ...), each with its own set
+// S2 = row_i & row_x, whose inner arcs it counts lane-parallel as kc_arcs_in does over a whole
+// wave. S holds word (lane mod 16) of row i in every group. Per arc (i, x) of the local graph
+// the full-wave walk paid one ballot, one staging round and one arc enumeration over 64 lanes
+// for a set that mostly fits a few words: 120 ms for K5 on R-MAT 20.
+constexpr int kKcGroup = 16;
+
+__device__ __forceinline__ int32_t group_incl_scan(int32_t v, int lg) {
+#pragma unroll
+  for (int o = 1; o < kKcGroup; o <<= 1) {
+    const int32_t t = __shfl_up(v, o, kKcGroup);
+    if (lg >= o) v += t;
+  }
+  return v;
+}
+
+// Position of the k-th set bit (k < popc) of the group's words, given their exclusive popcount
+// prefix: the last word o with excl <= k, then the bit inside it by halving popcounts.
+__device__ __forceinline__ int32_t group_kth_bit(uint32_t word, int32_t excl, int32_t k) {
+  int32_t o = 0;
+#pragma unroll
+  for (int32_t step = kKcGroup / 2; step > 0; step >>= 1)
+    if (__shfl(excl, o + step, kKcGroup) <= k) o += step;
+  uint32_t wv = (uint32_t)__shfl((int32_t)word, o, kKcGroup);
+  int32_t kk = k - __shfl(excl, o, kKcGroup);
+  int32_t pos = 0;
+#pragma unroll
+  for (int32_t width = 16; width > 0; width >>= 1) {
+    const int32_t c = __popc(wv & ((1u << width) - 1u));
+    if (kk >= c) {
+      kk -= c;
+      wv >>= width;
+      pos += width;
+    }
+  }
+  return o * 32 + pos;
+}
+
+__device__ uint64_t kc5_row_grouped(const uint32_t *rows, int32_t w2, uint32_t S, uint32_t *stage) {
+  const int lane = lane_id(), lg = lane & (kKcGroup - 1), base = lane & ~(kKcGroup - 1);
+  const int grp = lane / kKcGroup;
+  constexpr int kGroups = kWave / kKcGroup;
+  const int32_t nb = __popc(S);
+  const int32_t incl = group_incl_scan(nb, lg);
+  const int32_t excl = incl - nb;
+  const int32_t nbits = __builtin_amdgcn_readlane(incl, kKcGroup - 1);  // the same in every group
+  uint64_t acc = 0;
+  for (int32_t k0 = 0; k0 < nbits; k0 += kGroups) {
+    const int32_t k = k0 + grp;
+    const bool act = k < nbits;
+    const int32_t x = act ? group_kth_bit(S, excl, k) : 0;
+    const uint32_t S2 = act && lg < w2 ? (S & rows[(int64_t)x * w2 + lg]) : 0u;
+    stage[lane] = S2;
+    wave_sync();
+    const int32_t nb2 = __popc(S2);
+    const int32_t incl2 = group_incl_scan(nb2, lg);
+    const int32_t excl2 = incl2 - nb2;
+    const int32_t bits2 = __shfl(incl2, kKcGroup - 1, kKcGroup);
+    // words of S2 past its last nonzero one add nothing
+    const uint32_t nzw = (uint32_t)(__ballot(S2 != 0u) >> base) & ((1u << kKcGroup) - 1u);
+    const int32_t wend = 32 - __clz((int)nzw);
+    int32_t maxbits = 0;
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q)
+      maxbits = max(maxbits, __builtin_amdgcn_readlane(incl2, q * kKcGroup + kKcGroup - 1));
+    for (int32_t a0 = 0; a0 < maxbits; a0 += kKcGroup) {
+      const int32_t a = a0 + lg;
+      const int32_t y = group_kth_bit(S2, excl2, min(a, max(bits2 - 1, 0)));
+      if (a >= bits2) continue;
+      const uint32_t *ry = rows + (int64_t)y * w2;
+      uint32_t c = 0;
+      for (int32_t w = (y + 1) >> 5; w < wend; ++w) c += (uint32_t)__popc(stage[base + w] & ry[w]);
+      acc += c;
+    }
+    wave_sync();  // the stage is rewritten for the next bits
+  }
+  return acc;
+}
+
 struct U128Acc {
   unsigned long long lo, hi;
 };
@@ -666,6 +746,14 @@ __global__ void __launch_bounds__(WAVES * kWave)
         i = __shfl(i, 0, kWave);
       }
       if (i >= d) break;
+      if constexpr (K == 5) {
+        if (w2 <= kKcGroup) {  // four 16-lane groups per wave (row i in each)
+          const int lg = lane & (kKcGroup - 1);
+          const uint32_t S16 = lg < w2 ? rows[(int64_t)i * w2 + lg] : 0u;
+          acc += kc5_row_grouped(rows, w2, S16, stage[wib]);
+          continue;
+        }
+      }
       const uint32_t S = lane < w2 ? rows[(int64_t)i * w2 + lane] : 0u;
       if constexpr (K == 3) {
         acc += (uint32_t)__popc(S);
