@@ -780,8 +780,9 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       }
     }
     // The claim's set plans, one pivot per lane in vector code, packed as 1 | s << 1 |
-    // log_t << 6 | cwords << 10 (0: the set does not fit; the words follow from them and d)
-    // with the span beside it. Planned per pivot, the plan's scalar code ran on the CU's one
+    // log_t << 6 | cwords << 10 | piped << 22 (0: the set does not fit; the words follow from
+    // them and d; piped: at most kWave out-neighbours and no reverse rows) with the span
+    // beside it. Planned per pivot, the plan's scalar code ran on the CU's one
     // scalar unit for every pivot of every wave: 1.8 ms of a 2,000-graph batch launch (20 M
     // pivots).
     static_assert(TABLE_WORDS / WAVES <= 4096, "coarse words fit 12 bits");
@@ -790,7 +791,8 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     if (lane < cnt && hd >= 2) {
       PivotSet ps;
       if (pivot_plan(hlo, hhi, hd, TABLE_WORDS / WAVES, ps)) {
-        hplan = 1 | ps.s << 1 | ps.log_t << 6 | ps.cwords << 10;
+        const bool pip = hd <= kWave && hr.y == hr.x;
+        hplan = 1 | ps.s << 1 | ps.log_t << 6 | ps.cwords << 10 | (pip ? 1 << 22 : 0);
         hspan = ps.span;
       }
     }
@@ -807,10 +809,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       return (int64_t)(((uint64_t)hi << 32) | lo);
     };
     auto piped = [&](int32_t jj) {
-      const int32_t dj = __builtin_amdgcn_readlane(hd, jj);
-      const int32_t rj = __builtin_amdgcn_readlane((int32_t)hr.y, jj) -
-                         __builtin_amdgcn_readlane((int32_t)hr.x, jj);
-      return dj >= 2 && dj <= kWave && rj == 0;
+      return ((__builtin_amdgcn_readlane(hplan, jj) >> 22) & 1) != 0;  // set only with d >= 2
     };
     auto prefetch = [&](int32_t jj) {
       if (jj >= cnt || !piped(jj)) return -1;
@@ -852,7 +851,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
       s.words = s.log_t ? 1 << s.log_t : s.cwords + 1 + (d << (s.s - 5));
       s.ebase = 4u * (uint32_t)(s.cwords + 1);
       s.emask = s.log_t ? 0u : ((1u << (s.s - 3)) - 1u) & ~3u;
-      if (piped(j)) {
+      if ((pl >> 22) & 1) {
         int64_t pv;
         int32_t dv;
         forward_row(g, e, dmax, e >= 0 && (!Lroute || Lroute[lane] == kRouteBitmap), pv, dv);
