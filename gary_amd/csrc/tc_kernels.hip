@@ -1108,11 +1108,13 @@ __global__ void __launch_bounds__(256) rev_dsat_kernel(const int32_t *__restrict
 // source, du[q] = d+(u). The test reads the byte dsat[v] and the exact out-degree only when it
 // saturates; it is written as selects: with early returns inside the unrolled loop the
 // compiler kept v[q] for every arc.
+template <bool ROWS = false>
 __device__ __forceinline__ void rev_load_chunk(const gary_device_graph &g,
                                                const uint8_t *__restrict__ route,
                                                const uint8_t *__restrict__ dsat, int64_t e,
                                                int64_t e_end, int32_t (&u)[kRevPer],
-                                               int32_t (&v)[kRevPer], int32_t (&du)[kRevPer]) {
+                                               int32_t (&v)[kRevPer], int32_t (&du)[kRevPer],
+                                               uint32_t *pu = nullptr) {
   const uint32_t m = (uint32_t)g.csr.m;
 #pragma unroll
   for (int q = 0; q < kRevPer; ++q) {
@@ -1127,6 +1129,8 @@ __device__ __forceinline__ void rev_load_chunk(const gary_device_graph &g,
     const bool ok = (uint32_t)u[q] < m && (uint32_t)v[q] < m;
     du[q] = ok ? g.csr.deg[u[q]] : INT32_MAX - kRevSlack;
     sv[q] = ok ? (int32_t)dsat[v[q]] : 0;
+    // the source's row start beside its degree, not after the test (one dependent load less)
+    if constexpr (ROWS) pu[q] = ok ? (uint32_t)g.csr.indPtr[u[q]] : 0u;
   }
 #pragma unroll
   for (int q = 0; q < kRevPer; ++q) {
@@ -1190,10 +1194,8 @@ __global__ void __launch_bounds__(kRevBlock)
     for (int i = threadIdx.x; i < nb; i += kRevBlock) s_fill[i] = 0;
     __syncthreads();
     int32_t u[kRevPer], v[kRevPer], du[kRevPer];
-    rev_load_chunk(g, route, dsat, e, e_end, u, v, du);
     uint32_t pu[kRevPer];  // row starts of the sources (consecutive arcs mostly share one)
-#pragma unroll
-    for (int q = 0; q < kRevPer; ++q) pu[q] = v[q] >= 0 ? (uint32_t)g.csr.indPtr[u[q]] : 0u;
+    rev_load_chunk<true>(g, route, dsat, e, e_end, u, v, du, pu);
 #pragma unroll
     for (int q = 0; q < kRevPer; ++q)
       if (v[q] >= 0) atomicAdd(&s_fill[rev_bucket((uint32_t)v[q], sh)], 1u);
