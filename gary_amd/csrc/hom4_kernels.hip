@@ -1146,7 +1146,7 @@ __global__ void __launch_bounds__(kC4Block)
                      const uint64_t *__restrict__ runs, int64_t *__restrict__ cursors,
                      int32_t *__restrict__ next_win, int64_t stride,
                      int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
-  __shared__ uint32_t cnt[kC4Words];
+  __shared__ __align__(16) uint32_t cnt[kC4Words];
   __shared__ int32_t slots[kC4Waves][kWave];
   __shared__ int32_t s_item, s_kskip, s_batch;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
@@ -1334,14 +1334,36 @@ __global__ void __launch_bounds__(kC4Block)
       __syncthreads();
       // read and clear every counter of the pass (counting with fetch-adds that return nothing
       // and listing first touches, up to 2,048 a pass, cost more: 91.6 -> 84.7 ms per R-MAT 22
-      // C4 call with the sweep)
-      const int32_t words = ((whi - wlo) + (1 << lgc) - 1) >> lgc;
-      for (int32_t x = threadIdx.x; x < words; x += kC4Block) {
-        const uint32_t v = cnt[x];
-        cnt[x] = 0u;
-        for (int q = 0; q < (1 << lgc); ++q) {
-          const uint64_t c = (v >> (q * bits)) & cmask;
-          acc += c * (c - (c > 0)) / 2;
+      // C4 call with the sweep), four words per 16-byte LDS access. C(c, 2) per word: 8-bit
+      // counters (c <= 255) by two byte dot products, (sum c^2 - sum c) / 2, exact in 32 bits;
+      // wider ones in 64 bits. (Counter by counter in 64-bit arithmetic, the sweeps cost about
+      // 60 vector instructions per word: some 30 ms of an R-MAT 22 C4 call.)
+      static_assert(kC4Words % 4 == 0, "whole 16-byte groups of counter words");
+      const int32_t words4 = ((((whi - wlo) + (1 << lgc) - 1) >> lgc) + 3) >> 2;
+      uint4 *cnt4 = reinterpret_cast<uint4 *>(cnt);
+      for (int32_t x = threadIdx.x; x < words4; x += kC4Block) {
+        const uint4 v4 = cnt4[x];
+        cnt4[x] = make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t w[4] = {v4.x, v4.y, v4.z, v4.w};
+        if (bits == 8) {
+          uint32_t s = 0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            s += (__builtin_amdgcn_udot4(w[q], w[q], 0u, false) -
+                  __builtin_amdgcn_udot4(w[q], 0x01010101u, 0u, false)) >> 1;
+          acc += s;
+        } else if (bits == 16) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint64_t lo = w[q] & 0xffffu, hi = w[q] >> 16;
+            acc += (lo * (lo - (lo > 0)) + hi * (hi - (hi > 0))) >> 1;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint64_t c = w[q];
+            acc += (c * (c - (c > 0))) >> 1;
+          }
         }
       }
       __syncthreads();
