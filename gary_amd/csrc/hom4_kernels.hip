@@ -1148,7 +1148,7 @@ __global__ void __launch_bounds__(kC4Block)
                      int32_t *__restrict__ item_cursor, unsigned long long *c4_acc) {
   __shared__ __align__(16) uint32_t cnt[kC4Words];
   __shared__ int32_t slots[kC4Waves][kWave];
-  __shared__ int32_t s_item, s_kskip, s_batch;
+  __shared__ int32_t s_item, s_kskip, s_batch, s_any;
   const int lane = lane_id(), wib = threadIdx.x / kWave;
   const int32_t *__restrict__ col = g.csr.colIdx;
   const int64_t nheavy = *nheavy_p;
@@ -1160,6 +1160,7 @@ __global__ void __launch_bounds__(kC4Block)
   for (int32_t x = threadIdx.x; x < kC4Words; x += kC4Block) cnt[x] = 0u;  // zero between windows
   if (threadIdx.x == 0) {
     s_batch = 0;
+    s_any = 0;
     // An endpoint w of degree <= 1 closes no 4-cycle (c_u(w) <= d(w) <= 1), and ids are ranked
     // by degree, so the windows below the first id of degree >= 2 are skipped whole (on R-MAT 22,
     // 35 of its 64 windows: 43% of the ids are isolated and 56% have degree below 2).
@@ -1277,6 +1278,7 @@ __global__ void __launch_bounds__(kC4Block)
         const uint32_t A = (uint32_t)(p & 7) + 8u * (uint32_t)start;
         const uint32_t Bv = A + (uint32_t)len;
         const int32_t span = whi - wlo;
+        if (total > 0 && lane == 0) s_any = 1;  // the pass has counters to sweep
         // kC4Windows windows per trip: each window's hand-off, then its loads, so the loads of
         // the trip's windows are in flight together before any endpoint is counted
         for (int32_t w0 = 0; w0 < total; w0 += kC4Windows * kWave) {
@@ -1339,7 +1341,8 @@ __global__ void __launch_bounds__(kC4Block)
       // wider ones in 64 bits. (Counter by counter in 64-bit arithmetic, the sweeps cost about
       // 60 vector instructions per word: some 30 ms of an R-MAT 22 C4 call.)
       static_assert(kC4Words % 4 == 0, "whole 16-byte groups of counter words");
-      const int32_t words4 = ((((whi - wlo) + (1 << lgc) - 1) >> lgc) + 3) >> 2;
+      // (a pass in which no neighbour had a run has nothing to sweep)
+      const int32_t words4 = s_any ? ((((whi - wlo) + (1 << lgc) - 1) >> lgc) + 3) >> 2 : 0;
       uint4 *cnt4 = reinterpret_cast<uint4 *>(cnt);
       for (int32_t x = threadIdx.x; x < words4; x += kC4Block) {
         const uint4 v4 = cnt4[x];
@@ -1367,7 +1370,10 @@ __global__ void __launch_bounds__(kC4Block)
         }
       }
       __syncthreads();
-      if (threadIdx.x == 0) s_batch = 0;
+      if (threadIdx.x == 0) {
+        s_batch = 0;
+        s_any = 0;
+      }
       __syncthreads();
       if (bits == 32 && !advance) {
         half = 1;
