@@ -337,7 +337,9 @@ __global__ void __launch_bounds__(WAVES * kWave)
     // 32-byte chunks, as the triangle kernel does, cut this pass to 67.9 ms per R-MAT 22 K4 +
     // diamond call without the t(v, w) atomics, but a chunk per lane spread each atomic
     // instruction over many rows' lines: 113.8 ms with them, 90.2 ms transposed to 8 chunks per
-    // instruction, against 81.3 ms here, where a wave's lanes hit one row.)
+    // instruction, against 81.3 ms here, where a wave's lanes hit one row. Two rows streamed by
+    // one wave as a single run, which fills the lanes a short row leaves idle, changed nothing:
+    // 78.9 against 77.5 ms on R-MAT 22, 14.9 against 15.0 ms on R-MAT 20.)
     for (int32_t i = wib; i < d; i += WAVES) {
       const int32_t v = L[i];
       if ((uint32_t)v >= (uint32_t)m) continue;
