@@ -56,7 +56,9 @@ constexpr int kHomSmallPool = kLdsWords / 4 - 1;                         // + s_
 constexpr int kHomBigPool = kLdsWords - 1;                               // + s_item
 constexpr int kKcSmallPool = kLdsWords / 4 - kSmallWaves * 64 - 1;      // + stage, s_item
 constexpr int kKcBigPool = kLdsWords - kBigWaves * 64 - 1;              // + stage, s_item
-constexpr int kStreamUnroll = 4;  // N+(v) loads a lane issues before resolving them
+// N+(v) loads a lane issues before resolving them (R-MAT 22 K4 + diamond: 2 -> 78.1 ms,
+// 4 -> 72.3 ms, 8 -> 70.6 ms, 16 -> 71.6 ms)
+constexpr int kStreamUnroll = 8;
 
 // LDS words beyond the set for a pivot of out-degree d: bitset rows (2 * ceil(d / 64) words
 // each) and, for the 4-vertex pass, d column counters.
@@ -142,23 +144,23 @@ __device__ void posset_build(int32_t *t, const PosSet &ps, const int32_t *__rest
     __syncthreads();
     return;
   }
+  // coarse word k and its rank prefix at t[2k], t[2k + 1]; exact word k and its prefix at
+  // t[2 cwords + 2k], t[2 cwords + 2k + 1]: a lookup level is one 8-byte LDS read
   const int32_t s = ps.s;
   uint32_t *coarse = (uint32_t *)t;
-  int32_t *cpre = t + ps.cwords;
   uint32_t *exact = (uint32_t *)(t + 2 * ps.cwords);
-  int32_t *epre = t + 2 * ps.cwords + ps.ewords;
   for (int32_t i = tid; i < d; i += nthr) {
     const uint32_t x = (uint32_t)(L[i] - ps.base);
-    if (x < ps.span) atomicOr(&coarse[x >> (s + 5)], 1u << ((x >> s) & 31));
+    if (x < ps.span) atomicOr(&coarse[2 * (x >> (s + 5))], 1u << ((x >> s) & 31));
   }
   __syncthreads();
   if (tid < kWave) {
     uint32_t carry = 0;
     for (int32_t c0 = 0; c0 < ps.cwords; c0 += kWave) {
       const int32_t i = c0 + tid;
-      const uint32_t v = i < ps.cwords ? (uint32_t)__popc(coarse[i]) : 0u;
+      const uint32_t v = i < ps.cwords ? (uint32_t)__popc(coarse[2 * i]) : 0u;
       const uint32_t inc = wave_inclusive_scan_u32(v);
-      if (i < ps.cwords) cpre[i] = (int32_t)(carry + inc - v);
+      if (i < ps.cwords) coarse[2 * i + 1] = carry + inc - v;
       carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
     }
   }
@@ -167,17 +169,17 @@ __device__ void posset_build(int32_t *t, const PosSet &ps, const int32_t *__rest
     const uint32_t x = (uint32_t)(L[i] - ps.base);
     if (x >= ps.span) continue;
     const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
-    const uint32_t r = (uint32_t)cpre[cw] + (uint32_t)__popc(coarse[cw] & ((1u << b) - 1));
-    atomicOr(&exact[(r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5)], 1u << (x & 31));
+    const uint32_t r = coarse[2 * cw + 1] + (uint32_t)__popc(coarse[2 * cw] & ((1u << b) - 1));
+    atomicOr(&exact[2 * ((r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5))], 1u << (x & 31));
   }
   __syncthreads();
   if (tid < kWave) {
     uint32_t carry = 0;
     for (int32_t c0 = 0; c0 < ps.ewords; c0 += kWave) {
       const int32_t i = c0 + tid;
-      const uint32_t v = i < ps.ewords ? (uint32_t)__popc(exact[i]) : 0u;
+      const uint32_t v = i < ps.ewords ? (uint32_t)__popc(exact[2 * i]) : 0u;
       const uint32_t inc = wave_inclusive_scan_u32(v);
-      if (i < ps.ewords) epre[i] = (int32_t)(carry + inc - v);
+      if (i < ps.ewords) exact[2 * i + 1] = carry + inc - v;
       carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
     }
   }
@@ -191,13 +193,13 @@ __device__ __forceinline__ int32_t posset_find(const int32_t *t, const PosSet &p
   if (ps.log_t == 0) {
     const int32_t s = ps.s;
     const uint32_t cw = x >> (s + 5), b = (x >> s) & 31;
-    const uint32_t word = (uint32_t)t[cw];
-    if (!((word >> b) & 1u)) return -1;
-    const uint32_t r = (uint32_t)t[ps.cwords + cw] + (uint32_t)__popc(word & ((1u << b) - 1));
+    const uint2 c = reinterpret_cast<const uint2 *>(t)[cw];  // {word, rank prefix}
+    if (!((c.x >> b) & 1u)) return -1;
+    const uint32_t r = c.y + (uint32_t)__popc(c.x & ((1u << b) - 1));
     const uint32_t ei = (r << (s - 5)) + ((x & ((1u << s) - 1)) >> 5);
-    const uint32_t e = (uint32_t)t[2 * ps.cwords + ei];
-    if (!((e >> (x & 31)) & 1u)) return -1;
-    return t[2 * ps.cwords + ps.ewords + ei] + __popc(e & ((1u << (x & 31)) - 1));
+    const uint2 e = reinterpret_cast<const uint2 *>(t + 2 * ps.cwords)[ei];
+    if (!((e.x >> (x & 31)) & 1u)) return -1;
+    return (int32_t)(e.y + (uint32_t)__popc(e.x & ((1u << (x & 31)) - 1)));
   }
   const uint32_t mask = (1u << ps.log_t) - 1;
   uint32_t h = hslot(w, ps.log_t);
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(WAVES * kWave)
                      int32_t *__restrict__ cursor, uint32_t *__restrict__ row_scratch,
                      int64_t scratch_words, HomTotals *tot, int32_t *__restrict__ unsupported) {
   constexpr int kBlock = WAVES * kWave;
-  __shared__ int32_t pool[POOL];
+  __shared__ __align__(8) int32_t pool[POOL];
   __shared__ int32_t s_item;
   int32_t *const set = pool;
   int32_t *const colcnt = pool + kSetWords;
@@ -340,25 +342,43 @@ __global__ void __launch_bounds__(WAVES * kWave)
     // instruction, against 81.3 ms here, where a wave's lanes hit one row. Two rows streamed by
     // one wave as a single run, which fills the lanes a short row leaves idle, changed nothing:
     // 78.9 against 77.5 ms on R-MAT 22, 14.9 against 15.0 ms on R-MAT 20.)
-    for (int32_t i = wib; i < d; i += WAVES) {
-      const int32_t v = L[i];
-      if ((uint32_t)v >= (uint32_t)m) continue;
-      const int64_t vb = up_start[v];
-      const int32_t dv = up_len[v];
-      for (int32_t p0 = lane; p0 < dv; p0 += kStreamUnroll * kWave) {
-        int32_t w[kStreamUnroll];
-#pragma unroll
-        for (int k = 0; k < kStreamUnroll; ++k) {
-          const int32_t p = p0 + k * kWave;
-          w[k] = p < dv ? col[vb + p] : -1;  // -1 lies outside every set
+    // The headers (start, length of N+(v)) of a wave's next 64 rows are loaded lane-parallel,
+    // one row a lane, and handed out by shuffles, so a row's loads wait on one round trip to
+    // memory instead of three dependent ones (R-MAT 22 K4 + diamond 77.7 -> 72.6 ms).
+    for (int32_t r0 = wib; r0 < d; r0 += WAVES * kWave) {
+      const int32_t ih = r0 + lane * WAVES;
+      int64_t hb = 0;
+      int32_t hl = 0;
+      if (ih < d) {
+        const int32_t v = L[ih];
+        if ((uint32_t)v < (uint32_t)m) {
+          hb = up_start[v];
+          hl = up_len[v];
         }
+      }
+      const int32_t nr = min(kWave, (d - r0 + WAVES - 1) / WAVES);
+      for (int32_t q = 0; q < nr; ++q) {
+        const int32_t i = r0 + q * WAVES;
+        const int32_t dv = __shfl(hl, q, kWave);
+        if (dv == 0) continue;
+        const uint32_t vb_hi = (uint32_t)__shfl((int32_t)(hb >> 32), q, kWave);
+        const uint32_t vb_lo = (uint32_t)__shfl((int32_t)hb, q, kWave);
+        const int64_t vb = (int64_t)(((uint64_t)vb_hi << 32) | vb_lo);
+        for (int32_t p0 = lane; p0 < dv; p0 += kStreamUnroll * kWave) {
+          int32_t w[kStreamUnroll];
 #pragma unroll
-        for (int k = 0; k < kStreamUnroll; ++k) {
-          const int32_t j = posset_find(set, ps, w[k]);
-          if (j >= 0) {
-            atomicOr(&rows[(int64_t)i * W2 + (j >> 5)], 1u << (j & 31));
-            atomicAdd(&colcnt[j], 1);
-            atomicAdd(&tcount[vb + p0 + k * kWave], 1u);  // t(v, w) from pivot u
+          for (int k = 0; k < kStreamUnroll; ++k) {
+            const int32_t p = p0 + k * kWave;
+            w[k] = p < dv ? col[vb + p] : -1;  // -1 lies outside every set
+          }
+#pragma unroll
+          for (int k = 0; k < kStreamUnroll; ++k) {
+            const int32_t j = posset_find(set, ps, w[k]);
+            if (j >= 0) {
+              atomicOr(&rows[(int64_t)i * W2 + (j >> 5)], 1u << (j & 31));
+              atomicAdd(&colcnt[j], 1);
+              atomicAdd(&tcount[vb + p0 + k * kWave], 1u);  // t(v, w) from pivot u
+            }
           }
         }
       }
@@ -631,7 +651,7 @@ __global__ void __launch_bounds__(WAVES * kWave)
                    uint32_t *__restrict__ row_scratch, int64_t scratch_words, U128Acc *total,
                    int32_t *__restrict__ unsupported) {
   constexpr int kBlock = WAVES * kWave;
-  __shared__ int32_t pool[POOL];
+  __shared__ __align__(8) int32_t pool[POOL];
   __shared__ uint32_t stage[WAVES][kWave];
   __shared__ int32_t s_item;
   int32_t *const set = pool;
